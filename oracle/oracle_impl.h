@@ -1,0 +1,547 @@
+/*
+ * Precision-generic body of the CPU oracle. Included twice by
+ * homelbm_oracle.c with R = float / double and SFX = f32 / f64.
+ * TEST INFRASTRUCTURE ONLY (see homelbm_oracle.h).
+ */
+#define CAT2(a, b) a##_##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+static void FN(wall_velocity_2d)(const hlo_env* e, int sid, R hx, R hy, R up[2], R r[2]);
+static void FN(wall_velocity_3d)(const hlo_env* e, int sid, R hx, R hy, R hz, R up[3], R r[3]);
+static void FN(d3q27_feq_S)(R rho, R ux, R uy, R uz, R S[6]);
+
+/* ------------------------------------------------------------------ D2Q9 */
+
+/* mlCalDistribution, envs/lbm/lbm_func.py:685-788 (same operation order). */
+static R FN(d2q9_recon)(R rho, R ux, R uy, R Sxx, R Syy, R Sxy, int i)
+{
+    const R one = 1, two = 2, three = 3, half = (R)0.5, c45 = (R)4.5, nine = 9;
+    R Axxy = Sxx * uy + two * Sxy * ux - two * ux * ux * uy;
+    R Axyy = Syy * ux + two * Sxy * uy - two * ux * uy * uy;
+    const R w0 = (R)4 / (R)9, w1 = (R)1 / (R)9, w5 = (R)1 / (R)36;
+    switch (i) {
+    case 0: return w0 * rho * (one - half * three * Sxx - half * three * Syy);
+    case 1: return w1 * rho * (one + three * ux + three * Sxx - half * three * Syy - c45 * Axyy);
+    case 2: return w1 * rho * (one - three * ux + three * Sxx - half * three * Syy + c45 * Axyy);
+    case 3: return w1 * rho * (one + three * uy - half * three * Sxx + three * Syy - c45 * Axxy);
+    case 4: return w1 * rho * (one - three * uy - half * three * Sxx + three * Syy + c45 * Axxy);
+    case 5: return w5 * rho * (one + three * (ux + uy) + three * (Sxx + Syy + three * Sxy) + nine * (Axyy + Axxy));
+    case 6: return w5 * rho * (one - three * ux + three * uy + three * (Sxx + Syy - three * Sxy) + nine * (Axxy - Axyy));
+    case 7: return w5 * rho * (one - three * ux - three * uy + three * (Sxx + Syy + three * Sxy) - nine * (Axxy + Axyy));
+    default: return w5 * rho * (one + three * ux - three * uy + three * (Sxx + Syy - three * Sxy) - nine * (Axxy - Axyy));
+    }
+}
+
+/* Equilibrium second moment (S part) of a D2Q9 cell: f_eq_home,
+ * envs/lbm/lbm_func.py:660-682 (populations clamped at 0). */
+static void FN(d2q9_feq_S)(R rho, R ux, R uy, R S[3])
+{
+    R pop[9];
+    R U_sqr = ux * ux + uy * uy;
+    for (int i = 0; i < 9; ++i) {
+        R cu = (R)D2_CX[i] * ux + (R)D2_CY[i] * uy;
+        pop[i] = rho * FN(d2_w)[i] * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * U_sqr);
+        if (pop[i] < (R)0) pop[i] = 0;
+    }
+    R invRho = (R)1 / rho;
+    R pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+    R piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    R pixy = pop[5] + pop[7] - pop[6] - pop[8];
+    S[0] = pixx * invRho - FN(cs2_2d);
+    S[1] = piyy * invRho - FN(cs2_2d);
+    S[2] = pixy * invRho;
+}
+
+/* Post-collision moments of interior cell (x, y): stream_and_collide,
+ * envs/lbm/lbm_func.py:322-467, with the cut-link test replaced by the mask
+ * rule occ(x) != occ(x - c_i) and the wall point at the link midpoint. */
+static void FN(d2q9_cell)(const hlo_env* e, const R* src, int x, int y, R out[6])
+{
+    const int64_t st = e->stride;
+    const int nx = e->nx;
+    const int64_t c = (int64_t)x + (int64_t)nx * y;
+    R pop[9];
+    R rho_c = src[c], ux_c = src[st + c], uy_c = src[2 * st + c];
+    R Sxx_c = src[3 * st + c], Syy_c = src[4 * st + c], Sxy_c = src[5 * st + c];
+    int occ_c = e->mask ? (e->mask[c] != 0) : 0;
+    for (int i = 0; i < 9; ++i) {
+        int x1 = x - D2_CX[i], y1 = y - D2_CY[i];
+        int64_t s = (int64_t)x1 + (int64_t)nx * y1;
+        int cut = 0, sid = -1;
+        if (e->mask) {
+            int occ_s = e->mask[s] != 0;
+            if (occ_s != occ_c) {
+                cut = 1;
+                sid = occ_c ? e->mask[c] - 1 : e->mask[s] - 1;
+            }
+        }
+        if (cut) {
+            R up[2];
+            FN(wall_velocity_2d)(e, sid, (R)x - (R)0.5 * (R)D2_CX[i], (R)y - (R)0.5 * (R)D2_CY[i], up, NULL);
+            R Sxx_s = up[0] * up[0] + Sxx_c - ux_c * ux_c;
+            R Syy_s = up[1] * up[1] + Syy_c - uy_c * uy_c;
+            R Sxy_s = up[0] * up[1] + Sxy_c - ux_c * uy_c;
+            pop[i] = FN(d2q9_recon)(rho_c, up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+            if (pop[i] < (R)0) pop[i] = 0;
+        } else {
+            pop[i] = FN(d2q9_recon)(src[s], src[st + s], src[2 * st + s], src[3 * st + s],
+                                    src[4 * st + s], src[5 * st + s], i);
+            if (pop[i] < (R)0) pop[i] = 0;
+        }
+    }
+    R Fx = 0, Fy = 0;
+    if (e->force) { Fx = (R)e->force[c]; Fy = (R)e->force[st + c]; }
+    R rhoVar = pop[0] + pop[1] + pop[2] + pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    R invRho = (R)1 / rhoVar;
+    R ux = (pop[1] - pop[2] + pop[5] - pop[6] - pop[7] + pop[8] + (R)0.5 * Fx) * invRho;
+    R uy = (pop[3] - pop[4] + pop[5] + pop[6] - pop[7] - pop[8] + (R)0.5 * Fy) * invRho;
+    R pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+    R piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    R pixy = pop[5] + pop[7] - pop[6] - pop[8];
+    R Omega = (R)1 / (((R)e->nu) * (R)3 + (R)0.5);
+    R Ru2 = rhoVar * ux * ux;
+    R Rv2 = rhoVar * uy * uy;
+    R pixx_part = (pixx - piyy) / (R)2;
+    R piyy_part = (piyy - pixx) / (R)2;
+    R one = 1;
+    pixy = (one - Omega) * pixy + ux * uy * rhoVar * Omega + (one - (R)0.5 * Omega) * (Fy * ux + Fx * uy);
+    R pixx_n = rhoVar / (R)3 + pixx_part * (one - Omega) + (one + Omega) / (R)2 * Ru2
+             + (one - Omega) / (R)2 * Rv2 + Fx * ux + (one - Omega) / (R)2 * (Fx * ux - Fy * uy);
+    R piyy_n = rhoVar / (R)3 + piyy_part * (one - Omega) + (one + Omega) / (R)2 * Rv2
+             + (one - Omega) / (R)2 * Ru2 + Fy * uy + (one - Omega) / (R)2 * (Fy * uy - Fx * ux);
+    out[0] = rhoVar;
+    out[1] = ux + Fx * invRho / (R)2;
+    out[2] = uy + Fy * invRho / (R)2;
+    out[3] = pixx_n * invRho - FN(cs2_2d);
+    out[4] = piyy_n * invRho - FN(cs2_2d);
+    out[5] = pixy * invRho;
+}
+
+/* One boundary cell b from its inner neighbour n: apply_bc_core,
+ * envs/lbm/lbm_func.py:634-657 (2D) and envs/lbm3d/lbm_func_3d.py:627-648 (3D).
+ * Type 0: velocity, type 1: zero gradient (reference). Type 2 (free slip,
+ * no reference counterpart): u_b = u_n with the face-normal component zeroed. */
+static void FN(bc_apply)(const hlo_env* e, R* dst, int64_t b, int64_t n, int face, int axis)
+{
+    const int64_t st = e->stride;
+    const int d3 = e->lattice == 27;
+    const int nu_ = d3 ? 3 : 2;
+    int type = e->bc_type[face];
+    R u_b[3], u_n[3];
+    for (int a = 0; a < nu_; ++a) u_n[a] = dst[(1 + a) * st + n];
+    for (int a = 0; a < nu_; ++a) u_b[a] = dst[(1 + a) * st + b];
+    if (type == 0) {
+        for (int a = 0; a < nu_; ++a) u_b[a] = (R)e->bc_value[face][a];
+    } else if (type == 1) {
+        for (int a = 0; a < nu_; ++a) u_b[a] = u_n[a];
+    } else if (type == 2) {
+        for (int a = 0; a < nu_; ++a) u_b[a] = (a == axis) ? (R)0 : u_n[a];
+    }
+    for (int a = 0; a < nu_; ++a) dst[(1 + a) * st + b] = u_b[a];
+    R rho_n = dst[n];
+    dst[b] = rho_n;
+    if (!d3) {
+        R Sb[3], Sn[3];
+        FN(d2q9_feq_S)(rho_n, u_b[0], u_b[1], Sb);
+        FN(d2q9_feq_S)(rho_n, u_n[0], u_n[1], Sn);
+        for (int k = 0; k < 3; ++k)
+            dst[(3 + k) * st + b] = Sb[k] - Sn[k] + dst[(3 + k) * st + n];
+    } else {
+        /* f_eq_home_3d returns (xx, xy, xz, yy, yz, zz); storage is (xx, yy, zz, xy, xz, yz). */
+        R Sb[6], Sn[6];
+        FN(d3q27_feq_S)(rho_n, u_b[0], u_b[1], u_b[2], Sb);
+        FN(d3q27_feq_S)(rho_n, u_n[0], u_n[1], u_n[2], Sn);
+        static const int slot[6] = {4, 7, 8, 5, 9, 6};
+        for (int k = 0; k < 6; ++k)
+            dst[slot[k] * st + b] = Sb[k] - Sn[k] + dst[slot[k] * st + n];
+    }
+}
+
+/* Boundary passes in a fixed order: x faces, then y faces, then z faces.
+ * The reference's apply_bc / apply_bc_3d run all faces in one launch and
+ * race at edges and corners (lbm_func.py:616-631, lbm_func_3d.py:613-624);
+ * this sequential order is one of the outcomes that launch can produce. */
+static void FN(bc_passes)(const hlo_env* e, R* dst)
+{
+    const int nx = e->nx, ny = e->ny, nz = e->nz;
+    const int d3 = e->lattice == 27;
+    /* face indices: 2D left 0, top 1, right 2, bottom 3;
+     *               3D left 0, right 1, top 2, bottom 3, front 4, back 5. */
+    const int f_left = 0, f_right = d3 ? 1 : 2, f_top = d3 ? 2 : 1, f_bottom = 3;
+#define IDX(X, Y, Z) ((int64_t)(X) + (int64_t)nx * ((int64_t)(Y) + (int64_t)ny * (Z)))
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            FN(bc_apply)(e, dst, IDX(0, y, z), IDX(1, y, z), f_left, 0);
+            FN(bc_apply)(e, dst, IDX(nx - 1, y, z), IDX(nx - 2, y, z), f_right, 0);
+        }
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            FN(bc_apply)(e, dst, IDX(x, ny - 1, z), IDX(x, ny - 2, z), f_top, 1);
+            FN(bc_apply)(e, dst, IDX(x, 0, z), IDX(x, 1, z), f_bottom, 1);
+        }
+    if (d3) {
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                FN(bc_apply)(e, dst, IDX(x, y, nz - 1), IDX(x, y, nz - 2), 4, 2);
+                FN(bc_apply)(e, dst, IDX(x, y, 0), IDX(x, y, 1), 5, 2);
+            }
+    }
+#undef IDX
+}
+
+/* ----------------------------------------------------------------- D3Q27 */
+
+/* mlCalEquilibrium3D_s, envs/lbm3d/lbm_func_3d.py:224-251 (argument order
+ * rho, ux, uy, uz, Sxx, Sxy, Sxz, Syy, Syz, Szz as in the reference). */
+static R FN(d3q27_recon_s)(R rho, R ux, R uy, R uz, R Sxx, R Sxy, R Sxz, R Syy, R Syz, R Szz,
+                           R dx, R dy, R dz, R wi)
+{
+    R cu = dx * ux + dy * uy + dz * uz;
+    R U_sqr = ux * ux + uy * uy + uz * uz;
+    R third = FN(third);
+    R Qxx = dx * dx - third, Qyy = dy * dy - third, Qzz = dz * dz - third;
+    R Qxy = dx * dy, Qxz = dx * dz, Qyz = dy * dz;
+    R f_neq = (R)1.5 * (Qxx * Sxx + Qyy * Syy + Qzz * Szz + (R)2 * (Qxy * Sxy + Qxz * Sxz + Qyz * Syz));
+    return wi * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * U_sqr + f_neq);
+}
+
+/* Equilibrium S of a D3Q27 cell: f_eq_home_3d, envs/lbm3d/lbm_func_3d.py:651-686
+ * (no clamp). S = (xx, xy, xz, yy, yz, zz). */
+static void FN(d3q27_feq_S)(R rho, R ux, R uy, R uz, R S[6])
+{
+    R p[27];
+    R U_sqr = ux * ux + uy * uy + uz * uz;
+    for (int i = 0; i < 27; ++i) {
+        R cu = (R)D3_CX[i] * ux + (R)D3_CY[i] * uy + (R)D3_CZ[i] * uz;
+        p[i] = FN(d3_w)[i] * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * U_sqr);
+    }
+    R invRho = (R)1 / rho;
+    R pixx = p[1] + p[2] + p[7] + p[8] + p[9] + p[10] + p[13] + p[14] + p[15] + p[16] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    R pixy = (p[7] + p[8] + p[19] + p[20] + p[21] + p[22]) - (p[13] + p[14] + p[23] + p[24] + p[25] + p[26]);
+    R pixz = (p[9] + p[10] + p[19] + p[20] + p[23] + p[24]) - (p[15] + p[16] + p[21] + p[22] + p[25] + p[26]);
+    R piyy = p[3] + p[4] + p[7] + p[8] + p[11] + p[12] + p[13] + p[14] + p[17] + p[18] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    R piyz = (p[11] + p[12] + p[19] + p[20] + p[25] + p[26]) - (p[17] + p[18] + p[21] + p[22] + p[23] + p[24]);
+    R pizz = p[5] + p[6] + p[9] + p[10] + p[11] + p[12] + p[15] + p[16] + p[17] + p[18] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    S[0] = pixx * invRho - FN(cs2_3d);
+    S[1] = pixy * invRho;
+    S[2] = pixz * invRho;
+    S[3] = piyy * invRho - FN(cs2_3d);
+    S[4] = piyz * invRho;
+    S[5] = pizz * invRho - FN(cs2_3d);
+}
+
+/* Post-collision moments of interior cell (x, y, z): stream_and_collide_3d,
+ * envs/lbm3d/lbm_func_3d.py:364-604. Cut links use the mask rule and the link
+ * midpoint; the cut branch is not clamped (lbm_func_3d.py:487-489), the plain
+ * branch is (525-526); moments are accumulated by sign group in ascending i
+ * (528-562). */
+static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, R out[10])
+{
+    const int64_t st = e->stride;
+    const int nx = e->nx, ny = e->ny;
+    const int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+    R rho_c = src[c], ux_c = src[st + c], uy_c = src[2 * st + c], uz_c = src[3 * st + c];
+    R Sxx_c = src[4 * st + c], Syy_c = src[5 * st + c], Szz_c = src[6 * st + c];
+    R Sxy_c = src[7 * st + c], Sxz_c = src[8 * st + c], Syz_c = src[9 * st + c];
+    int occ_c = e->mask ? (e->mask[c] != 0) : 0;
+    R rho_acc = 0, uxp = 0, uxm = 0, uyp = 0, uym = 0, uzp = 0, uzm = 0;
+    R pixx = 0, piyy = 0, pizz = 0, pixy_p = 0, pixy_m = 0, pixz_p = 0, pixz_m = 0, piyz_p = 0, piyz_m = 0;
+    for (int i = 0; i < 27; ++i) {
+        R dx = (R)D3_CX[i], dy = (R)D3_CY[i], dz = (R)D3_CZ[i], wi = FN(d3_w)[i];
+        int x1 = x - D3_CX[i], y1 = y - D3_CY[i], z1 = z - D3_CZ[i];
+        int64_t s = (int64_t)x1 + (int64_t)nx * ((int64_t)y1 + (int64_t)ny * z1);
+        int cut = 0, sid = -1;
+        if (e->mask) {
+            int occ_s = e->mask[s] != 0;
+            if (occ_s != occ_c) {
+                cut = 1;
+                sid = occ_c ? e->mask[c] - 1 : e->mask[s] - 1;
+            }
+        }
+        R pop_i;
+        if (cut) {
+            R up[3];
+            FN(wall_velocity_3d)(e, sid, (R)x - (R)0.5 * dx, (R)y - (R)0.5 * dy, (R)z - (R)0.5 * dz, up, NULL);
+            R Sxx_s = up[0] * up[0] + Sxx_c - ux_c * ux_c;
+            R Sxy_s = up[0] * up[1] + Sxy_c - ux_c * uy_c;
+            R Sxz_s = up[0] * up[2] + Sxz_c - ux_c * uz_c;
+            R Syy_s = up[1] * up[1] + Syy_c - uy_c * uy_c;
+            R Syz_s = up[1] * up[2] + Syz_c - uy_c * uz_c;
+            R Szz_s = up[2] * up[2] + Szz_c - uz_c * uz_c;
+            pop_i = FN(d3q27_recon_s)(rho_c, up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s,
+                                      dx, dy, dz, wi);
+        } else {
+            pop_i = FN(d3q27_recon_s)(src[s], src[st + s], src[2 * st + s], src[3 * st + s],
+                                      src[4 * st + s], src[7 * st + s], src[8 * st + s],
+                                      src[5 * st + s], src[9 * st + s], src[6 * st + s], dx, dy, dz, wi);
+            if (pop_i < (R)0) pop_i = 0;
+        }
+        rho_acc = rho_acc + pop_i;
+        if (dx > 0) { uxp = uxp + pop_i; pixx = pixx + pop_i; }
+        else if (dx < 0) { uxm = uxm + pop_i; pixx = pixx + pop_i; }
+        if (dy > 0) { uyp = uyp + pop_i; piyy = piyy + pop_i; }
+        else if (dy < 0) { uym = uym + pop_i; piyy = piyy + pop_i; }
+        if (dz > 0) { uzp = uzp + pop_i; pizz = pizz + pop_i; }
+        else if (dz < 0) { uzm = uzm + pop_i; pizz = pizz + pop_i; }
+        R dxy = dx * dy, dxz = dx * dz, dyz = dy * dz;
+        if (dxy > 0) pixy_p = pixy_p + pop_i; else if (dxy < 0) pixy_m = pixy_m + pop_i;
+        if (dxz > 0) pixz_p = pixz_p + pop_i; else if (dxz < 0) pixz_m = pixz_m + pop_i;
+        if (dyz > 0) piyz_p = piyz_p + pop_i; else if (dyz < 0) piyz_m = piyz_m + pop_i;
+    }
+    R Fx = 0, Fy = 0, Fz = 0;
+    if (e->force) { Fx = (R)e->force[c]; Fy = (R)e->force[st + c]; Fz = (R)e->force[2 * st + c]; }
+    R rhoVar = rho_acc;
+    R invRho = (R)1 / rhoVar;
+    R ux = (uxp - uxm + (R)0.5 * Fx) * invRho;
+    R uy = (uyp - uym + (R)0.5 * Fy) * invRho;
+    R uz = (uzp - uzm + (R)0.5 * Fz) * invRho;
+    R pixy = pixy_p - pixy_m, pixz = pixz_p - pixz_m, piyz = piyz_p - piyz_m;
+    R Omega = (R)1 / (((R)e->nu) * (R)3 + (R)0.5);
+    R one = 1, two = 2, three = 3;
+    R pixx_part = (two * pixx - piyy - pizz) / three;
+    R piyy_part = (two * piyy - pixx - pizz) / three;
+    R pizz_part = (two * pizz - pixx - piyy) / three;
+    R RU2 = rhoVar * ux * ux, RV2 = rhoVar * uy * uy, RW2 = rhoVar * uz * uz;
+    R RUVW2 = (RU2 + RV2 + RW2) / three;
+    R pixx_n = rhoVar / three + pixx_part * (one - Omega) + RUVW2 + (two * RU2 * Omega) / three
+             - (RV2 * Omega) / three - (RW2 * Omega) / three + Fx * ux;
+    R piyy_n = rhoVar / three + piyy_part * (one - Omega) + RUVW2 - (RU2 * Omega) / three
+             + (two * RV2 * Omega) / three - (RW2 * Omega) / three + Fy * uy;
+    R pizz_n = rhoVar / three + pizz_part * (one - Omega) + RUVW2 - (RU2 * Omega) / three
+             - (RV2 * Omega) / three + (two * RW2 * Omega) / three + Fz * uz;
+    R pixy_n = pixy - pixy * Omega + ux * uy * rhoVar * Omega + (Fy * ux + Fx * uy) / two;
+    R pixz_n = pixz - pixz * Omega + ux * uz * rhoVar * Omega + (Fz * ux + Fx * uz) / two;
+    R piyz_n = piyz - piyz * Omega + uy * uz * rhoVar * Omega + (Fz * uy + Fy * uz) / two;
+    out[0] = rhoVar;
+    out[1] = ux + Fx * invRho / two;
+    out[2] = uy + Fy * invRho / two;
+    out[3] = uz + Fz * invRho / two;
+    out[4] = pixx_n * invRho - FN(cs2_3d);
+    out[5] = piyy_n * invRho - FN(cs2_3d);
+    out[6] = pizz_n * invRho - FN(cs2_3d);
+    out[7] = pixy_n * invRho;
+    out[8] = pixz_n * invRho;
+    out[9] = piyz_n * invRho;
+}
+
+/* --------------------------------------------------------- wall velocity */
+
+/* Velocity of the body material point at h (time step 1): the point's
+ * current position minus the position the same body-frame point had one
+ * step earlier. 2D follows the material-point difference of
+ * envs/lbm/lbm_func.py:354-370; 3D follows lbm_func_3d.py:456-463.
+ * Pose table 2D: px, py, cos, sin | last: px, py, cos, sin.
+ * Pose table 3D: p[3], R[9] row-major | last: p[3], R[9]. */
+static void FN(wall_velocity_2d)(const hlo_env* e, int sid, R hx, R hy, R up[2], R r[2])
+{
+    if (sid < 0 || sid >= e->nsolid) { up[0] = up[1] = 0; if (r) { r[0] = r[1] = 0; } return; }
+    const float* P = e->solids + 8 * sid;
+    R px = P[0], py = P[1], cn = P[2], sn = P[3];
+    R pxl = P[4], pyl = P[5], cl = P[6], sl = P[7];
+    R dx = hx - px, dy = hy - py;
+    R qx = cn * dx + sn * dy;
+    R qy = cn * dy - sn * dx;
+    R cx = cn * qx - sn * qy + px;
+    R cy = sn * qx + cn * qy + py;
+    R lx = cl * qx - sl * qy + pxl;
+    R ly = sl * qx + cl * qy + pyl;
+    up[0] = cx - lx;
+    up[1] = cy - ly;
+    if (r) { r[0] = hx - px; r[1] = hy - py; }
+}
+
+static void FN(wall_velocity_3d)(const hlo_env* e, int sid, R hx, R hy, R hz, R up[3], R r[3])
+{
+    if (sid < 0 || sid >= e->nsolid) {
+        up[0] = up[1] = up[2] = 0;
+        if (r) { r[0] = r[1] = r[2] = 0; }
+        return;
+    }
+    const float* P = e->solids + 24 * sid;
+    const float* M = P + 3;
+    const float* L = P + 12;
+    const float* ML = P + 15;
+    R d0 = hx - (R)P[0], d1 = hy - (R)P[1], d2 = hz - (R)P[2];
+    /* q = R^T (h - p) */
+    R q0 = (R)M[0] * d0 + (R)M[3] * d1 + (R)M[6] * d2;
+    R q1 = (R)M[1] * d0 + (R)M[4] * d1 + (R)M[7] * d2;
+    R q2 = (R)M[2] * d0 + (R)M[5] * d1 + (R)M[8] * d2;
+    R c0 = (R)M[0] * q0 + (R)M[1] * q1 + (R)M[2] * q2 + (R)P[0];
+    R c1 = (R)M[3] * q0 + (R)M[4] * q1 + (R)M[5] * q2 + (R)P[1];
+    R c2 = (R)M[6] * q0 + (R)M[7] * q1 + (R)M[8] * q2 + (R)P[2];
+    R l0 = (R)ML[0] * q0 + (R)ML[1] * q1 + (R)ML[2] * q2 + (R)L[0];
+    R l1 = (R)ML[3] * q0 + (R)ML[4] * q1 + (R)ML[5] * q2 + (R)L[1];
+    R l2 = (R)ML[6] * q0 + (R)ML[7] * q1 + (R)ML[8] * q2 + (R)L[2];
+    up[0] = c0 - l0; up[1] = c1 - l1; up[2] = c2 - l2;
+    if (r) { r[0] = d0; r[1] = d1; r[2] = d2; }
+}
+
+/* ------------------------------------------------------------ public API */
+
+void FN(hlo_step)(const hlo_env* e, const R* src, R* dst)
+{
+    const int64_t st = e->stride;
+    if (e->lattice == 9) {
+        for (int y = 1; y < e->ny - 1; ++y)
+            for (int x = 1; x < e->nx - 1; ++x) {
+                R out[6];
+                FN(d2q9_cell)(e, src, x, y, out);
+                int64_t c = (int64_t)x + (int64_t)e->nx * y;
+                for (int f = 0; f < 6; ++f) dst[f * st + c] = out[f];
+            }
+    } else {
+        for (int z = 1; z < e->nz - 1; ++z)
+            for (int y = 1; y < e->ny - 1; ++y)
+                for (int x = 1; x < e->nx - 1; ++x) {
+                    R out[10];
+                    FN(d3q27_cell)(e, src, x, y, z, out);
+                    int64_t c = (int64_t)x + (int64_t)e->nx * ((int64_t)y + (int64_t)e->ny * z);
+                    for (int f = 0; f < 10; ++f) dst[f * st + c] = out[f];
+                }
+    }
+    FN(bc_passes)(e, dst);
+}
+
+/* Momentum exchange per cut link, accumulated in FP64 in cell order
+ * (z, y, x) and ascending direction: lbm_func.py:383-401 (2D, torque scaled
+ * by grid_length = 1/nx) and lbm_func_3d.py:491-511 (3D). */
+void FN(hlo_forces)(const hlo_env* e, const R* src, double* out)
+{
+    const int64_t st = e->stride;
+    const int nx = e->nx, ny = e->ny, nz = e->nz;
+    const int d3 = e->lattice == 27;
+    for (int k = 0; k < 6 * e->nsolid; ++k) out[k] = 0.0;
+    if (!e->mask) return;
+    const int q = d3 ? 27 : 9;
+    const R grid_length = (R)1 / (R)nx;
+    for (int z = d3 ? 1 : 0; z < (d3 ? nz - 1 : 1); ++z)
+        for (int y = 1; y < ny - 1; ++y)
+            for (int x = 1; x < nx - 1; ++x) {
+                int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+                int occ_c = e->mask[c] != 0;
+                for (int i = 0; i < q; ++i) {
+                    int cx = d3 ? D3_CX[i] : D2_CX[i];
+                    int cy = d3 ? D3_CY[i] : D2_CY[i];
+                    int cz = d3 ? D3_CZ[i] : 0;
+                    int64_t s = (int64_t)(x - cx) + (int64_t)nx * ((int64_t)(y - cy) + (int64_t)ny * (z - cz));
+                    int occ_s = e->mask[s] != 0;
+                    if (occ_s == occ_c) continue;
+                    int sid = occ_c ? e->mask[c] - 1 : e->mask[s] - 1;
+                    if (sid < 0 || sid >= e->nsolid) continue;
+                    double* o = out + 6 * sid;
+                    if (!d3) {
+                        R rho = src[c], ux = src[st + c], uy = src[2 * st + c];
+                        R Sxx = src[3 * st + c], Syy = src[4 * st + c], Sxy = src[5 * st + c];
+                        R up[2], r[2];
+                        FN(wall_velocity_2d)(e, sid, (R)x - (R)0.5 * (R)cx, (R)y - (R)0.5 * (R)cy, up, r);
+                        R Sxx_s = up[0] * up[0] + Sxx - ux * ux;
+                        R Syy_s = up[1] * up[1] + Syy - uy * uy;
+                        R Sxy_s = up[0] * up[1] + Sxy - ux * uy;
+                        R pop = FN(d2q9_recon)(rho, up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+                        if (pop < (R)0) pop = 0;
+                        int inv = D2_INV[i];
+                        R cur_f = FN(d2q9_recon)(rho, ux, uy, Sxx, Syy, Sxy, inv);
+                        R Fx = cur_f * ((R)D2_CX[inv] - up[0]) - pop * ((R)cx - up[0]);
+                        R Fy = cur_f * ((R)D2_CY[inv] - up[1]) - pop * ((R)cy - up[1]);
+                        R Tz = (r[0] * Fy - r[1] * Fx) * grid_length;
+                        o[0] += (double)Fx; o[1] += (double)Fy; o[5] += (double)Tz;
+                    } else {
+                        R rho = src[c], ux = src[st + c], uy = src[2 * st + c], uz = src[3 * st + c];
+                        R Sxx = src[4 * st + c], Syy = src[5 * st + c], Szz = src[6 * st + c];
+                        R Sxy = src[7 * st + c], Sxz = src[8 * st + c], Syz = src[9 * st + c];
+                        R dx = (R)cx, dy = (R)cy, dz = (R)cz;
+                        R up[3], r[3];
+                        FN(wall_velocity_3d)(e, sid, (R)x - (R)0.5 * dx, (R)y - (R)0.5 * dy, (R)z - (R)0.5 * dz, up, r);
+                        R Sxx_s = up[0] * up[0] + Sxx - ux * ux;
+                        R Sxy_s = up[0] * up[1] + Sxy - ux * uy;
+                        R Sxz_s = up[0] * up[2] + Sxz - ux * uz;
+                        R Syy_s = up[1] * up[1] + Syy - uy * uy;
+                        R Syz_s = up[1] * up[2] + Syz - uy * uz;
+                        R Szz_s = up[2] * up[2] + Szz - uz * uz;
+                        R pop = FN(d3q27_recon_s)(rho, up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s,
+                                                  dx, dy, dz, FN(d3_w)[i]);
+                        int inv = D3_INV[i];
+                        R ex = (R)D3_CX[inv], ey = (R)D3_CY[inv], ez = (R)D3_CZ[inv];
+                        R cur_f = FN(d3q27_recon_s)(rho, ux, uy, uz, Sxx, Sxy, Sxz, Syy, Syz, Szz, ex, ey, ez,
+                                                    FN(d3_w)[inv]);
+                        R Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
+                        R Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
+                        R Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
+                        R Tx = r[1] * Fz - r[2] * Fy;
+                        R Ty = r[2] * Fx - r[0] * Fz;
+                        R Tz = r[0] * Fy - r[1] * Fx;
+                        o[0] += (double)Fx; o[1] += (double)Fy; o[2] += (double)Fz;
+                        o[3] += (double)Tx; o[4] += (double)Ty; o[5] += (double)Tz;
+                    }
+                }
+            }
+}
+
+/* Uniform-flow initialisation: set_uniform_flow_kernel,
+ * envs/lbm/karman/karman_lbm_env.py:15-53, and set_uniform_flow_3d_kernel,
+ * envs/lbm3d/karman/karman_lbm_env_3d.py:15-64. With u = 0 this is also
+ * InitFlow / InitFlow3D (lbm_func.py:26-63, lbm_func_3d.py:22-83). The
+ * diagonal S subtracts the raw pixx (cs2_local = pixx), so Sxx = 0 and
+ * Syy = piyy - pixx at rho = 1. */
+void FN(hlo_init_uniform)(const hlo_env* e, R ux, R uy, R uz, R* dst)
+{
+    const int64_t st = e->stride;
+    const int64_t n = (int64_t)e->nx * e->ny * e->nz;
+    R m[10];
+    R rho = 1;
+    if (e->lattice == 9) {
+        R pop[9];
+        R U2 = ux * ux + uy * uy;
+        for (int i = 0; i < 9; ++i) {
+            R cu = (R)D2_CX[i] * ux + (R)D2_CY[i] * uy;
+            pop[i] = rho * FN(d2_w)[i] * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * U2);
+        }
+        R inv = (R)1 / rho;
+        R pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+        R piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+        R pixy = pop[5] + pop[7] - pop[6] - pop[8];
+        R cs2_local = pixx;
+        m[0] = rho; m[1] = ux; m[2] = uy;
+        m[3] = pixx * inv - cs2_local; m[4] = piyy * inv - cs2_local; m[5] = pixy * inv;
+        for (int f = 0; f < 6; ++f)
+            for (int64_t c = 0; c < n; ++c) dst[f * st + c] = m[f];
+    } else {
+        R p[27];
+        R U2 = ux * ux + uy * uy + uz * uz;
+        for (int i = 0; i < 27; ++i) {
+            R cu = (R)D3_CX[i] * ux + (R)D3_CY[i] * uy + (R)D3_CZ[i] * uz;
+            p[i] = FN(d3_w)[i] * rho * ((R)1 + (R)3 * cu + (R)4.5 * cu * cu - (R)1.5 * U2);
+        }
+        R inv = (R)1 / rho;
+        R pixx = p[1] + p[2] + p[7] + p[8] + p[9] + p[10] + p[13] + p[14] + p[15] + p[16] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        R pixy = (p[7] + p[8] + p[19] + p[20] + p[21] + p[22]) - (p[13] + p[14] + p[23] + p[24] + p[25] + p[26]);
+        R pixz = (p[9] + p[10] + p[19] + p[20] + p[23] + p[24]) - (p[15] + p[16] + p[21] + p[22] + p[25] + p[26]);
+        R piyy = p[3] + p[4] + p[7] + p[8] + p[11] + p[12] + p[13] + p[14] + p[17] + p[18] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        R piyz = (p[11] + p[12] + p[19] + p[20] + p[25] + p[26]) - (p[17] + p[18] + p[21] + p[22] + p[23] + p[24]);
+        R pizz = p[5] + p[6] + p[9] + p[10] + p[11] + p[12] + p[15] + p[16] + p[17] + p[18] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        R cs2_local = pixx;
+        m[0] = rho; m[1] = ux; m[2] = uy; m[3] = uz;
+        m[4] = pixx * inv - cs2_local; m[5] = piyy * inv - cs2_local; m[6] = pizz * inv - cs2_local;
+        m[7] = pixy * inv; m[8] = pixz * inv; m[9] = piyz * inv;
+        for (int f = 0; f < 10; ++f)
+            for (int64_t c = 0; c < n; ++c) dst[f * st + c] = m[f];
+    }
+}
+
+R FN(hlo_recon)(int lattice, const R* m, int i)
+{
+    if (lattice == 9) return FN(d2q9_recon)(m[0], m[1], m[2], m[3], m[4], m[5], i);
+    return FN(d3q27_recon_s)(m[0], m[1], m[2], m[3], m[4], m[7], m[8], m[5], m[9], m[6],
+                             (R)D3_CX[i], (R)D3_CY[i], (R)D3_CZ[i], FN(d3_w)[i]);
+}
+
+#undef FN
+#undef CAT
+#undef CAT2
