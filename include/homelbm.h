@@ -1,0 +1,119 @@
+/*
+ * homelbm: MI355X (gfx950) HOME-LBM time step, C ABI.
+ *
+ * Plain C types only: device pointers, sizes and a descriptor struct. Every
+ * entry point returns 0 on success or a negative HL_E_* code; the message of
+ * the last failure on the calling thread is returned by hl_last_error().
+ * Nothing throws across the ABI. Work is enqueued on the caller's HIP stream
+ * (a hipStream_t passed as void*, NULL = default stream) and is asynchronous.
+ *
+ * The reference exposes a Python class API over NVIDIA Warp and has no C
+ * layer; each entry point below names the reference call it replaces. The
+ * Python package openhomelbm_amd binds these with ctypes behind the
+ * reference's LBM_Solver / LBM_Solver3D / HomeFlow / HomeFlow3D API.
+ *
+ * Moment storage (owned by the caller, e.g. PyTorch tensors): two ping-pong
+ * buffers, structure-of-arrays, cell (x, y, z) at x + nx*(y + ny*z), field f
+ * at offset f*stride. Field order:
+ *   D2Q9  : rho, ux, uy, Sxx, Syy, Sxy
+ *   D3Q27 : rho, ux, uy, uz, Sxx, Syy, Szz, Sxy, Sxz, Syz
+ * 16-bit storage holds (rho - 1)*scale[0] and value*scale[f] for f > 0.
+ */
+#ifndef HOMELBM_H
+#define HOMELBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HL_D2Q9 9
+#define HL_D3Q27 27
+
+#define HL_F32 0
+#define HL_F64 1
+#define HL_F16 2
+#define HL_BF16 3
+
+/* Step kernels. */
+#define HL_KERNEL_AUTO 0   /* fastest kernel for the lattice / dtype */
+#define HL_KERNEL_GATHER 1 /* per-cell gather in the reference's operation order */
+#define HL_KERNEL_TILED 2  /* LDS-staged source-population kernel */
+
+#define HL_OK 0
+#define HL_E_INVALID (-1)
+#define HL_E_UNSUPPORTED (-2)
+#define HL_E_HIP (-3)
+#define HL_E_NOMEM (-4)
+
+#define HL_MAX_SOLIDS 64
+
+typedef struct hl_env {
+    int32_t lattice;      /* HL_D2Q9 or HL_D3Q27 */
+    int32_t dtype;        /* HL_F32, HL_F64, HL_F16, HL_BF16 */
+    int32_t nx, ny, nz;   /* nz = 1 for D2Q9; every extent >= 3 */
+    int32_t nsolid;       /* solids in the pose table (<= HL_MAX_SOLIDS) */
+    int64_t stride;       /* elements between moment fields (>= nx*ny*nz) */
+    void* moments[2];     /* ping-pong moment buffers */
+    const uint8_t* mask;  /* nullable; 0 = fluid, k > 0 = solid k-1 */
+    int32_t mask_box[6];  /* x0, x1, y0, y1, z0, z1 (half-open): mask is 0 outside */
+    const float* force;   /* nullable per-cell body force, [dim][stride] FP32 */
+    const float* solids;  /* device pose table: 8 floats/solid (2D), 24 (3D) */
+    float nu;             /* kinematic viscosity (lattice units) */
+    int32_t bc_type[6];   /* 0 velocity, 1 zero gradient, 2 free slip */
+    float bc_value[6][3]; /* face velocity for type 0 */
+    float scale[10];      /* 16-bit storage scale per field */
+} hl_env;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t hl_version(void);
+/* sizeof(hl_env) as compiled, so bindings can check their struct layout. */
+int32_t hl_env_bytes(void);
+/* Message of the last failed call on this thread ("" if none). */
+const char* hl_last_error(void);
+
+/* Advance nsteps time steps. Buffer `cur` (0/1) holds the current state;
+ * afterwards the current state is in buffer cur ^ (nsteps & 1).
+ * Replaces LBM_Solver.step (envs/lbm/lbm_solver.py:97-141) and
+ * LBM_Solver3D.step (envs/lbm3d/lbm_solver_3d.py:88-127): the fused kernel
+ * does stream_and_collide{,_3d} + apply_bc{,_3d}; Swap_Mom{,_3D} becomes the
+ * buffer parity. `kernel` selects HL_KERNEL_*. If force_history is not NULL
+ * (device, [nsteps][nsolid][6]) step k also stores the per-solid force and
+ * torque of the state it reads, as the reference accumulates them inside the
+ * step (lbm_func.py:392-401, lbm_func_3d.py:497-511). */
+int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history,
+                void* stream);
+
+/* Per-solid momentum-exchange force and torque of state `cur`, FP64,
+ * deterministic. out: device [nsolid][6] = Fx, Fy, Fz, Tx, Ty, Tz.
+ * Replaces the atomics of lbm_func.py:392-401 / lbm_func_3d.py:497-511 and the
+ * readback LBM_Solver3D.get_forces_and_torques (lbm_solver_3d.py:308-346). */
+int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream);
+
+/* Uniform flow into buffer `buf`: set_uniform_flow_kernel
+ * (envs/lbm/karman/karman_lbm_env.py:15-53) / set_uniform_flow_3d_kernel
+ * (envs/lbm3d/karman/karman_lbm_env_3d.py:15-64); u = 0 is InitFlow{,3D}
+ * (lbm_func.py:26-63, lbm_func_3d.py:22-83). */
+int32_t hl_init_uniform(const hl_env* env, int32_t buf, float ux, float uy, float uz, void* stream);
+
+/* Capsule voxelizer: writes mask over the half-open box (device caps:
+ * [ncap][8] = ax, ay, az, bx, by, bz, radius, solid_id). No reference
+ * counterpart (the reference tests links against polygon edges,
+ * get_cutcell lbm_func.py:545-607, or a BVH, lbm_func_3d.py:254-316). */
+int32_t hl_voxelize(const hl_env* env, uint8_t* mask, const float* caps, int32_t ncap,
+                    const int32_t box[6], void* stream);
+
+/* Tile shape (TX x TY columns, ZC planes per block) of the tiled kernel for a
+ * lattice; used by the autotuner. Results do not depend on the tile shape.
+ * No reference counterpart (Warp chooses its own launch shape). */
+int32_t hl_set_tiling(int32_t lattice, int32_t tx, int32_t ty, int32_t zc);
+int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc);
+
+/* FP64 sum of rho over all cells of state `cur` into device out[0]. */
+int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,98 @@
+"""ctypes binding of libhomelbm.so (the C ABI in include/homelbm.h).
+
+There is no fallback: if the library is missing or a call fails, this module
+raises. The library is built in-tree by openhomelbm_amd.build_native.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+from . import build_native
+
+HL_D2Q9, HL_D3Q27 = 9, 27
+HL_F32, HL_F64, HL_F16, HL_BF16 = 0, 1, 2, 3
+HL_KERNEL_AUTO, HL_KERNEL_GATHER, HL_KERNEL_TILED = 0, 1, 2
+HL_MAX_SOLIDS = 64
+
+#: every entry point declared in include/homelbm.h
+EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces", "hl_init_uniform",
+           "hl_voxelize", "hl_mass", "hl_set_tiling", "hl_get_tiling")
+
+
+class HLEnv(C.Structure):
+    """Mirror of struct hl_env."""
+
+    _fields_ = [
+        ("lattice", C.c_int32), ("dtype", C.c_int32),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("nsolid", C.c_int32),
+        ("stride", C.c_int64),
+        ("moments", C.c_void_p * 2),
+        ("mask", C.c_void_p),
+        ("mask_box", C.c_int32 * 6),
+        ("force", C.c_void_p),
+        ("solids", C.c_void_p),
+        ("nu", C.c_float),
+        ("bc_type", C.c_int32 * 6),
+        ("bc_value", (C.c_float * 3) * 6),
+        ("scale", C.c_float * 10),
+    ]
+
+
+class HLError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return build_native.LIB
+
+
+def load():
+    """Load libhomelbm.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise HLError(f"{path} is missing: build it with `python -m openhomelbm_amd.build_native` "
+                      "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(path)
+    P, I32 = C.c_void_p, C.c_int32
+    L.hl_version.restype = I32
+    L.hl_env_bytes.restype = I32
+    L.hl_last_error.restype = C.c_char_p
+    env_p = C.POINTER(HLEnv)
+    L.hl_step.argtypes = [env_p, I32, I32, I32, P, P]
+    L.hl_forces.argtypes = [env_p, I32, P, P]
+    L.hl_init_uniform.argtypes = [env_p, I32, C.c_float, C.c_float, C.c_float, P]
+    L.hl_voxelize.argtypes = [env_p, P, P, I32, P, P]
+    L.hl_mass.argtypes = [env_p, I32, P, P]
+    L.hl_set_tiling.argtypes = [I32, I32, I32, I32]
+    L.hl_get_tiling.argtypes = [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
+    for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_mass", "hl_set_tiling",
+                 "hl_get_tiling"):
+        getattr(L, name).restype = I32
+    if L.hl_env_bytes() != C.sizeof(HLEnv):
+        raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = load().hl_last_error().decode(errors="replace")
+        raise HLError(f"{what} failed ({rc}): {msg}")
+
+
+def set_tiling(lattice: int, tx: int, ty: int, zc: int) -> None:
+    check(load().hl_set_tiling(lattice, tx, ty, zc), "hl_set_tiling")
+
+
+def get_tiling(lattice: int):
+    a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+    check(load().hl_get_tiling(lattice, C.byref(a), C.byref(b), C.byref(c)), "hl_get_tiling")
+    return a.value, b.value, c.value

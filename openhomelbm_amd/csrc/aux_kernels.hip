@@ -1,0 +1,365 @@
+// Auxiliary kernels: uniform initialisation, capsule voxelizer, per-solid
+// momentum-exchange force/torque and the FP64 mass sum.
+//
+// Forces and mass are reduced in FP64 in a fixed order (wavefront butterfly,
+// then waves of a block in index order, then blocks in index order), so the
+// result does not depend on scheduling: repeated runs are bitwise equal,
+// unlike the reference's FP32 atomics (lbm_func.py:399-401,
+// lbm_func_3d.py:510-511).
+#pragma clang fp contract(off)
+
+#include "kernels.h"
+#include "lbm_device.h"
+
+namespace hl {
+
+constexpr int kReduceBlock = 256;
+constexpr int kWaves = kReduceBlock / 64;
+constexpr int kMaxReduceBlocks = 1024;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t v)
+{
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
+        hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// ---------------------------------------------------------------- init
+
+template <int DT>
+__global__ void k_fill(void* buf, int64_t stride, int64_t n, int nf, EnvArgs a, double m0, double m1, double m2,
+                       double m3, double m4, double m5, double m6, double m7, double m8, double m9)
+{
+    using ST = typename Storage<DT>::type;
+    ST* p = static_cast<ST*>(buf);
+    const double m[10] = {m0, m1, m2, m3, m4, m5, m6, m7, m8, m9};
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        for (int f = 0; f < nf; ++f) {
+            if (DT == HL_F64) Storage<DT>::template store<double>(p, f * stride + c, m[f], f, a.scale[f]);
+            else Storage<DT>::template store<float>(p, f * stride + c, (float)m[f], f, a.scale[f]);
+        }
+    }
+}
+
+hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, const double m[10], hipStream_t s)
+{
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    const int nf = lattice == HL_D2Q9 ? 6 : 10;
+    const int blocks = (int)((n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096);
+    switch (dtype) {
+    case HL_F32: hipLaunchKernelGGL(k_fill<HL_F32>, blocks, 256, 0, s, buf, a.stride, n, nf, a, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9]); break;
+    case HL_F64: hipLaunchKernelGGL(k_fill<HL_F64>, blocks, 256, 0, s, buf, a.stride, n, nf, a, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9]); break;
+    case HL_F16: hipLaunchKernelGGL(k_fill<HL_F16>, blocks, 256, 0, s, buf, a.stride, n, nf, a, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9]); break;
+    case HL_BF16: hipLaunchKernelGGL(k_fill<HL_BF16>, blocks, 256, 0, s, buf, a.stride, n, nf, a, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], m[9]); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------- voxelizer
+
+// Cell centre p is inside capsule (a, b, r) iff its squared distance to the
+// segment ab is <= r*r; the lowest-index capsule containing p sets
+// mask = solid_id + 1. FP32 without contraction, same order as the CPU
+// voxelizer, so masks are bitwise identical.
+__global__ void k_voxelize(uint8_t* mask, int nx, int ny, const float* caps, int ncap, int bx0, int by0, int bz0,
+                           int bw, int bh, int bd)
+{
+    const int64_t n = (int64_t)bw * bh * bd;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int x = bx0 + (int)(t % bw);
+        const int y = by0 + (int)((t / bw) % bh);
+        const int z = bz0 + (int)(t / ((int64_t)bw * bh));
+        const float px = (float)x, py = (float)y, pz = (float)z;
+        uint8_t v = 0;
+        for (int k = 0; k < ncap; ++k) {
+            const float* C = caps + 8 * k;
+            float abx = C[3] - C[0], aby = C[4] - C[1], abz = C[5] - C[2];
+            float apx = px - C[0], apy = py - C[1], apz = pz - C[2];
+            float len2 = abx * abx + aby * aby + abz * abz;
+            float tt = 0.0f;
+            if (len2 > 0.0f) {
+                tt = (apx * abx + apy * aby + apz * abz) / len2;
+                tt = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
+            }
+            float dx = apx - tt * abx, dy = apy - tt * aby, dz = apz - tt * abz;
+            float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= C[6] * C[6]) { v = (uint8_t)((int)C[7] + 1); break; }
+        }
+        mask[(int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z)] = v;
+    }
+}
+
+hipError_t voxelize(int lattice, const EnvArgs& a, uint8_t* mask, const float* caps, int ncap, const int32_t box[6],
+                    hipStream_t s)
+{
+    const int bw = box[1] - box[0], bh = box[3] - box[2], bd = box[5] - box[4];
+    if (bw <= 0 || bh <= 0 || bd <= 0) return hipSuccess;
+    const int64_t n = (int64_t)bw * bh * bd;
+    const int blocks = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+    hipLaunchKernelGGL(k_voxelize, blocks, 256, 0, s, mask, a.nx, a.ny, caps, ncap, box[0], box[2], box[4], bw, bh, bd);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------------- forces
+
+struct Region {
+    int x0, y0, z0, w, h, d;
+    __host__ __device__ int64_t n() const { return (int64_t)w * h * d; }
+};
+
+// Interior cells within one cell of the mask box: the only cells with cut links.
+static Region force_region(const EnvArgs& a, int lattice)
+{
+    Region r;
+    const int x0 = a.box[0] - 1 > 1 ? a.box[0] - 1 : 1;
+    const int x1 = a.box[1] + 1 < a.nx - 1 ? a.box[1] + 1 : a.nx - 1;
+    const int y0 = a.box[2] - 1 > 1 ? a.box[2] - 1 : 1;
+    const int y1 = a.box[3] + 1 < a.ny - 1 ? a.box[3] + 1 : a.ny - 1;
+    int z0 = 0, z1 = 1;
+    if (lattice == HL_D3Q27) {
+        z0 = a.box[4] - 1 > 1 ? a.box[4] - 1 : 1;
+        z1 = a.box[5] + 1 < a.nz - 1 ? a.box[5] + 1 : a.nz - 1;
+    }
+    r.x0 = x0; r.y0 = y0; r.z0 = z0;
+    r.w = x1 > x0 ? x1 - x0 : 0;
+    r.h = y1 > y0 ? y1 - y0 : 0;
+    r.d = z1 > z0 ? z1 - z0 : 0;
+    return r;
+}
+
+static int reduce_blocks(int64_t n)
+{
+    int64_t b = (n + kReduceBlock - 1) / kReduceBlock;
+    if (b < 1) b = 1;
+    return (int)(b < kMaxReduceBlocks ? b : kMaxReduceBlocks);
+}
+
+size_t forces_workspace_bytes(const EnvArgs& a, int lattice)
+{
+    return (size_t)kMaxReduceBlocks * HL_MAX_SOLIDS * 6 * sizeof(double);
+}
+
+// Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
+// torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
+template <typename T, int DT, int Q>
+__device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x, int y, int z, int i, int sid, T F[6])
+{
+    if (Q == 9) {
+        T up[2], r[2];
+        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+        T pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+        if (pop < T(0)) pop = T(0);
+        const int inv = D2INV[i];
+        T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+        T Fx = cur_f * (T(D2CX[inv]) - up[0]) - pop * (T(D2CX[i]) - up[0]);
+        T Fy = cur_f * (T(D2CY[inv]) - up[1]) - pop * (T(D2CY[i]) - up[1]);
+        T grid_length = T(1) / T(a.nx);
+        F[0] = Fx; F[1] = Fy; F[2] = 0; F[3] = 0; F[4] = 0;
+        F[5] = (r[0] * Fy - r[1] * Fx) * grid_length;
+    } else {
+        const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
+        T up[3], r[3];
+        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+        T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
+        T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
+        T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
+        T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
+        T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
+        T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
+        T pop = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, d3w<T>(i));
+        const int inv = D3INV[i];
+        const T ex = T(D3CX[inv]), ey = T(D3CY[inv]), ez = T(D3CZ[inv]);
+        T cur_f = d3q27_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[7], mc[8], mc[5], mc[9], mc[6], ex, ey, ez,
+                                 d3w<T>(inv));
+        T Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
+        T Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
+        T Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
+        F[0] = Fx; F[1] = Fy; F[2] = Fz;
+        F[3] = r[1] * Fz - r[2] * Fy;
+        F[4] = r[2] * Fx - r[0] * Fz;
+        F[5] = r[0] * Fy - r[1] * Fx;
+    }
+}
+
+template <typename T, int DT, int Q>
+__global__ void __launch_bounds__(kReduceBlock) k_forces(EnvArgs a, Region rg, double* partial)
+{
+    __shared__ double acc[kWaves][HL_MAX_SOLIDS][6];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = threadIdx.x; k < kWaves * HL_MAX_SOLIDS * 6; k += kReduceBlock) (&acc[0][0][0])[k] = 0.0;
+    __syncthreads();
+    using ST = typename Storage<DT>::type;
+    const ST* src = static_cast<const ST*>(a.src);
+    const int64_t n = rg.n();
+    const int nx = a.nx, ny = a.ny;
+    // uniform trip count across the block so every lane takes part in the wave reductions
+    const int64_t per_iter = (int64_t)gridDim.x * kReduceBlock;
+    const int64_t iters = (n + per_iter - 1) / per_iter;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t t = it * per_iter + (int64_t)blockIdx.x * kReduceBlock + threadIdx.x;
+        const bool valid = t < n;
+        int x = 0, y = 0, z = 0;
+        int64_t c = 0;
+        uint64_t mine = 0;
+        int mval_c = 0;
+        if (valid) {
+            x = rg.x0 + (int)(t % rg.w);
+            y = rg.y0 + (int)((t / rg.w) % rg.h);
+            z = rg.z0 + (int)(t / ((int64_t)rg.w * rg.h));
+            c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+            mval_c = a.mask[c];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+                const int mval_s = a.mask[(int64_t)(x - cx) + (int64_t)nx * ((int64_t)(y - cy) + (int64_t)ny * (z - cz))];
+                if ((mval_s != 0) != (mval_c != 0)) {
+                    const int sid = mval_c ? mval_c - 1 : mval_s - 1;
+                    if (sid < a.nsolid) mine |= 1ull << sid;
+                }
+            }
+        }
+        uint64_t uni = wave_or(mine);
+        if (uni == 0) continue;
+        T mc[10];
+        if (mine) {
+#pragma unroll
+            for (int f = 0; f < (Q == 9 ? 6 : 10); ++f) mc[f] = Storage<DT>::template load<T>(src, f * a.stride + c, f, a.inv_scale[f]);
+        }
+        while (uni) {
+            const int s = __builtin_ctzll(uni);
+            uni &= uni - 1;
+            double sum[6] = {0, 0, 0, 0, 0, 0};
+            if ((mine >> s) & 1ull) {
+                for (int i = 0; i < Q; ++i) {
+                    const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+                    const int mval_s = a.mask[(int64_t)(x - cx) + (int64_t)nx * ((int64_t)(y - cy) + (int64_t)ny * (z - cz))];
+                    if ((mval_s != 0) == (mval_c != 0)) continue;
+                    const int sid = mval_c ? mval_c - 1 : mval_s - 1;
+                    if (sid != s) continue;
+                    T F[6];
+                    link_force<T, DT, Q>(a, mc, x, y, z, i, sid, F);
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) sum[k] += (double)F[k];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sum[k] = wave_sum(sum[k]);
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[wave][s][k] += sum[k];
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < a.nsolid * 6; k += kReduceBlock) {
+        const int s = k / 6, q = k % 6;
+        double v = 0.0;
+        for (int w = 0; w < kWaves; ++w) v += acc[w][s][q];
+        partial[(int64_t)blockIdx.x * a.nsolid * 6 + k] = v;
+    }
+}
+
+__global__ void k_sum_partials(const double* partial, int nblocks, int width, double* out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    double v = 0.0;
+    for (int b = 0; b < nblocks; ++b) v += partial[(int64_t)b * width + k];
+    out[k] = v;
+}
+
+template <typename T, int DT>
+static hipError_t launch_forces(int lattice, const EnvArgs& a, const Region& rg, double* ws, int nb, hipStream_t s)
+{
+    if (lattice == HL_D2Q9) hipLaunchKernelGGL((k_forces<T, DT, 9>), nb, kReduceBlock, 0, s, a, rg, ws);
+    else hipLaunchKernelGGL((k_forces<T, DT, 27>), nb, kReduceBlock, 0, s, a, rg, ws);
+    return hipGetLastError();
+}
+
+__global__ void k_zero(double* out, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = 0.0;
+}
+
+hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s)
+{
+    const int width = a.nsolid * 6;
+    if (width == 0) return hipSuccess;
+    const Region rg = force_region(a, lattice);
+    if (a.mask == nullptr || rg.n() == 0) {
+        hipLaunchKernelGGL(k_zero, (width + 63) / 64, 64, 0, s, out, width);
+        return hipGetLastError();
+    }
+    const int nb = reduce_blocks(rg.n());
+    double* part = static_cast<double*>(ws);
+    hipError_t e;
+    switch (dtype) {
+    case HL_F32: e = launch_forces<float, HL_F32>(lattice, a, rg, part, nb, s); break;
+    case HL_F64: e = launch_forces<double, HL_F64>(lattice, a, rg, part, nb, s); break;
+    case HL_F16: e = launch_forces<float, HL_F16>(lattice, a, rg, part, nb, s); break;
+    case HL_BF16: e = launch_forces<float, HL_BF16>(lattice, a, rg, part, nb, s); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_partials, (width + 63) / 64, 64, 0, s, part, nb, width, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- mass
+
+size_t mass_workspace_bytes(const EnvArgs& a, int lattice) { return (size_t)kMaxReduceBlocks * sizeof(double); }
+
+template <int DT>
+__global__ void __launch_bounds__(kReduceBlock) k_mass(const void* srcv, float inv_scale0, int64_t n, double* partial)
+{
+    __shared__ double wsum[kWaves];
+    using ST = typename Storage<DT>::type;
+    const ST* src = static_cast<const ST*>(srcv);
+    double v = 0.0;
+    for (int64_t c = blockIdx.x * (int64_t)kReduceBlock + threadIdx.x; c < n; c += (int64_t)gridDim.x * kReduceBlock)
+        v += Storage<DT>::template load<double>(src, c, 0, inv_scale0);
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kWaves; ++w) t += wsum[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s)
+{
+    const int64_t n = (int64_t)a.nx * a.ny * a.nz;
+    const int nb = reduce_blocks(n);
+    double* part = static_cast<double*>(ws);
+    switch (dtype) {
+    case HL_F32: hipLaunchKernelGGL(k_mass<HL_F32>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
+    case HL_F64: hipLaunchKernelGGL(k_mass<HL_F64>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
+    case HL_F16: hipLaunchKernelGGL(k_mass<HL_F16>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
+    case HL_BF16: hipLaunchKernelGGL(k_mass<HL_BF16>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sum_partials, 1, 64, 0, s, part, nb, 1, out);
+    return hipGetLastError();
+}
+
+}  // namespace hl

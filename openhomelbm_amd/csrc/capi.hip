@@ -1,0 +1,297 @@
+// C ABI of libhomelbm (declared in include/homelbm.h): argument checks,
+// launch of the step / auxiliary kernels, and the library-owned reduction
+// workspace. Errors are reported by return code and hl_last_error().
+#pragma clang fp contract(off)
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "homelbm.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int32_t fail(int32_t code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+int32_t hip_fail(hipError_t e, const char* where)
+{
+    return fail(HL_E_HIP, "%s: %s", where, hipGetErrorString(e));
+}
+
+int nfields(int lattice) { return lattice == HL_D2Q9 ? 6 : 10; }
+
+int32_t validate(const hl_env* e, bool need_moments = true)
+{
+    if (!e) return fail(HL_E_INVALID, "env is NULL");
+    if (e->lattice != HL_D2Q9 && e->lattice != HL_D3Q27) return fail(HL_E_INVALID, "lattice must be 9 or 27, got %d", e->lattice);
+    if (e->dtype < HL_F32 || e->dtype > HL_BF16) return fail(HL_E_INVALID, "unknown dtype %d", e->dtype);
+    if (e->nx < 3 || e->ny < 3) return fail(HL_E_INVALID, "grid %dx%d: every extent must be >= 3", e->nx, e->ny);
+    if (e->lattice == HL_D2Q9 && e->nz != 1) return fail(HL_E_INVALID, "D2Q9 needs nz == 1, got %d", e->nz);
+    if (e->lattice == HL_D3Q27 && e->nz < 3) return fail(HL_E_INVALID, "D3Q27 needs nz >= 3, got %d", e->nz);
+    const int64_t n = (int64_t)e->nx * e->ny * e->nz;
+    if (n > ((int64_t)1 << 40)) return fail(HL_E_INVALID, "grid too large");
+    if (e->stride < n) return fail(HL_E_INVALID, "stride %lld < cell count %lld", (long long)e->stride, (long long)n);
+    if (need_moments && (!e->moments[0] || !e->moments[1])) return fail(HL_E_INVALID, "moment buffers are NULL");
+    if (e->nsolid < 0 || e->nsolid > HL_MAX_SOLIDS) return fail(HL_E_INVALID, "nsolid %d outside [0, %d]", e->nsolid, HL_MAX_SOLIDS);
+    if (e->nsolid > 0 && !e->solids) return fail(HL_E_INVALID, "nsolid > 0 but the pose table is NULL");
+    if (!(e->nu > 0.0f) || !std::isfinite(e->nu)) return fail(HL_E_INVALID, "viscosity must be > 0, got %g", (double)e->nu);
+    const int nfaces = e->lattice == HL_D2Q9 ? 4 : 6;
+    for (int f = 0; f < nfaces; ++f)
+        if (e->bc_type[f] < 0 || e->bc_type[f] > 2) return fail(HL_E_INVALID, "bc_type[%d] = %d not in {0, 1, 2}", f, e->bc_type[f]);
+    if (e->mask) {
+        const int32_t* b = e->mask_box;
+        if (b[0] < 0 || b[1] > e->nx || b[2] < 0 || b[3] > e->ny || b[4] < 0 || b[5] > e->nz || b[0] > b[1] ||
+            b[2] > b[3] || b[4] > b[5])
+            return fail(HL_E_INVALID, "mask_box (%d,%d,%d,%d,%d,%d) outside the grid", b[0], b[1], b[2], b[3], b[4], b[5]);
+    }
+    if (e->dtype == HL_F16 || e->dtype == HL_BF16)
+        for (int f = 0; f < nfields(e->lattice); ++f)
+            if (!(e->scale[f] > 0.0f) || !std::isfinite(e->scale[f]))
+                return fail(HL_E_INVALID, "16-bit storage needs scale[%d] > 0", f);
+    return HL_OK;
+}
+
+hl::EnvArgs make_args(const hl_env* e, int cur)
+{
+    hl::EnvArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.nsolid = e->nsolid;
+    a.stride = e->stride;
+    a.src = e->moments[cur & 1];
+    a.dst = e->moments[(cur & 1) ^ 1];
+    a.mask = e->mask;
+    for (int k = 0; k < 6; ++k) a.box[k] = e->mask_box[k];
+    if (!e->mask) { a.box[0] = a.box[2] = a.box[4] = 1 << 29; a.box[1] = a.box[3] = a.box[5] = -(1 << 29); }
+    a.force = e->force;
+    a.solids = e->solids;
+    a.nu = e->nu;
+    for (int k = 0; k < 6; ++k) {
+        a.bc_type[k] = e->bc_type[k];
+        for (int c = 0; c < 3; ++c) a.bc_value[k][c] = e->bc_value[k][c];
+    }
+    for (int f = 0; f < 10; ++f) {
+        const bool sixteen = e->dtype == HL_F16 || e->dtype == HL_BF16;
+        a.scale[f] = sixteen ? e->scale[f] : 1.0f;
+        a.inv_scale[f] = sixteen ? 1.0f / e->scale[f] : 1.0f;
+    }
+    a.region[0] = 0; a.region[1] = e->nx; a.region[2] = 0; a.region[3] = e->ny; a.region[4] = 0; a.region[5] = e->nz;
+    for (int k = 0; k < 6; ++k) a.skip[k] = 0;
+    return a;
+}
+
+// Grow-only reduction workspace, one per device, owned by the library until
+// the process ends. Sized on first use; call a reduction once outside any
+// stream capture so the allocation does not happen while capturing.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_ws_mutex;
+std::vector<Workspace> g_ws;
+
+int32_t workspace(size_t bytes, void** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
+    Workspace& w = g_ws[dev];
+    if (w.bytes < bytes) {
+        if (w.ptr) {
+            e = hipFree(w.ptr);
+            if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
+            w.ptr = nullptr;
+            w.bytes = 0;
+        }
+        e = hipMalloc(&w.ptr, bytes);
+        if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
+        w.bytes = bytes;
+    }
+    *out = w.ptr;
+    return HL_OK;
+}
+
+// Uniform-flow moments in working precision T, in the operation order of
+// set_uniform_flow_kernel (karman_lbm_env.py:15-53) and
+// set_uniform_flow_3d_kernel (karman_lbm_env_3d.py:15-64).
+template <typename T> void uniform_moments(int lattice, T ux, T uy, T uz, double m[10])
+{
+    T rho = 1;
+    for (int f = 0; f < 10; ++f) m[f] = 0.0;
+    if (lattice == HL_D2Q9) {
+        T pop[9];
+        T U2 = ux * ux + uy * uy;
+        for (int i = 0; i < 9; ++i) {
+            T cu = T(hl::D2CX[i]) * ux + T(hl::D2CY[i]) * uy;
+            pop[i] = rho * hl::d2w<T>(i) * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * U2);
+        }
+        T inv = T(1) / rho;
+        T pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+        T piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+        T pixy = pop[5] + pop[7] - pop[6] - pop[8];
+        T cs2_local = pixx;
+        m[0] = rho; m[1] = ux; m[2] = uy;
+        m[3] = pixx * inv - cs2_local; m[4] = piyy * inv - cs2_local; m[5] = pixy * inv;
+    } else {
+        T p[27];
+        T U2 = ux * ux + uy * uy + uz * uz;
+        for (int i = 0; i < 27; ++i) {
+            T cu = T(hl::D3CX[i]) * ux + T(hl::D3CY[i]) * uy + T(hl::D3CZ[i]) * uz;
+            p[i] = hl::d3w<T>(i) * rho * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * U2);
+        }
+        T inv = T(1) / rho;
+        T pixx = p[1] + p[2] + p[7] + p[8] + p[9] + p[10] + p[13] + p[14] + p[15] + p[16] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        T pixy = (p[7] + p[8] + p[19] + p[20] + p[21] + p[22]) - (p[13] + p[14] + p[23] + p[24] + p[25] + p[26]);
+        T pixz = (p[9] + p[10] + p[19] + p[20] + p[23] + p[24]) - (p[15] + p[16] + p[21] + p[22] + p[25] + p[26]);
+        T piyy = p[3] + p[4] + p[7] + p[8] + p[11] + p[12] + p[13] + p[14] + p[17] + p[18] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        T piyz = (p[11] + p[12] + p[19] + p[20] + p[25] + p[26]) - (p[17] + p[18] + p[21] + p[22] + p[23] + p[24]);
+        T pizz = p[5] + p[6] + p[9] + p[10] + p[11] + p[12] + p[15] + p[16] + p[17] + p[18] + p[19] + p[20]
+               + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+        T cs2_local = pixx;
+        m[0] = rho; m[1] = ux; m[2] = uy; m[3] = uz;
+        m[4] = pixx * inv - cs2_local; m[5] = piyy * inv - cs2_local; m[6] = pizz * inv - cs2_local;
+        m[7] = pixy * inv; m[8] = pixz * inv; m[9] = piyz * inv;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hl_version(void) { return 100; }
+
+int32_t hl_env_bytes(void) { return (int32_t)sizeof(hl_env); }
+
+const char* hl_last_error(void) { return g_error.c_str(); }
+
+int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    void* ws = nullptr;
+    if (force_history && env->nsolid > 0) {
+        rc = workspace(hl::forces_workspace_bytes(make_args(env, cur), env->lattice), &ws);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        hl::EnvArgs a = make_args(env, cur);
+        hipError_t e = hipErrorNotSupported;
+        if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) hl::plan_tiled(env->lattice, env->dtype, a);
+        if (ws) {
+            e = hl::forces(env->lattice, env->dtype, a, force_history + (int64_t)k * env->nsolid * 6, ws, s);
+            if (e != hipSuccess) return hip_fail(e, "forces kernel launch");
+            e = hipErrorNotSupported;
+        }
+        if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) {
+            // the tiled kernel leaves blocks near the body mask to the gather kernel
+            e = hl::step_tiled(env->lattice, env->dtype, a, s);
+            if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
+                return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
+            if (e == hipSuccess && a.region[0] < a.region[1]) e = hl::step_gather(env->lattice, env->dtype, a, s);
+        }
+        if (e == hipErrorNotSupported) e = hl::step_gather(env->lattice, env->dtype, a, s);
+        if (e != hipSuccess) return hip_fail(e, "step kernel launch");
+        cur ^= 1;
+    }
+    return HL_OK;
+}
+
+int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (!out) return fail(HL_E_INVALID, "out is NULL");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    hl::EnvArgs a = make_args(env, cur);
+    void* ws = nullptr;
+    rc = workspace(hl::forces_workspace_bytes(a, env->lattice), &ws);
+    if (rc) return rc;
+    hipError_t e = hl::forces(env->lattice, env->dtype, a, out, ws, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "forces kernel launch");
+    return HL_OK;
+}
+
+int32_t hl_init_uniform(const hl_env* env, int32_t buf, float ux, float uy, float uz, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (buf != 0 && buf != 1) return fail(HL_E_INVALID, "buf must be 0 or 1");
+    double m[10];
+    if (env->dtype == HL_F64) uniform_moments<double>(env->lattice, ux, uy, uz, m);
+    else uniform_moments<float>(env->lattice, ux, uy, uz, m);
+    hl::EnvArgs a = make_args(env, 0);
+    hipError_t e = hl::init_uniform(env->lattice, env->dtype, a, env->moments[buf], m, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "init kernel launch");
+    return HL_OK;
+}
+
+int32_t hl_voxelize(const hl_env* env, uint8_t* mask, const float* caps, int32_t ncap, const int32_t box[6], void* stream)
+{
+    int32_t rc = validate(env, false);
+    if (rc) return rc;
+    if (!mask) return fail(HL_E_INVALID, "mask is NULL");
+    if (ncap < 0 || (ncap > 0 && !caps)) return fail(HL_E_INVALID, "bad capsule table");
+    if (!box) return fail(HL_E_INVALID, "box is NULL");
+    if (box[0] < 0 || box[1] > env->nx || box[2] < 0 || box[3] > env->ny || box[4] < 0 || box[5] > env->nz)
+        return fail(HL_E_INVALID, "voxelize box outside the grid");
+    hl::EnvArgs a = make_args(env, 0);
+    hipError_t e = hl::voxelize(env->lattice, a, mask, caps, ncap, box, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "voxelize kernel launch");
+    return HL_OK;
+}
+
+int32_t hl_set_tiling(int32_t lattice, int32_t tx, int32_t ty, int32_t zc)
+{
+    if (!hl::tiling_supported(lattice, tx, ty, zc))
+        return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d with tile %dx%dx%d", lattice, tx, ty, zc);
+    hl::set_tiling(lattice, hl::TileCfg{tx, ty, zc});
+    return HL_OK;
+}
+
+int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc)
+{
+    if (!tx || !ty || !zc) return fail(HL_E_INVALID, "NULL output");
+    hl::TileCfg c = hl::get_tiling(lattice);
+    *tx = c.tx; *ty = c.ty; *zc = c.zc;
+    return HL_OK;
+}
+
+int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (!out) return fail(HL_E_INVALID, "out is NULL");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    hl::EnvArgs a = make_args(env, cur);
+    void* ws = nullptr;
+    rc = workspace(hl::mass_workspace_bytes(a, env->lattice), &ws);
+    if (rc) return rc;
+    hipError_t e = hl::mass(env->lattice, env->dtype, a, out, ws, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "mass kernel launch");
+    return HL_OK;
+}
+
+}  // extern "C"

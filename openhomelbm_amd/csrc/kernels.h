@@ -1,0 +1,36 @@
+// Internal launcher interface between the C ABI (capi.hip) and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "homelbm.h"
+#include "lbm_device.h"
+
+namespace hl {
+
+struct TileCfg {
+    int tx, ty, zc;
+};
+bool tiling_supported(int lattice, int tx, int ty, int zc);
+void set_tiling(int lattice, TileCfg c);
+TileCfg get_tiling(int lattice);
+
+hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
+// Returns hipErrorNotSupported when no tiled kernel exists for this case.
+hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
+// Sets a.skip (tile blocks next to the body mask) and a.region (the cells of
+// those blocks, for the gather kernel). Leaves a.region empty when no block
+// is skipped; leaves both untouched when the tiled kernel does not apply.
+void plan_tiled(int lattice, int dtype, EnvArgs& a);
+
+hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, const double m[10], hipStream_t s);
+hipError_t voxelize(int lattice, const EnvArgs& a, uint8_t* mask, const float* caps, int ncap, const int32_t box[6],
+                    hipStream_t s);
+// ws: workspace of at least forces_workspace_bytes() bytes.
+size_t forces_workspace_bytes(const EnvArgs& a, int lattice);
+hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
+size_t mass_workspace_bytes(const EnvArgs& a, int lattice);
+hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
+
+}  // namespace hl

@@ -1,0 +1,323 @@
+// Device-side building blocks of the HOME-LBM step for gfx950: lattice
+// tables, moment storage codecs, population reconstruction, collision, face
+// boundary conditions and the wall velocity of a rigid body. Every function is
+// templated on the arithmetic type T (float or double) so the same source
+// gives the FP32 production path and the FP64 GPU oracle.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "homelbm.h"
+
+namespace hl {
+
+// D2Q9 / D3Q27 velocity sets in the reference's index order
+// (envs/lbm/lbm_core.py:127-130, envs/lbm3d/lbm_core_3d.py:12-35). The moment
+// sums and the inverse-direction map depend on this order.
+constexpr int D2CX[9] = {0, 1, -1, 0, 0, 1, -1, -1, 1};
+constexpr int D2CY[9] = {0, 0, 0, 1, -1, 1, 1, -1, -1};
+constexpr int D2INV[9] = {0, 2, 1, 4, 3, 7, 8, 5, 6};
+constexpr int D3CX[27] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1};
+constexpr int D3CY[27] = {0, 0, 0, 1, -1, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1};
+constexpr int D3CZ[27] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 1, -1, 1, -1, 0, 0, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, 1, -1};
+constexpr int D3INV[27] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 20, 19, 22, 21, 24, 23, 26, 25};
+
+// Weights: the exact fractions rounded once to T.
+template <typename T> __host__ __device__ constexpr T d2w(int i)
+{
+    return i == 0 ? T(4.0 / 9.0) : (i <= 4 ? T(1.0 / 9.0) : T(1.0 / 36.0));
+}
+template <typename T> __host__ __device__ constexpr T d3w(int i)
+{
+    return i == 0 ? T(8.0 / 27.0) : (i <= 6 ? T(2.0 / 27.0) : (i <= 18 ? T(1.0 / 54.0) : T(1.0 / 216.0)));
+}
+// c_s^2 as the reference rounds it: the float32 sum of D2Q9 weights
+// (lbm_func.py:4) and float32(1/3) (lbm_core_3d.py:37).
+template <typename T> __host__ __device__ constexpr T cs2_2d()
+{
+    return sizeof(T) == 4 ? T(1.0 / 9 + 1.0 / 9 + 1.0 / 36 + 1.0 / 36 + 1.0 / 36 + 1.0 / 36) : T(1.0 / 3.0);
+}
+template <typename T> __host__ __device__ constexpr T cs2_3d() { return T(1.0 / 3.0); }
+
+// ------------------------------------------------------------ storage
+
+// Moment codecs. 32/64-bit fields are stored as is. 16-bit fields hold
+// (rho - 1) * scale for the density and value * scale for every other field;
+// decode multiplies by 1/scale (precomputed by the host).
+template <int DT> struct Storage;
+template <> struct Storage<HL_F32> {
+    using type = float;
+    template <typename T> __device__ static T load(const float* p, int64_t i, int, float) { return T(p[i]); }
+    template <typename T> __device__ static void store(float* p, int64_t i, T v, int, float) { p[i] = float(v); }
+};
+template <> struct Storage<HL_F64> {
+    using type = double;
+    template <typename T> __device__ static T load(const double* p, int64_t i, int, float) { return T(p[i]); }
+    template <typename T> __device__ static void store(double* p, int64_t i, T v, int, float) { p[i] = double(v); }
+};
+template <> struct Storage<HL_F16> {
+    using type = __half;
+    template <typename T> __device__ static T load(const __half* p, int64_t i, int f, float inv_scale)
+    {
+        float v = __half2float(p[i]) * inv_scale;
+        return T(f == 0 ? v + 1.0f : v);
+    }
+    template <typename T> __device__ static void store(__half* p, int64_t i, T v, int f, float scale)
+    {
+        float w = f == 0 ? float(v) - 1.0f : float(v);
+        p[i] = __float2half_rn(w * scale);
+    }
+};
+// bfloat16 as raw 16-bit words: round-to-nearest-even on encode.
+__device__ __forceinline__ uint16_t f32_to_bf16(float v)
+{
+    uint32_t u = __float_as_uint(v);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+template <> struct Storage<HL_BF16> {
+    using type = uint16_t;
+    template <typename T> __device__ static T load(const uint16_t* p, int64_t i, int f, float inv_scale)
+    {
+        float v = bf16_to_f32(p[i]) * inv_scale;
+        return T(f == 0 ? v + 1.0f : v);
+    }
+    template <typename T> __device__ static void store(uint16_t* p, int64_t i, T v, int f, float scale)
+    {
+        float w = f == 0 ? float(v) - 1.0f : float(v);
+        p[i] = f32_to_bf16(w * scale);
+    }
+};
+
+// ------------------------------------------------------- reconstruction
+
+// D2Q9 population i from (rho, u, S): mlCalDistribution,
+// envs/lbm/lbm_func.py:685-788, same operation order.
+template <typename T>
+__device__ __forceinline__ T d2q9_recon(T rho, T ux, T uy, T Sxx, T Syy, T Sxy, int i)
+{
+    const T one = 1, two = 2, three = 3, half = T(0.5), c45 = T(4.5), nine = 9;
+    T Axxy = Sxx * uy + two * Sxy * ux - two * ux * ux * uy;
+    T Axyy = Syy * ux + two * Sxy * uy - two * ux * uy * uy;
+    const T w0 = T(4) / T(9), w1 = T(1) / T(9), w5 = T(1) / T(36);
+    switch (i) {
+    case 0: return w0 * rho * (one - half * three * Sxx - half * three * Syy);
+    case 1: return w1 * rho * (one + three * ux + three * Sxx - half * three * Syy - c45 * Axyy);
+    case 2: return w1 * rho * (one - three * ux + three * Sxx - half * three * Syy + c45 * Axyy);
+    case 3: return w1 * rho * (one + three * uy - half * three * Sxx + three * Syy - c45 * Axxy);
+    case 4: return w1 * rho * (one - three * uy - half * three * Sxx + three * Syy + c45 * Axxy);
+    case 5: return w5 * rho * (one + three * (ux + uy) + three * (Sxx + Syy + three * Sxy) + nine * (Axyy + Axxy));
+    case 6: return w5 * rho * (one - three * ux + three * uy + three * (Sxx + Syy - three * Sxy) + nine * (Axxy - Axyy));
+    case 7: return w5 * rho * (one - three * ux - three * uy + three * (Sxx + Syy + three * Sxy) - nine * (Axxy + Axyy));
+    default: return w5 * rho * (one + three * ux - three * uy + three * (Sxx + Syy - three * Sxy) - nine * (Axxy - Axyy));
+    }
+}
+
+// D3Q27 population with velocity (dx, dy, dz) and weight wi:
+// mlCalEquilibrium3D_s, envs/lbm3d/lbm_func_3d.py:224-251 (argument order
+// rho, u, Sxx, Sxy, Sxz, Syy, Syz, Szz as in the reference).
+template <typename T>
+__device__ __forceinline__ T d3q27_recon(T rho, T ux, T uy, T uz, T Sxx, T Sxy, T Sxz, T Syy, T Syz, T Szz,
+                                         T dx, T dy, T dz, T wi)
+{
+    T cu = dx * ux + dy * uy + dz * uz;
+    T U_sqr = ux * ux + uy * uy + uz * uz;
+    const T third = T(1) / T(3);
+    T Qxx = dx * dx - third, Qyy = dy * dy - third, Qzz = dz * dz - third;
+    T Qxy = dx * dy, Qxz = dx * dz, Qyz = dy * dz;
+    T f_neq = T(1.5) * (Qxx * Sxx + Qyy * Syy + Qzz * Szz + T(2) * (Qxy * Sxy + Qxz * Sxz + Qyz * Syz));
+    return wi * rho * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * U_sqr + f_neq);
+}
+
+// --------------------------------------------------------- equilibrium S
+
+// S of the second-order equilibrium from its populations: f_eq_home
+// (lbm_func.py:660-682, populations clamped at 0) and f_eq_home_3d
+// (lbm_func_3d.py:651-686, no clamp). 2D out: xx, yy, xy; 3D out in
+// storage order xx, yy, zz, xy, xz, yz.
+template <typename T> __device__ __forceinline__ void d2q9_feq_S(T rho, T ux, T uy, T S[3])
+{
+    T pop[9];
+    T U_sqr = ux * ux + uy * uy;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        T cu = T(D2CX[i]) * ux + T(D2CY[i]) * uy;
+        pop[i] = rho * d2w<T>(i) * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * U_sqr);
+        if (pop[i] < T(0)) pop[i] = T(0);
+    }
+    T invRho = T(1) / rho;
+    T pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+    T piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    T pixy = pop[5] + pop[7] - pop[6] - pop[8];
+    S[0] = pixx * invRho - cs2_2d<T>();
+    S[1] = piyy * invRho - cs2_2d<T>();
+    S[2] = pixy * invRho;
+}
+
+template <typename T> __device__ __forceinline__ void d3q27_feq_S(T rho, T ux, T uy, T uz, T S[6])
+{
+    T p[27];
+    T U_sqr = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+        T cu = T(D3CX[i]) * ux + T(D3CY[i]) * uy + T(D3CZ[i]) * uz;
+        p[i] = d3w<T>(i) * rho * (T(1) + T(3) * cu + T(4.5) * cu * cu - T(1.5) * U_sqr);
+    }
+    T invRho = T(1) / rho;
+    T pixx = p[1] + p[2] + p[7] + p[8] + p[9] + p[10] + p[13] + p[14] + p[15] + p[16] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    T pixy = (p[7] + p[8] + p[19] + p[20] + p[21] + p[22]) - (p[13] + p[14] + p[23] + p[24] + p[25] + p[26]);
+    T pixz = (p[9] + p[10] + p[19] + p[20] + p[23] + p[24]) - (p[15] + p[16] + p[21] + p[22] + p[25] + p[26]);
+    T piyy = p[3] + p[4] + p[7] + p[8] + p[11] + p[12] + p[13] + p[14] + p[17] + p[18] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    T piyz = (p[11] + p[12] + p[19] + p[20] + p[25] + p[26]) - (p[17] + p[18] + p[21] + p[22] + p[23] + p[24]);
+    T pizz = p[5] + p[6] + p[9] + p[10] + p[11] + p[12] + p[15] + p[16] + p[17] + p[18] + p[19] + p[20]
+           + p[21] + p[22] + p[23] + p[24] + p[25] + p[26];
+    S[0] = pixx * invRho - cs2_3d<T>();
+    S[1] = piyy * invRho - cs2_3d<T>();
+    S[2] = pizz * invRho - cs2_3d<T>();
+    S[3] = pixy * invRho;
+    S[4] = pixz * invRho;
+    S[5] = piyz * invRho;
+}
+
+// ----------------------------------------------------------- face BCs
+
+// One face boundary condition: the boundary cell takes its value from the
+// inner neighbour m (apply_bc_core, lbm_func.py:634-657 and
+// lbm_func_3d.py:627-648). m is replaced by the boundary cell's value.
+// Type 0: velocity; 1: zero gradient; 2: free slip (normal velocity zero,
+// no reference counterpart). D = 2 or 3; m = rho, u[D], S[...].
+template <typename T, int D>
+__device__ __forceinline__ void bc_face(T* m, int type, const float* value, int axis)
+{
+    T u_n[D], u_b[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) { u_n[a] = m[1 + a]; u_b[a] = m[1 + a]; }
+    if (type == 0) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) u_b[a] = T(value[a]);
+    } else if (type == 2) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) u_b[a] = (a == axis) ? T(0) : u_n[a];
+    }
+    const T rho_n = m[0];
+    if (D == 2) {
+        T Sb[3], Sn[3];
+        d2q9_feq_S<T>(rho_n, u_b[0], u_b[1], Sb);
+        d2q9_feq_S<T>(rho_n, u_n[0], u_n[1], Sn);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) m[3 + k] = Sb[k] - Sn[k] + m[3 + k];
+    } else {
+        T Sb[6], Sn[6];
+        d3q27_feq_S<T>(rho_n, u_b[0], u_b[1], u_b[D - 1], Sb);
+        d3q27_feq_S<T>(rho_n, u_n[0], u_n[1], u_n[D - 1], Sn);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) m[4 + k] = Sb[k] - Sn[k] + m[4 + k];
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) m[1 + a] = u_b[a];
+}
+
+// Face value of a cell from the post-collision value m of its projection
+// onto the interior: x faces, then y faces, then z faces. This is the
+// sequential outcome of the reference's one-launch apply_bc{,_3d}
+// (lbm_func.py:610-631, lbm_func_3d.py:607-624), whose edge and corner
+// threads race; the fixed order removes the race.
+template <typename T, int D>
+__device__ __forceinline__ void bc_chain(T* m, int x, int y, int z, int nx, int ny, int nz,
+                                         const int32_t* bc_type, const float (*bc_value)[3])
+{
+    // face ids: 2D left 0, top 1, right 2, bottom 3; 3D left 0, right 1, top 2, bottom 3, front 4, back 5
+    const int f_right = D == 3 ? 1 : 2, f_top = D == 3 ? 2 : 1;
+    if (x == 0) bc_face<T, D>(m, bc_type[0], bc_value[0], 0);
+    else if (x == nx - 1) bc_face<T, D>(m, bc_type[f_right], bc_value[f_right], 0);
+    if (y == ny - 1) bc_face<T, D>(m, bc_type[f_top], bc_value[f_top], 1);
+    else if (y == 0) bc_face<T, D>(m, bc_type[3], bc_value[3], 1);
+    if (D == 3) {
+        if (z == nz - 1) bc_face<T, D>(m, bc_type[4], bc_value[4], 2);
+        else if (z == 0) bc_face<T, D>(m, bc_type[5], bc_value[5], 2);
+    }
+}
+
+// -------------------------------------------------------- wall velocity
+
+// Velocity of the body material point at h over one step (time step 1):
+// its current position minus the position of the same body-frame point one
+// step earlier (lbm_func.py:354-370, lbm_func_3d.py:456-463). Pose tables:
+// 2D px, py, cos, sin | last px, py, cos, sin; 3D p[3], R[9] | last p[3], R[9].
+// r = h - p (torque arm).
+template <typename T>
+__device__ __forceinline__ void wall_velocity_2d(const float* solids, int nsolid, int sid, T hx, T hy, T up[2], T r[2])
+{
+    if (sid < 0 || sid >= nsolid) { up[0] = up[1] = r[0] = r[1] = T(0); return; }
+    const float* P = solids + 8 * sid;
+    T px = P[0], py = P[1], cn = P[2], sn = P[3];
+    T pxl = P[4], pyl = P[5], cl = P[6], sl = P[7];
+    T dx = hx - px, dy = hy - py;
+    T qx = cn * dx + sn * dy;
+    T qy = cn * dy - sn * dx;
+    T cx = cn * qx - sn * qy + px;
+    T cy = sn * qx + cn * qy + py;
+    T lx = cl * qx - sl * qy + pxl;
+    T ly = sl * qx + cl * qy + pyl;
+    up[0] = cx - lx;
+    up[1] = cy - ly;
+    r[0] = dx;
+    r[1] = dy;
+}
+
+template <typename T>
+__device__ __forceinline__ void wall_velocity_3d(const float* solids, int nsolid, int sid, T hx, T hy, T hz, T up[3], T r[3])
+{
+    if (sid < 0 || sid >= nsolid) { up[0] = up[1] = up[2] = r[0] = r[1] = r[2] = T(0); return; }
+    const float* P = solids + 24 * sid;
+    const float* M = P + 3;
+    const float* L = P + 12;
+    const float* ML = P + 15;
+    T d0 = hx - T(P[0]), d1 = hy - T(P[1]), d2 = hz - T(P[2]);
+    T q0 = T(M[0]) * d0 + T(M[3]) * d1 + T(M[6]) * d2;
+    T q1 = T(M[1]) * d0 + T(M[4]) * d1 + T(M[7]) * d2;
+    T q2 = T(M[2]) * d0 + T(M[5]) * d1 + T(M[8]) * d2;
+    T c0 = T(M[0]) * q0 + T(M[1]) * q1 + T(M[2]) * q2 + T(P[0]);
+    T c1 = T(M[3]) * q0 + T(M[4]) * q1 + T(M[5]) * q2 + T(P[1]);
+    T c2 = T(M[6]) * q0 + T(M[7]) * q1 + T(M[8]) * q2 + T(P[2]);
+    T l0 = T(ML[0]) * q0 + T(ML[1]) * q1 + T(ML[2]) * q2 + T(L[0]);
+    T l1 = T(ML[3]) * q0 + T(ML[4]) * q1 + T(ML[5]) * q2 + T(L[1]);
+    T l2 = T(ML[6]) * q0 + T(ML[7]) * q1 + T(ML[8]) * q2 + T(L[2]);
+    up[0] = c0 - l0; up[1] = c1 - l1; up[2] = c2 - l2;
+    r[0] = d0; r[1] = d1; r[2] = d2;
+}
+
+// Kernel-side view of an hl_env for one precision (by value, so it can be a
+// kernel argument and a graph node parameter).
+struct EnvArgs {
+    int32_t nx, ny, nz, nsolid;
+    int64_t stride;
+    const void* src;
+    void* dst;
+    const uint8_t* mask;
+    int32_t box[6];
+    const float* force;
+    const float* solids;
+    float nu;
+    int32_t bc_type[6];
+    float bc_value[6][3];
+    float scale[10];
+    float inv_scale[10];
+    int32_t region[6];  // cells the gather kernel updates: x0, x1, y0, y1, z0, z1 (half-open)
+    int32_t skip[6];    // tile-unit block box the tiled kernel leaves to the gather kernel
+};
+
+__device__ __forceinline__ bool in_box_halo(const EnvArgs& a, int x, int y, int z)
+{
+    return x >= a.box[0] - 1 && x <= a.box[1] && y >= a.box[2] - 1 && y <= a.box[3] &&
+           z >= a.box[4] - 1 && z <= a.box[5];
+}
+
+}  // namespace hl

@@ -1,0 +1,380 @@
+// LDS-staged source-population step kernel for D3Q27 (HL_KERNEL_TILED).
+//
+// The reference pulls, for every cell, the full moment set of each of its 27
+// neighbours and rebuilds one population from each (27 x 10 loads and 27
+// reconstructions per cell, stream_and_collide_3d, lbm_func_3d.py:414-526).
+// A population f_i(x) depends only on the moments at the source x - c_i, so
+// here every cell rebuilds its own 27 outgoing populations once, with the
+// shared subexpressions of the 13 direction pairs, and hands them to its
+// neighbours through LDS. Arithmetically this is the same pull; rounding
+// differs from the reference's order at the 1e-7 level.
+//
+// Work decomposition (one block = a TX x TY column tile and a chunk of z):
+//   * 27 population slots per column in LDS (one plane's worth);
+//   * the block marches z upward. At plane z it rebuilds the sources of
+//     plane z+1; their cz = -1 ("down") populations go to LDS at once, their
+//     cz = 0 ("mid") ones are held in registers one step and their cz = +1
+//     ("up") ones two steps, so the slots always hold down(z+1), mid(z),
+//     up(z-1): exactly what the cells of plane z pull;
+//   * source columns cover the tile plus a one-cell halo, one per thread;
+//     the first TX*TY threads also own the destination cells;
+//   * moments of the next source plane are prefetched into registers;
+//   * face cells gather at their projection onto the interior and apply the
+//     face conditions in the order x, y, z (plane 0 and nz-1 are written by
+//     the chunks that own planes 1 and nz-2);
+//   * blocks whose cells can have cut links (within one cell of the body
+//     mask box) are skipped here and updated by the gather kernel, which
+//     applies the mask wall rule in the reference's operation order; the
+//     tiled kernel itself carries no wall-rule code and so stays small in
+//     registers.
+#include "kernels.h"
+#include "lbm_device.h"
+
+namespace hl {
+
+namespace {
+
+constexpr int kUp[9] = {5, 9, 11, 16, 18, 19, 22, 23, 25};     // cz = +1
+constexpr int kMid[9] = {0, 1, 2, 3, 4, 7, 8, 13, 14};        // cz = 0
+constexpr int kDown[9] = {6, 10, 12, 15, 17, 20, 21, 24, 26}; // cz = -1
+
+// All 27 populations of one cell (lbm_func_3d.py:224-251 for every i),
+// grouped by direction pairs: f_{+c} = w rho (E + O), f_{-c} = w rho (E - O)
+// with even part E = A + 4.5 (c.u)^2 + 1.5 sum_a c_a^2 S_aa
+// + 3 sum_{a<b} c_a c_b S_ab, A = 1 - 1.5 u.u - 0.5 tr S, and odd part
+// O = 3 c.u. Populations are clamped at 0 as on the reference's plain
+// streaming branch (lbm_func_3d.py:525-526).
+__device__ __forceinline__ void recon27(const float* m, float* f)
+{
+    const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+    const float sxx = m[4], syy = m[5], szz = m[6], sxy = m[7], sxz = m[8], syz = m[9];
+    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
+    const float trS = sxx + syy + szz;
+    const float A = 1.0f - 1.5f * (ux2 + uy2 + uz2) - 0.5f * trS;
+    const float r0 = (float)(8.0 / 27.0) * rho, r1 = (float)(2.0 / 27.0) * rho;
+    const float r2 = (float)(1.0 / 54.0) * rho, r3 = (float)(1.0 / 216.0) * rho;
+    f[0] = r0 * A;
+    {   // axes
+        float E = A + 4.5f * ux2 + 1.5f * sxx, O = 3.0f * ux;
+        f[1] = r1 * (E + O); f[2] = r1 * (E - O);
+        E = A + 4.5f * uy2 + 1.5f * syy; O = 3.0f * uy;
+        f[3] = r1 * (E + O); f[4] = r1 * (E - O);
+        E = A + 4.5f * uz2 + 1.5f * szz; O = 3.0f * uz;
+        f[5] = r1 * (E + O); f[6] = r1 * (E - O);
+    }
+    {   // edges
+        const float Pxy = A + 1.5f * (sxx + syy), Pxz = A + 1.5f * (sxx + szz), Pyz = A + 1.5f * (syy + szz);
+        float s = ux + uy, E = Pxy + 3.0f * sxy + 4.5f * s * s, O = 3.0f * s;
+        f[7] = r2 * (E + O); f[8] = r2 * (E - O);
+        s = ux - uy; E = Pxy - 3.0f * sxy + 4.5f * s * s; O = 3.0f * s;
+        f[13] = r2 * (E + O); f[14] = r2 * (E - O);
+        s = ux + uz; E = Pxz + 3.0f * sxz + 4.5f * s * s; O = 3.0f * s;
+        f[9] = r2 * (E + O); f[10] = r2 * (E - O);
+        s = ux - uz; E = Pxz - 3.0f * sxz + 4.5f * s * s; O = 3.0f * s;
+        f[15] = r2 * (E + O); f[16] = r2 * (E - O);
+        s = uy + uz; E = Pyz + 3.0f * syz + 4.5f * s * s; O = 3.0f * s;
+        f[11] = r2 * (E + O); f[12] = r2 * (E - O);
+        s = uy - uz; E = Pyz - 3.0f * syz + 4.5f * s * s; O = 3.0f * s;
+        f[17] = r2 * (E + O); f[18] = r2 * (E - O);
+    }
+    {   // corners
+        const float R = A + 1.5f * trS;
+        float s = ux + uy + uz, E = R + 3.0f * (sxy + sxz + syz) + 4.5f * s * s, O = 3.0f * s;
+        f[19] = r3 * (E + O); f[20] = r3 * (E - O);
+        s = ux + uy - uz; E = R + 3.0f * (sxy - sxz - syz) + 4.5f * s * s; O = 3.0f * s;
+        f[21] = r3 * (E + O); f[22] = r3 * (E - O);
+        s = ux - uy + uz; E = R + 3.0f * (sxz - sxy - syz) + 4.5f * s * s; O = 3.0f * s;
+        f[23] = r3 * (E + O); f[24] = r3 * (E - O);
+        s = uy + uz - ux; E = R + 3.0f * (syz - sxy - sxz) + 4.5f * s * s; O = 3.0f * s;
+        f[25] = r3 * (E + O); f[26] = r3 * (E - O);
+    }
+#pragma unroll
+    for (int i = 0; i < 27; ++i) f[i] = fmaxf(f[i], 0.0f);
+}
+
+// Moment accumulators of the pulled populations, grouped by the sign of each
+// velocity component (the reference's sign groups, lbm_func_3d.py:528-562).
+struct Acc {
+    float xp = 0, xm = 0, x0 = 0, yp = 0, ym = 0, zp = 0, zm = 0;
+    float pxy = 0, pxz = 0, pyz = 0;
+};
+
+template <int I>
+__device__ __forceinline__ void accumulate(Acc& s, float g)
+{
+    constexpr int cx = D3CX[I], cy = D3CY[I], cz = D3CZ[I];
+    if (cx > 0) s.xp += g; else if (cx < 0) s.xm += g; else s.x0 += g;
+    if (cy > 0) s.yp += g; else if (cy < 0) s.ym += g;
+    if (cz > 0) s.zp += g; else if (cz < 0) s.zm += g;
+    if (cx * cy > 0) s.pxy += g; else if (cx * cy < 0) s.pxy -= g;
+    if (cx * cz > 0) s.pxz += g; else if (cx * cz < 0) s.pxz -= g;
+    if (cy * cz > 0) s.pyz += g; else if (cy * cz < 0) s.pyz -= g;
+}
+
+// Regularized collision of lbm_func_3d.py:564-604 on the accumulated
+// moments, written as trace / deviator relaxation:
+//   S'_aa = (1 - Om) (dev_aa / rho + u.u / 3) + Om u_a^2 + F_a u_a / rho
+//   S'_ab = (1 - Om) Pi_ab / rho + Om u_a u_b + (F_a u_b + F_b u_a) / (2 rho)
+// which equals the reference's Pi'/rho - 1/3 without the cancellation
+// against 1/3. out: rho, u + F/(2 rho), S (storage order).
+__device__ __forceinline__ void collide27(const Acc& s, const float* F, float omega, float* out)
+{
+    const float pxx = s.xp + s.xm, pyy = s.yp + s.ym, pzz = s.zp + s.zm;
+    const float rho = pxx + s.x0;
+    const float inv = 1.0f / rho;
+    const float ux = (s.xp - s.xm + 0.5f * F[0]) * inv;
+    const float uy = (s.yp - s.ym + 0.5f * F[1]) * inv;
+    const float uz = (s.zp - s.zm + 0.5f * F[2]) * inv;
+    const float t3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
+    const float usq3 = (ux * ux + uy * uy + uz * uz) * (1.0f / 3.0f);
+    const float om1 = 1.0f - omega;
+    out[0] = rho;
+    out[1] = ux + 0.5f * F[0] * inv;
+    out[2] = uy + 0.5f * F[1] * inv;
+    out[3] = uz + 0.5f * F[2] * inv;
+    out[4] = om1 * ((pxx - t3) * inv + usq3) + omega * ux * ux + F[0] * ux * inv;
+    out[5] = om1 * ((pyy - t3) * inv + usq3) + omega * uy * uy + F[1] * uy * inv;
+    out[6] = om1 * ((pzz - t3) * inv + usq3) + omega * uz * uz + F[2] * uz * inv;
+    out[7] = om1 * s.pxy * inv + omega * ux * uy + 0.5f * (F[0] * uy + F[1] * ux) * inv;
+    out[8] = om1 * s.pxz * inv + omega * ux * uz + 0.5f * (F[0] * uz + F[2] * ux) * inv;
+    out[9] = om1 * s.pyz * inv + omega * uy * uz + 0.5f * (F[1] * uz + F[2] * uy) * inv;
+}
+
+// One face condition with the equilibrium S in closed form: f_eq_home_3d
+// (lbm_func_3d.py:651-686) builds unclamped populations whose second moment
+// is rho (I/3 + u u), so S_eq = u u and
+// S_b = u_b u_b - u_n u_n + S_n (apply_bc_core_3d, lbm_func_3d.py:643-648).
+__device__ __forceinline__ void bc_face_closed(float* m, int type, const float* value, int axis)
+{
+    const float un[3] = {m[1], m[2], m[3]};
+    float ub[3] = {m[1], m[2], m[3]};
+    if (type == 0) { ub[0] = value[0]; ub[1] = value[1]; ub[2] = value[2]; }
+    else if (type == 2) ub[axis] = 0.0f;
+    m[4] += ub[0] * ub[0] - un[0] * un[0];
+    m[5] += ub[1] * ub[1] - un[1] * un[1];
+    m[6] += ub[2] * ub[2] - un[2] * un[2];
+    m[7] += ub[0] * ub[1] - un[0] * un[1];
+    m[8] += ub[0] * ub[2] - un[0] * un[2];
+    m[9] += ub[1] * ub[2] - un[1] * un[2];
+    m[1] = ub[0]; m[2] = ub[1]; m[3] = ub[2];
+}
+
+template <int DT>
+__device__ __forceinline__ void load_moments(const EnvArgs& a, int64_t c, float* m)
+{
+    using ST = typename Storage<DT>::type;
+    const ST* src = static_cast<const ST*>(a.src);
+#pragma unroll
+    for (int f = 0; f < 10; ++f) m[f] = Storage<DT>::template load<float>(src, f * a.stride + c, f, a.inv_scale[f]);
+}
+
+template <int DT>
+__device__ __forceinline__ void store_moments(const EnvArgs& a, int64_t c, const float* m)
+{
+    using ST = typename Storage<DT>::type;
+    ST* dst = static_cast<ST*>(a.dst);
+#pragma unroll
+    for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+}
+
+}  // namespace
+
+template <int DT, int TX, int TY>
+__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+k_tiled_d3q27(EnvArgs a, int zc)
+{
+    constexpr int NX = TX + 2, NY = TY + 2, NCOL = NX * NY;
+    __shared__ float sp[27][NCOL];
+
+    // blocks next to the body mask are updated by the gather kernel (a.region)
+    if ((int)blockIdx.x >= a.skip[0] && (int)blockIdx.x < a.skip[1] && (int)blockIdx.y >= a.skip[2] &&
+        (int)blockIdx.y < a.skip[3] && (int)blockIdx.z >= a.skip[4] && (int)blockIdx.z < a.skip[5])
+        return;
+    const int tid = threadIdx.x;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int64_t plane = (int64_t)nx * ny;
+    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
+    const int zb = 1 + blockIdx.z * zc;
+    const int ze = min(zb + zc, nz - 1);  // destination planes [zb, ze)
+
+    // source column of this thread (tile + halo)
+    const int sgx = x0 - 1 + tid % NX, sgy = y0 - 1 + tid / NX;
+    const bool src_ok = tid < NCOL && sgx >= 0 && sgx < nx && sgy >= 0 && sgy < ny;
+    const int64_t scol = (int64_t)sgy * nx + sgx;
+
+    // destination cell of this thread
+    const int dx = x0 + tid % TX, dy = y0 + tid / TX;
+    const bool dst_ok = tid < TX * TY && dx < nx && dy < ny;
+    const int px = min(max(dx, 1), nx - 2), py = min(max(dy, 1), ny - 2);
+    const int cp = (py - y0 + 1) * NX + (px - x0 + 1);
+    const int64_t pcol = (int64_t)py * nx + px;
+
+    const float omega = 1.0f / (a.nu * 3.0f + 0.5f);
+
+    float hold_mid[9], hold_up0[9], hold_up1[9];
+    float nxt[10];
+
+    // prologue: up pops of plane zb-1 and mid/up pops of plane zb
+    if (src_ok) {
+        float m[10], f[27];
+        load_moments<DT>(a, (int64_t)(zb - 1) * plane + scol, m);
+        recon27(m, f);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hold_up0[k] = f[kUp[k]];
+        load_moments<DT>(a, (int64_t)zb * plane + scol, m);
+        recon27(m, f);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; hold_up1[k] = f[kUp[k]]; }
+        load_moments<DT>(a, (int64_t)(zb + 1) * plane + scol, nxt);
+    }
+
+    for (int z = zb; z < ze; ++z) {
+        if (src_ok) {
+            float m[10], f[27];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) m[k] = nxt[k];
+            if (z + 2 <= ze) load_moments<DT>(a, (int64_t)(z + 2) * plane + scol, nxt);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                sp[kMid[k]][tid] = hold_mid[k];
+                sp[kUp[k]][tid] = hold_up0[k];
+                hold_up0[k] = hold_up1[k];
+            }
+            recon27(m, f);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                sp[kDown[k]][tid] = f[kDown[k]];
+                hold_up1[k] = f[kUp[k]];
+                hold_mid[k] = f[kMid[k]];
+            }
+        }
+        __syncthreads();
+        if (dst_ok) {
+            const int64_t pc = (int64_t)z * plane + pcol;
+            Acc acc;
+#define HL_PULL(I) accumulate<I>(acc, sp[I][cp - D3CX[I] - D3CY[I] * NX]);
+            HL_PULL(0) HL_PULL(1) HL_PULL(2) HL_PULL(3) HL_PULL(4) HL_PULL(5) HL_PULL(6) HL_PULL(7) HL_PULL(8)
+            HL_PULL(9) HL_PULL(10) HL_PULL(11) HL_PULL(12) HL_PULL(13) HL_PULL(14) HL_PULL(15) HL_PULL(16)
+            HL_PULL(17) HL_PULL(18) HL_PULL(19) HL_PULL(20) HL_PULL(21) HL_PULL(22) HL_PULL(23) HL_PULL(24)
+            HL_PULL(25) HL_PULL(26)
+#undef HL_PULL
+            float F[3] = {0.0f, 0.0f, 0.0f};
+            if (a.force) { F[0] = a.force[pc]; F[1] = a.force[a.stride + pc]; F[2] = a.force[2 * a.stride + pc]; }
+            float out[10];
+            collide27(acc, F, omega, out);
+            // faces: x, then y (this plane), then z (planes 0 and nz-1 from planes 1 and nz-2)
+            if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
+            else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
+            if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
+            else if (dy == 0) bc_face_closed(out, a.bc_type[3], a.bc_value[3], 1);
+            store_moments<DT>(a, (int64_t)z * plane + (int64_t)dy * nx + dx, out);
+            if (z == 1) {
+                float face[10];
+#pragma unroll
+                for (int k = 0; k < 10; ++k) face[k] = out[k];
+                bc_face_closed(face, a.bc_type[5], a.bc_value[5], 2);
+                store_moments<DT>(a, (int64_t)dy * nx + dx, face);
+            }
+            if (z == nz - 2) {
+                bc_face_closed(out, a.bc_type[4], a.bc_value[4], 2);
+                store_moments<DT>(a, (int64_t)(nz - 1) * plane + (int64_t)dy * nx + dx, out);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Tile shape of the D3Q27 kernel (set by hl_set_tiling / the autotuner).
+static TileCfg g_tile3d{32, 8, 16};
+
+bool tiling_supported(int lattice, int tx, int ty, int zc)
+{
+    if (lattice != HL_D3Q27 || zc < 1 || zc > 4096) return false;
+    return (tx == 32 && ty == 16) || (tx == 32 && ty == 8) || (tx == 64 && ty == 8) || (tx == 64 && ty == 4) ||
+           (tx == 16 && ty == 16);
+}
+
+void set_tiling(int lattice, TileCfg c)
+{
+    if (lattice == HL_D3Q27) g_tile3d = c;
+}
+
+TileCfg get_tiling(int lattice) { return g_tile3d; }
+
+namespace {
+
+TileCfg tile_config(const EnvArgs&) { return g_tile3d; }
+
+template <int DT, int TX, int TY>
+hipError_t launch_tiled(const EnvArgs& a, int zc, hipStream_t s)
+{
+    // every tile must hold at least 3 columns per axis for the face projection
+    if ((a.nx % TX) == 1 || (a.nx % TX) == 2 || (a.ny % TY) == 1 || (a.ny % TY) == 2) return hipErrorNotSupported;
+    if (zc < 1) return hipErrorInvalidValue;
+    constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+    dim3 grid((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (a.nz - 2 + zc - 1) / zc);
+    hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY>), grid, dim3(threads), 0, s, a, zc);
+    return hipGetLastError();
+}
+
+template <int DT>
+hipError_t dispatch_tile(const EnvArgs& a, hipStream_t s)
+{
+    const TileCfg c = tile_config(a);
+    if (c.tx == 32 && c.ty == 16) return launch_tiled<DT, 32, 16>(a, c.zc, s);
+    if (c.tx == 32 && c.ty == 8) return launch_tiled<DT, 32, 8>(a, c.zc, s);
+    if (c.tx == 64 && c.ty == 8) return launch_tiled<DT, 64, 8>(a, c.zc, s);
+    if (c.tx == 64 && c.ty == 4) return launch_tiled<DT, 64, 4>(a, c.zc, s);
+    if (c.tx == 16 && c.ty == 16) return launch_tiled<DT, 16, 16>(a, c.zc, s);
+    return hipErrorNotSupported;
+}
+
+}  // namespace
+
+void plan_tiled(int lattice, int dtype, EnvArgs& a)
+{
+    if (lattice != HL_D3Q27 || dtype == HL_F64 || a.mask == nullptr) return;
+    const TileCfg c = g_tile3d;
+    const int nbx = (a.nx + c.tx - 1) / c.tx, nby = (a.ny + c.ty - 1) / c.ty, nbz = (a.nz - 2 + c.zc - 1) / c.zc;
+    // a block holds a cut link only if its cells (or their interior
+    // projections) come within one cell of the mask box
+    int bx0 = nbx, bx1 = 0, by0 = nby, by1 = 0, bz0 = nbz, bz1 = 0;
+    for (int b = 0; b < nbx; ++b) {
+        const int x0 = b * c.tx;
+        if (x0 + c.tx >= a.box[0] - 1 && x0 - 1 <= a.box[1]) { bx0 = bx0 < b ? bx0 : b; bx1 = b + 1; }
+    }
+    for (int b = 0; b < nby; ++b) {
+        const int y0 = b * c.ty;
+        if (y0 + c.ty >= a.box[2] - 1 && y0 - 1 <= a.box[3]) { by0 = by0 < b ? by0 : b; by1 = b + 1; }
+    }
+    for (int b = 0; b < nbz; ++b) {
+        const int zb = 1 + b * c.zc, ze = zb + c.zc < a.nz - 1 ? zb + c.zc : a.nz - 1;
+        if (zb <= a.box[5] && ze - 1 >= a.box[4] - 1) { bz0 = bz0 < b ? bz0 : b; bz1 = b + 1; }
+    }
+    if (bx0 >= bx1 || by0 >= by1 || bz0 >= bz1) {
+        a.region[1] = a.region[0];  // nothing to hand over
+        return;
+    }
+    a.skip[0] = bx0; a.skip[1] = bx1; a.skip[2] = by0; a.skip[3] = by1; a.skip[4] = bz0; a.skip[5] = bz1;
+    const int zlo = 1 + bz0 * c.zc;
+    const int zhi = 1 + bz1 * c.zc < a.nz - 1 ? 1 + bz1 * c.zc : a.nz - 1;
+    a.region[0] = bx0 * c.tx;
+    a.region[1] = bx1 * c.tx < a.nx ? bx1 * c.tx : a.nx;
+    a.region[2] = by0 * c.ty;
+    a.region[3] = by1 * c.ty < a.ny ? by1 * c.ty : a.ny;
+    a.region[4] = zlo == 1 ? 0 : zlo;          // plane 0 is written with plane 1
+    a.region[5] = zhi == a.nz - 1 ? a.nz : zhi;  // plane nz-1 with plane nz-2
+}
+
+hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s)
+{
+    if (lattice != HL_D3Q27) return hipErrorNotSupported;
+    switch (dtype) {
+    case HL_F32: return dispatch_tile<HL_F32>(a, s);
+    case HL_F16: return dispatch_tile<HL_F16>(a, s);
+    case HL_BF16: return dispatch_tile<HL_BF16>(a, s);
+    default: return hipErrorNotSupported;  // FP64 runs the gather kernel (the oracle path)
+    }
+}
+
+}  // namespace hl

@@ -1,0 +1,283 @@
+"""One resident lattice on the GPU: moment buffers in PyTorch-ROCm tensors and
+the HIP step / force / voxelizer entry points of libhomelbm.
+
+This is the layer the reference-compatible facades (openhomelbm_amd.lbm and
+openhomelbm_amd.lbm3d) and the batched / served engines are built on.
+
+Layout: moments live in one tensor ``buf[2, nfields, stride]`` (two ping-pong
+states, structure-of-arrays, x fastest). ``cur`` names the state that holds
+the present time level; a step writes the other one and flips ``cur`` (the
+reference instead swaps array handles with the Swap_Mom kernels,
+envs/lbm/lbm_func.py:475-500).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+_TORCH_DTYPE = {N.HL_F32: torch.float32, N.HL_F64: torch.float64, N.HL_F16: torch.float16,
+                N.HL_BF16: torch.bfloat16}
+DTYPE_CODES = {"f32": N.HL_F32, "f64": N.HL_F64, "f16": N.HL_F16, "bf16": N.HL_BF16}
+
+#: default 16-bit storage scales: rho-1, u, S (values are O(1e-3..1e-1) in lattice units)
+DEFAULT_SCALE_2D = (256.0, 16.0, 16.0, 1024.0, 1024.0, 1024.0)
+DEFAULT_SCALE_3D = (256.0, 16.0, 16.0, 16.0, 1024.0, 1024.0, 1024.0, 1024.0, 1024.0, 1024.0)
+
+
+def _stream_ptr(stream=None) -> int:
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def capsule_box(caps: np.ndarray, shape_xyz: Sequence[int], pad: int = 1):
+    """Half-open cell box (x0, x1, y0, y1, z0, z1) covering every capsule."""
+    caps = np.asarray(caps, dtype=np.float64).reshape(-1, 8)
+    if caps.shape[0] == 0:
+        return (0, 0, 0, 0, 0, 0)
+    r = caps[:, 6]
+    lo = np.minimum(caps[:, 0:3], caps[:, 3:6]) - r[:, None]
+    hi = np.maximum(caps[:, 0:3], caps[:, 3:6]) + r[:, None]
+    box = []
+    for a in range(3):
+        n = shape_xyz[a]
+        b0 = max(0, int(math.floor(lo[:, a].min())) - pad)
+        b1 = min(n, int(math.ceil(hi[:, a].max())) + pad + 1)
+        box += [b0, max(b0, b1)]
+    return tuple(box)
+
+
+def union_box(a, b):
+    if a is None or a[0] >= a[1] or a[2] >= a[3] or a[4] >= a[5]:
+        return tuple(b)
+    if b[0] >= b[1] or b[2] >= b[3] or b[4] >= b[5]:
+        return tuple(a)
+    return (min(a[0], b[0]), max(a[1], b[1]), min(a[2], b[2]), max(a[3], b[3]), min(a[4], b[4]), max(a[5], b[5]))
+
+
+class Lattice:
+    """A D2Q9 or D3Q27 lattice resident in HBM.
+
+    Args:
+        lattice: 9 or 27.
+        nx, ny, nz: extents (nz = 1 for D2Q9).
+        dtype: moment storage, "f32" (default), "f64", "f16" or "bf16".
+        nsolid: number of rigid solids in the pose table.
+        device: torch device (default the current CUDA device).
+    """
+
+    def __init__(self, lattice: int, nx: int, ny: int, nz: int = 1, dtype: str = "f32", nsolid: int = 0,
+                 device: Optional[torch.device] = None, scale: Optional[Sequence[float]] = None):
+        if lattice not in (9, 27):
+            raise ValueError(f"lattice must be 9 or 27, got {lattice}")
+        if lattice == 9 and nz != 1:
+            raise ValueError("D2Q9 lattices have nz == 1")
+        if min(nx, ny) < 3 or (lattice == 27 and nz < 3):
+            raise ValueError("every lattice extent must be >= 3")
+        if dtype not in DTYPE_CODES:
+            raise ValueError(f"dtype must be one of {sorted(DTYPE_CODES)}")
+        if not 0 <= nsolid <= N.HL_MAX_SOLIDS:
+            raise ValueError(f"nsolid must be in [0, {N.HL_MAX_SOLIDS}]")
+        self.lib = N.load()
+        self.lattice = lattice
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dtype = dtype
+        self.dtype_code = DTYPE_CODES[dtype]
+        self.nfields = 6 if lattice == 9 else 10
+        self.ncell = nx * ny * nz
+        self.stride = (self.ncell + 63) // 64 * 64
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.buf = torch.zeros((2, self.nfields, self.stride), dtype=_TORCH_DTYPE[self.dtype_code], device=self.device)
+        self.cur = 0
+        self.nsolid = nsolid
+        self.pose_width = 8 if lattice == 9 else 24
+        self.solids = torch.zeros((max(nsolid, 1), self.pose_width), dtype=torch.float32, device=self.device)
+        self._init_identity_poses()
+        self.mask: Optional[torch.Tensor] = None
+        self.mask_box = (0, 0, 0, 0, 0, 0)
+        self._mask_dirty = None
+        self.force: Optional[torch.Tensor] = None
+        self.nu = 0.01 if lattice == 9 else 0.1
+        self.bc_type = [1] * 6
+        self.bc_value = [[0.0, 0.0, 0.0] for _ in range(6)]
+        if scale is None:
+            scale = DEFAULT_SCALE_2D if lattice == 9 else DEFAULT_SCALE_3D
+        self.scale = list(scale) + [1.0] * (10 - len(scale))
+        self._forces_out = torch.zeros((max(nsolid, 1), 6), dtype=torch.float64, device=self.device)
+        self._mass_out = torch.zeros(1, dtype=torch.float64, device=self.device)
+
+    # ------------------------------------------------------------ helpers
+    @property
+    def shape(self):
+        """Array shape of one field in (z,) y, x order."""
+        return (self.ny, self.nx) if self.lattice == 9 else (self.nz, self.ny, self.nx)
+
+    def _init_identity_poses(self):
+        p = torch.zeros((max(self.nsolid, 1), self.pose_width), dtype=torch.float32)
+        if self.lattice == 9:
+            p[:, 2] = 1.0
+            p[:, 6] = 1.0
+        else:
+            eye = torch.eye(3).reshape(9)
+            p[:, 3:12] = eye
+            p[:, 15:24] = eye
+        self.solids.copy_(p)
+
+    def env(self) -> N.HLEnv:
+        e = N.HLEnv()
+        e.lattice = self.lattice
+        e.dtype = self.dtype_code
+        e.nx, e.ny, e.nz = self.nx, self.ny, self.nz
+        e.nsolid = self.nsolid
+        e.stride = self.stride
+        e.moments[0] = self.buf[0].data_ptr()
+        e.moments[1] = self.buf[1].data_ptr()
+        if self.mask is not None:
+            e.mask = self.mask.data_ptr()
+            for k in range(6):
+                e.mask_box[k] = int(self.mask_box[k])
+        if self.force is not None:
+            e.force = self.force.data_ptr()
+        e.solids = self.solids.data_ptr() if self.nsolid > 0 else None
+        e.nu = float(self.nu)
+        for k in range(6):
+            e.bc_type[k] = int(self.bc_type[k])
+            for a in range(3):
+                e.bc_value[k][a] = float(self.bc_value[k][a])
+        for f in range(10):
+            e.scale[f] = float(self.scale[f])
+        return e
+
+    def fields(self, which: Optional[int] = None) -> torch.Tensor:
+        """Device view of one state as (nfields, (z,) y, x)."""
+        b = self.cur if which is None else which
+        return self.buf[b, :, : self.ncell].view((self.nfields,) + self.shape)
+
+    # ------------------------------------------------------------- state
+    def set_state(self, m: np.ndarray) -> None:
+        """Load host moments (oracle layout) into the current state (both buffers)."""
+        m = np.asarray(m)
+        if m.shape != (self.nfields,) + self.shape:
+            raise ValueError(f"state shape {m.shape} != {(self.nfields,) + self.shape}")
+        t = torch.from_numpy(np.ascontiguousarray(m, dtype=np.float64))
+        if self.dtype in ("f16", "bf16"):
+            t = t.clone()
+            t[0] -= 1.0
+            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64).view((-1,) + (1,) * len(self.shape))
+            t = (t * sc).to(torch.float32)
+        dev = t.to(device=self.device, dtype=self.buf.dtype)
+        for b in (0, 1):
+            self.buf[b, :, : self.ncell].copy_(dev.reshape(self.nfields, self.ncell))
+
+    def get_state(self, dtype=np.float64) -> np.ndarray:
+        """Host copy of the current moments, decoded, in oracle layout."""
+        t = self.fields().to(torch.float64 if self.dtype == "f64" else torch.float32).cpu().to(torch.float64)
+        if self.dtype in ("f16", "bf16"):
+            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64).view((-1,) + (1,) * len(self.shape))
+            t = t / sc
+            t[0] += 1.0
+        return t.numpy().astype(dtype)
+
+    def init_uniform(self, u: Sequence[float] = (0.0, 0.0, 0.0)) -> None:
+        """set_uniform_flow into both buffers (karman_lbm_env.py:15-53)."""
+        u = list(u) + [0.0] * (3 - len(u))
+        e = self.env()
+        for b in (0, 1):
+            N.check(self.lib.hl_init_uniform(C.byref(e), b, u[0], u[1], u[2], _stream_ptr()), "hl_init_uniform")
+
+    # -------------------------------------------------------------- step
+    def step(self, nsteps: int = 1, kernel: int = N.HL_KERNEL_AUTO, force_history: Optional[torch.Tensor] = None,
+             stream=None) -> None:
+        """Advance nsteps. force_history (device FP64 [nsteps, nsolid, 6]) receives
+        the force/torque of the state each step reads."""
+        e = self.env()
+        fh = None
+        if force_history is not None:
+            if (force_history.dtype != torch.float64 or not force_history.is_contiguous()
+                    or force_history.numel() < nsteps * self.nsolid * 6 or force_history.device != self.device):
+                raise ValueError("force_history must be a contiguous FP64 device tensor [nsteps, nsolid, 6]")
+            fh = force_history.data_ptr()
+        N.check(self.lib.hl_step(C.byref(e), self.cur, int(nsteps), int(kernel), fh, _stream_ptr(stream)), "hl_step")
+        self.cur ^= nsteps & 1
+
+    # ------------------------------------------------------------ bodies
+    def set_mask(self, mask: Optional[np.ndarray], box=None) -> None:
+        """Install a host mask ((z,) y, x uint8); box defaults to its nonzero bounding box."""
+        if mask is None:
+            self.mask = None
+            self.mask_box = (0, 0, 0, 0, 0, 0)
+            return
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        if m.shape != self.shape:
+            raise ValueError(f"mask shape {m.shape} != {self.shape}")
+        if box is None:
+            nz = np.nonzero(m)
+            if len(nz[0]) == 0:
+                box = (0, 0, 0, 0, 0, 0)
+            else:
+                ax = nz[::-1]  # x, y(, z)
+                box = []
+                for a in range(3):
+                    if a < len(ax):
+                        box += [int(ax[a].min()), int(ax[a].max()) + 1]
+                    else:
+                        box += [0, 1]
+        self.mask = torch.from_numpy(m.reshape(-1)).to(self.device)
+        self.mask_box = tuple(box)
+        self._mask_dirty = tuple(box)
+
+    def voxelize(self, caps: np.ndarray, box=None, stream=None) -> None:
+        """Rebuild the mask from capsules [n, 8] (ax, ay, az, bx, by, bz, r, solid_id) on the GPU."""
+        caps = np.ascontiguousarray(caps, dtype=np.float32).reshape(-1, 8)
+        if caps.shape[0] and (caps[:, 7].max() >= min(self.nsolid, 255) or caps[:, 7].min() < 0):
+            raise ValueError("capsule solid ids must be in [0, nsolid)")
+        if self.mask is None:
+            self.mask = torch.zeros(self.ncell, dtype=torch.uint8, device=self.device)
+        if box is None:
+            box = capsule_box(caps, (self.nx, self.ny, self.nz))
+        write = union_box(self._mask_dirty, box)
+        self._caps_dev = torch.from_numpy(caps).to(self.device)
+        b = (C.c_int32 * 6)(*write)
+        e = self.env()
+        N.check(self.lib.hl_voxelize(C.byref(e), self.mask.data_ptr(), self._caps_dev.data_ptr(), caps.shape[0], b,
+                                     _stream_ptr(stream)), "hl_voxelize")
+        self.mask_box = tuple(box)
+        self._mask_dirty = tuple(box)
+
+    def set_poses(self, poses: np.ndarray) -> None:
+        """Pose table [nsolid, 8 | 24] (see include/homelbm.h)."""
+        p = np.ascontiguousarray(poses, dtype=np.float32).reshape(self.nsolid, self.pose_width)
+        self.solids[: self.nsolid].copy_(torch.from_numpy(p))
+
+    def set_force(self, force: Optional[np.ndarray]) -> None:
+        if force is None:
+            self.force = None
+            return
+        d = 2 if self.lattice == 9 else 3
+        f = np.ascontiguousarray(force, dtype=np.float32).reshape(d, self.ncell)
+        t = torch.zeros((d, self.stride), dtype=torch.float32)
+        t[:, : self.ncell] = torch.from_numpy(f)
+        self.force = t.to(self.device)
+
+    def forces(self, stream=None) -> torch.Tensor:
+        """Device FP64 [nsolid, 6] force/torque of the current state."""
+        e = self.env()
+        N.check(self.lib.hl_forces(C.byref(e), self.cur, self._forces_out.data_ptr(), _stream_ptr(stream)), "hl_forces")
+        return self._forces_out[: self.nsolid]
+
+    def mass(self, stream=None) -> float:
+        e = self.env()
+        N.check(self.lib.hl_mass(C.byref(e), self.cur, self._mass_out.data_ptr(), _stream_ptr(stream)), "hl_mass")
+        return float(self._mass_out.item())
+
+    # --------------------------------------------------------- metrics
+    @property
+    def bytes_per_cell_update(self) -> int:
+        """Algorithmic HBM bytes per cell update: every field read once, written once."""
+        size = {"f32": 4, "f64": 8, "f16": 2, "bf16": 2}[self.dtype]
+        return 2 * self.nfields * size

@@ -1,0 +1,70 @@
+"""The C-ABI library loads and exports every declared entry point; argument
+errors are reported without touching the GPU (runs on a CPU-only machine)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from openhomelbm_amd import _native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:const\s+)?[\w\*]+\s+\**(hl_\w+)\s*\(", text, re.M))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = N.load()
+    declared = _declared()
+    assert {"hl_step", "hl_forces", "hl_voxelize", "hl_mass", "hl_init_uniform"} <= declared
+    for name in sorted(declared):
+        assert hasattr(lib, name), name
+    assert set(N.EXPORTS) <= declared | {"hl_last_error"}
+
+
+def test_env_layout_matches_binding():
+    lib = N.load()
+    assert lib.hl_env_bytes() == C.sizeof(N.HLEnv)
+    assert lib.hl_version() >= 100
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("lattice", 5, "lattice"),
+    ("nx", 2, "extent"),
+    ("nu", -1.0, "viscosity"),
+    ("dtype", 9, "dtype"),
+])
+def test_invalid_env_is_rejected_before_any_gpu_call(field, value, msg):
+    lib = N.load()
+    e = N.HLEnv()
+    e.lattice, e.dtype, e.nx, e.ny, e.nz = 27, 0, 8, 8, 8
+    e.stride = 512
+    e.moments[0] = 0x1000
+    e.moments[1] = 0x2000
+    e.nu = 0.1
+    setattr(e, field, value)
+    rc = lib.hl_step(C.byref(e), 0, 1, 0, None, None)
+    assert rc == -1
+    assert msg in lib.hl_last_error().decode()
+    with pytest.raises(N.HLError):
+        N.check(rc, "hl_step")
+
+
+def test_bad_bc_type_rejected():
+    lib = N.load()
+    e = N.HLEnv()
+    e.lattice, e.dtype, e.nx, e.ny, e.nz = 9, 0, 8, 8, 1
+    e.stride = 64
+    e.moments[0] = 0x1000
+    e.moments[1] = 0x2000
+    e.nu = 0.1
+    e.bc_type[2] = 7
+    assert lib.hl_step(C.byref(e), 0, 1, 0, None, None) == -1
+    assert "bc_type" in lib.hl_last_error().decode()
