@@ -110,6 +110,15 @@ int32_t hl_voxelize(const hl_env* env, uint8_t* mask, const float* caps, int32_t
 int32_t hl_set_tiling(int32_t lattice, int32_t tx, int32_t ty, int32_t zc);
 int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc);
 
+/* Even-odd polygon voxelizer (D2Q9): device verts [npoly][maxv][2] closed
+ * polygons in lattice coordinates, counts [npoly]; mask = index + 1 of the
+ * lowest polygon containing the cell centre, 0 elsewhere in the box.
+ * Restates the inside test of get_solid_boundary_img (lbm_func.py:804-864);
+ * the mask then drives the wall rule that get_cutcell (lbm_func.py:545-607)
+ * drives in the reference. */
+int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* verts, const int32_t* counts,
+                             int32_t npoly, int32_t maxv, const int32_t box[6], void* stream);
+
 /* FP64 sum of rho over all cells of state `cur` into device out[0]. */
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 

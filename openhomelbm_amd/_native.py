@@ -17,7 +17,7 @@ HL_MAX_SOLIDS = 64
 
 #: every entry point declared in include/homelbm.h
 EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces", "hl_init_uniform",
-           "hl_voxelize", "hl_mass", "hl_set_tiling", "hl_get_tiling")
+           "hl_voxelize", "hl_voxelize_polygons", "hl_mass", "hl_set_tiling", "hl_get_tiling")
 
 
 class HLEnv(C.Structure):
@@ -70,11 +70,12 @@ def load():
     L.hl_forces.argtypes = [env_p, I32, P, P]
     L.hl_init_uniform.argtypes = [env_p, I32, C.c_float, C.c_float, C.c_float, P]
     L.hl_voxelize.argtypes = [env_p, P, P, I32, P, P]
+    L.hl_voxelize_polygons.argtypes = [env_p, P, P, P, I32, I32, P, P]
     L.hl_mass.argtypes = [env_p, I32, P, P]
     L.hl_set_tiling.argtypes = [I32, I32, I32, I32]
     L.hl_get_tiling.argtypes = [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
-    for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_mass", "hl_set_tiling",
-                 "hl_get_tiling"):
+    for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
+                 "hl_set_tiling", "hl_get_tiling"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

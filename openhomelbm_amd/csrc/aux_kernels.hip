@@ -8,6 +8,7 @@
 // lbm_func_3d.py:510-511).
 #pragma clang fp contract(off)
 
+#include "gather_post.h"
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -112,30 +113,76 @@ hipError_t voxelize(int lattice, const EnvArgs& a, uint8_t* mask, const float* c
     return hipGetLastError();
 }
 
-// -------------------------------------------------------------- forces
+// Even-odd +x ray test of get_solid_boundary_img (envs/lbm/lbm_func.py:804-864,
+// border thickness 0) for closed 2D polygons in lattice coordinates; the
+// lowest-index polygon containing the cell centre sets mask = index + 1.
+// FP32 without contraction, same order as the CPU voxelizer.
+__global__ void k_voxelize_polygons(uint8_t* mask, int nx, const float* verts, const int32_t* counts, int npoly,
+                                    int maxv, int bx0, int by0, int bw, int bh)
+{
+    const int64_t n = (int64_t)bw * bh;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int x = bx0 + (int)(t % bw), y = by0 + (int)(t / bw);
+        const float px = (float)x, py = (float)y;
+        uint8_t v = 0;
+        for (int k = 0; k < npoly && !v; ++k) {
+            const float* P = verts + 2 * (int64_t)maxv * k;
+            const int nv = counts[k];
+            int crossings = 0;
+            for (int j = 0; j < nv; ++j) {
+                const float x1 = P[2 * j], y1 = P[2 * j + 1];
+                const int j2 = (j + 1) % nv;
+                const float x2 = P[2 * j2], y2 = P[2 * j2 + 1];
+                if ((y1 > py) != (y2 > py)) {
+                    const float ix = x1 + (py - y1) * (x2 - x1) / (y2 - y1);
+                    if (px < ix) ++crossings;
+                }
+            }
+            if (crossings & 1) v = (uint8_t)(k + 1);
+        }
+        mask[(int64_t)x + (int64_t)nx * y] = v;
+    }
+}
+
+hipError_t voxelize_polygons(const EnvArgs& a, uint8_t* mask, const float* verts, const int32_t* counts, int npoly,
+                             int maxv, const int32_t box[6], hipStream_t s)
+{
+    const int bw = box[1] - box[0], bh = box[3] - box[2];
+    if (bw <= 0 || bh <= 0) return hipSuccess;
+    const int64_t n = (int64_t)bw * bh;
+    const int blocks = (int)((n + 255) / 256 < 8192 ? (n + 255) / 256 : 8192);
+    hipLaunchKernelGGL(k_voxelize_polygons, blocks, 256, 0, s, mask, a.nx, verts, counts, npoly, maxv, box[0], box[2],
+                       bw, bh);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------- near-body fix-up
 
 struct Region {
     int x0, y0, z0, w, h, d;
     __host__ __device__ int64_t n() const { return (int64_t)w * h * d; }
 };
 
-// Interior cells within one cell of the mask box: the only cells with cut links.
-static Region force_region(const EnvArgs& a, int lattice)
+// Cells whose update can involve a cut link: interior cells within one cell
+// of the mask box, plus the face cells that project onto them.
+static void axis_range(int b0, int b1, int n, int& lo, int& hi)
 {
-    Region r;
-    const int x0 = a.box[0] - 1 > 1 ? a.box[0] - 1 : 1;
-    const int x1 = a.box[1] + 1 < a.nx - 1 ? a.box[1] + 1 : a.nx - 1;
-    const int y0 = a.box[2] - 1 > 1 ? a.box[2] - 1 : 1;
-    const int y1 = a.box[3] + 1 < a.ny - 1 ? a.box[3] + 1 : a.ny - 1;
-    int z0 = 0, z1 = 1;
-    if (lattice == HL_D3Q27) {
-        z0 = a.box[4] - 1 > 1 ? a.box[4] - 1 : 1;
-        z1 = a.box[5] + 1 < a.nz - 1 ? a.box[5] + 1 : a.nz - 1;
-    }
+    lo = b0 - 1 > 1 ? b0 - 1 : 1;
+    hi = b1 + 1 < n - 1 ? b1 + 1 : n - 1;
+    if (lo >= hi) { lo = hi = 0; return; }
+    if (lo == 1) lo = 0;
+    if (hi == n - 1) hi = n;
+}
+
+static Region fixup_region(const EnvArgs& a, int lattice)
+{
+    Region r{0, 0, 0, 0, 0, 0};
+    int x0, x1, y0, y1, z0 = 0, z1 = 1;
+    axis_range(a.box[0], a.box[1], a.nx, x0, x1);
+    axis_range(a.box[2], a.box[3], a.ny, y0, y1);
+    if (lattice == HL_D3Q27) axis_range(a.box[4], a.box[5], a.nz, z0, z1);
     r.x0 = x0; r.y0 = y0; r.z0 = z0;
-    r.w = x1 > x0 ? x1 - x0 : 0;
-    r.h = y1 > y0 ? y1 - y0 : 0;
-    r.d = z1 > z0 ? z1 - z0 : 0;
+    r.w = x1 - x0; r.h = y1 - y0; r.d = z1 - z0;
     return r;
 }
 
@@ -196,41 +243,81 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
     }
 }
 
+// Near-body fix-up. For every cell of the region whose projection onto the
+// interior has a cut link it (a) recomputes the update in the reference's
+// operation order with the wall rule (when `store`), overwriting what the
+// tiled kernel wrote there, and (b) accumulates the momentum-exchange force
+// and torque of interior cells into per-block FP64 partials (when
+// `partial`): per solid, a wavefront butterfly sum, then the block's waves in
+// index order. Cells without cut links are left untouched.
 template <typename T, int DT, int Q>
-__global__ void __launch_bounds__(kReduceBlock) k_forces(EnvArgs a, Region rg, double* partial)
+__global__ void __launch_bounds__(kReduceBlock) k_fixup(EnvArgs a, Region rg, int store, double* partial)
 {
     __shared__ double acc[kWaves][HL_MAX_SOLIDS][6];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = threadIdx.x; k < kWaves * HL_MAX_SOLIDS * 6; k += kReduceBlock) (&acc[0][0][0])[k] = 0.0;
+    if (partial)
+        for (int k = threadIdx.x; k < kWaves * HL_MAX_SOLIDS * 6; k += kReduceBlock) (&acc[0][0][0])[k] = 0.0;
     __syncthreads();
     using ST = typename Storage<DT>::type;
     const ST* src = static_cast<const ST*>(a.src);
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t n = rg.n();
-    const int nx = a.nx, ny = a.ny;
-    // uniform trip count across the block so every lane takes part in the wave reductions
     const int64_t per_iter = (int64_t)gridDim.x * kReduceBlock;
-    const int64_t iters = (n + per_iter - 1) / per_iter;
+    const int64_t iters = (n + per_iter - 1) / per_iter;  // uniform trip count for the wave reductions
     for (int64_t it = 0; it < iters; ++it) {
         const int64_t t = it * per_iter + (int64_t)blockIdx.x * kReduceBlock + threadIdx.x;
         const bool valid = t < n;
-        int x = 0, y = 0, z = 0;
-        int64_t c = 0;
-        uint64_t mine = 0;
+        int x = 0, y = 0, z = 0, px = 0, py = 0, pz = 0;
+        uint32_t cut = 0;
         int mval_c = 0;
         if (valid) {
             x = rg.x0 + (int)(t % rg.w);
             y = rg.y0 + (int)((t / rg.w) % rg.h);
             z = rg.z0 + (int)(t / ((int64_t)rg.w * rg.h));
-            c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
-            mval_c = a.mask[c];
+            px = min(max(x, 1), nx - 2);
+            py = min(max(y, 1), ny - 2);
+            pz = Q == 9 ? 0 : min(max(z, 1), nz - 2);
+            if (in_box_halo(a, px, py, pz)) {
+                const int64_t c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
+                mval_c = a.mask[c];
 #pragma unroll
-            for (int i = 0; i < Q; ++i) {
-                const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
-                const int mval_s = a.mask[(int64_t)(x - cx) + (int64_t)nx * ((int64_t)(y - cy) + (int64_t)ny * (z - cz))];
-                if ((mval_s != 0) != (mval_c != 0)) {
-                    const int sid = mval_c ? mval_c - 1 : mval_s - 1;
-                    if (sid < a.nsolid) mine |= 1ull << sid;
+                for (int i = 1; i < Q; ++i) {
+                    const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+                    const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
+                    if ((mv != 0) != (mval_c != 0)) cut |= 1u << i;
                 }
+            }
+        }
+        if (store && cut) {
+            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+            ST* dst = static_cast<ST*>(a.dst);
+            if (Q == 9) {
+                T m[6];
+                d2q9_post<T, DT>(a, px, py, m);
+                if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
+#pragma unroll
+                for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+            } else {
+                T m[10];
+                d3q27_post<T, DT>(a, px, py, pz, m);
+                if (px != x || py != y || pz != z) bc_chain<T, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
+#pragma unroll
+                for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+            }
+        }
+        if (!partial) continue;
+        // forces: interior cells only (a face cell repeats its projection)
+        uint64_t mine = 0;
+        const bool interior = valid && px == x && py == y && pz == z;
+        const int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+        if (interior && cut) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i) {
+                if (!(cut & (1u << i))) continue;
+                const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+                const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
+                const int sid = mval_c ? mval_c - 1 : mv - 1;
+                if (sid < a.nsolid) mine |= 1ull << sid;
             }
         }
         uint64_t uni = wave_or(mine);
@@ -245,11 +332,11 @@ __global__ void __launch_bounds__(kReduceBlock) k_forces(EnvArgs a, Region rg, d
             uni &= uni - 1;
             double sum[6] = {0, 0, 0, 0, 0, 0};
             if ((mine >> s) & 1ull) {
-                for (int i = 0; i < Q; ++i) {
+                for (int i = 1; i < Q; ++i) {
+                    if (!(cut & (1u << i))) continue;
                     const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
-                    const int mval_s = a.mask[(int64_t)(x - cx) + (int64_t)nx * ((int64_t)(y - cy) + (int64_t)ny * (z - cz))];
-                    if ((mval_s != 0) == (mval_c != 0)) continue;
-                    const int sid = mval_c ? mval_c - 1 : mval_s - 1;
+                    const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
+                    const int sid = mval_c ? mval_c - 1 : mv - 1;
                     if (sid != s) continue;
                     T F[6];
                     link_force<T, DT, Q>(a, mc, x, y, z, i, sid, F);
@@ -265,6 +352,7 @@ __global__ void __launch_bounds__(kReduceBlock) k_forces(EnvArgs a, Region rg, d
             }
         }
     }
+    if (!partial) return;
     __syncthreads();
     for (int k = threadIdx.x; k < a.nsolid * 6; k += kReduceBlock) {
         const int s = k / 6, q = k % 6;
@@ -274,21 +362,15 @@ __global__ void __launch_bounds__(kReduceBlock) k_forces(EnvArgs a, Region rg, d
     }
 }
 
-__global__ void k_sum_partials(const double* partial, int nblocks, int width, double* out)
+// out[k] = sum over blocks b of partial[b][k]: each lane sums a fixed strided
+// subset in order, then a wavefront butterfly; one wave per output.
+__global__ void __launch_bounds__(64) k_sum_partials(const double* partial, int nblocks, int width, double* out)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= width) return;
+    const int k = blockIdx.x;
     double v = 0.0;
-    for (int b = 0; b < nblocks; ++b) v += partial[(int64_t)b * width + k];
-    out[k] = v;
-}
-
-template <typename T, int DT>
-static hipError_t launch_forces(int lattice, const EnvArgs& a, const Region& rg, double* ws, int nb, hipStream_t s)
-{
-    if (lattice == HL_D2Q9) hipLaunchKernelGGL((k_forces<T, DT, 9>), nb, kReduceBlock, 0, s, a, rg, ws);
-    else hipLaunchKernelGGL((k_forces<T, DT, 27>), nb, kReduceBlock, 0, s, a, rg, ws);
-    return hipGetLastError();
+    for (int b = threadIdx.x; b < nblocks; b += 64) v += partial[(int64_t)b * width + k];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) out[k] = v;
 }
 
 __global__ void k_zero(double* out, int n)
@@ -297,28 +379,46 @@ __global__ void k_zero(double* out, int n)
     if (k < n) out[k] = 0.0;
 }
 
-hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s)
+template <typename T, int DT>
+static hipError_t launch_fixup(int lattice, const EnvArgs& a, const Region& rg, int store, double* part, int nb,
+                               hipStream_t s)
+{
+    if (lattice == HL_D2Q9) hipLaunchKernelGGL((k_fixup<T, DT, 9>), nb, kReduceBlock, 0, s, a, rg, store, part);
+    else hipLaunchKernelGGL((k_fixup<T, DT, 27>), nb, kReduceBlock, 0, s, a, rg, store, part);
+    return hipGetLastError();
+}
+
+hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* out, void* ws, hipStream_t s)
 {
     const int width = a.nsolid * 6;
-    if (width == 0) return hipSuccess;
-    const Region rg = force_region(a, lattice);
+    const bool want_forces = out != nullptr && width > 0;
+    const Region rg = fixup_region(a, lattice);
     if (a.mask == nullptr || rg.n() == 0) {
-        hipLaunchKernelGGL(k_zero, (width + 63) / 64, 64, 0, s, out, width);
-        return hipGetLastError();
+        if (want_forces) {
+            hipLaunchKernelGGL(k_zero, (width + 63) / 64, 64, 0, s, out, width);
+            return hipGetLastError();
+        }
+        return hipSuccess;
     }
+    if (!store && !want_forces) return hipSuccess;
     const int nb = reduce_blocks(rg.n());
-    double* part = static_cast<double*>(ws);
+    double* part = want_forces ? static_cast<double*>(ws) : nullptr;
     hipError_t e;
     switch (dtype) {
-    case HL_F32: e = launch_forces<float, HL_F32>(lattice, a, rg, part, nb, s); break;
-    case HL_F64: e = launch_forces<double, HL_F64>(lattice, a, rg, part, nb, s); break;
-    case HL_F16: e = launch_forces<float, HL_F16>(lattice, a, rg, part, nb, s); break;
-    case HL_BF16: e = launch_forces<float, HL_BF16>(lattice, a, rg, part, nb, s); break;
+    case HL_F32: e = launch_fixup<float, HL_F32>(lattice, a, rg, store, part, nb, s); break;
+    case HL_F64: e = launch_fixup<double, HL_F64>(lattice, a, rg, store, part, nb, s); break;
+    case HL_F16: e = launch_fixup<float, HL_F16>(lattice, a, rg, store, part, nb, s); break;
+    case HL_BF16: e = launch_fixup<float, HL_BF16>(lattice, a, rg, store, part, nb, s); break;
     default: return hipErrorInvalidValue;
     }
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_sum_partials, (width + 63) / 64, 64, 0, s, part, nb, width, out);
+    if (e != hipSuccess || !want_forces) return e;
+    hipLaunchKernelGGL(k_sum_partials, width, 64, 0, s, part, nb, width, out);
     return hipGetLastError();
+}
+
+hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s)
+{
+    return fixup(lattice, dtype, a, false, out, ws, s);
 }
 
 // ---------------------------------------------------------------- mass

@@ -91,7 +91,6 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
         a.inv_scale[f] = sixteen ? 1.0f / e->scale[f] : 1.0f;
     }
     a.region[0] = 0; a.region[1] = e->nx; a.region[2] = 0; a.region[3] = e->ny; a.region[4] = 0; a.region[5] = e->nz;
-    for (int k = 0; k < 6; ++k) a.skip[k] = 0;
     return a;
 }
 
@@ -198,21 +197,21 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
     }
     for (int k = 0; k < nsteps; ++k) {
         hl::EnvArgs a = make_args(env, cur);
+        double* fout = ws ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
         hipError_t e = hipErrorNotSupported;
-        if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) hl::plan_tiled(env->lattice, env->dtype, a);
-        if (ws) {
-            e = hl::forces(env->lattice, env->dtype, a, force_history + (int64_t)k * env->nsolid * 6, ws, s);
-            if (e != hipSuccess) return hip_fail(e, "forces kernel launch");
-            e = hipErrorNotSupported;
-        }
+        bool fixed = false;
         if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) {
-            // the tiled kernel leaves blocks near the body mask to the gather kernel
             e = hl::step_tiled(env->lattice, env->dtype, a, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
                 return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
-            if (e == hipSuccess && a.region[0] < a.region[1]) e = hl::step_gather(env->lattice, env->dtype, a, s);
+            if (e == hipSuccess) {
+                // cells next to bodies: reference-order wall rule, plus the forces
+                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, s);
+                fixed = true;
+            }
         }
         if (e == hipErrorNotSupported) e = hl::step_gather(env->lattice, env->dtype, a, s);
+        if (e == hipSuccess && !fixed && fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
         if (e != hipSuccess) return hip_fail(e, "step kernel launch");
         cur ^= 1;
     }
@@ -276,6 +275,23 @@ int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc)
     if (!tx || !ty || !zc) return fail(HL_E_INVALID, "NULL output");
     hl::TileCfg c = hl::get_tiling(lattice);
     *tx = c.tx; *ty = c.ty; *zc = c.zc;
+    return HL_OK;
+}
+
+int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* verts, const int32_t* counts,
+                             int32_t npoly, int32_t maxv, const int32_t box[6], void* stream)
+{
+    int32_t rc = validate(env, false);
+    if (rc) return rc;
+    if (env->lattice != HL_D2Q9) return fail(HL_E_UNSUPPORTED, "polygon voxelizer is D2Q9 only");
+    if (!mask || !box) return fail(HL_E_INVALID, "mask or box is NULL");
+    if (npoly < 0 || npoly > 255 || maxv < 0 || (npoly > 0 && (!verts || !counts)))
+        return fail(HL_E_INVALID, "bad polygon table (npoly %d, maxv %d)", npoly, maxv);
+    if (box[0] < 0 || box[1] > env->nx || box[2] < 0 || box[3] > env->ny)
+        return fail(HL_E_INVALID, "voxelize box outside the grid");
+    hl::EnvArgs a = make_args(env, 0);
+    hipError_t e = hl::voxelize_polygons(a, mask, verts, counts, npoly, maxv, box, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "polygon voxelize kernel launch");
     return HL_OK;
 }
 

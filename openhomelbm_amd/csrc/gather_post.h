@@ -1,0 +1,171 @@
+// Post-collision moments of one interior cell in the reference's operation
+// order (stream_and_collide, envs/lbm/lbm_func.py:322-467, and
+// stream_and_collide_3d, envs/lbm3d/lbm_func_3d.py:364-604), with the mask
+// wall rule on cut links. Shared by the gather step kernel and the near-body
+// fix-up kernel. Contraction is off so the FP32 build matches a literal FP32
+// evaluation.
+#pragma once
+#pragma clang fp contract(off)
+
+#include "lbm_device.h"
+
+namespace hl {
+
+template <typename T, int DT>
+__device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[6])
+{
+    using S = Storage<DT>;
+    using ST = typename S::type;
+    const ST* src = static_cast<const ST*>(a.src);
+    const int64_t st = a.stride;
+    const int nx = a.nx;
+    const int64_t c = (int64_t)x + (int64_t)nx * y;
+    T mc[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) mc[f] = S::template load<T>(src, f * st + c, f, a.inv_scale[f]);
+    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, 0);
+    const int mval_c = masked ? a.mask[c] : 0;
+    T pop[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const int64_t s = (int64_t)(x - D2CX[i]) + (int64_t)nx * (y - D2CY[i]);
+        int cut = 0, sid = -1;
+        if (masked) {
+            const int mval_s = a.mask[s];
+            if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
+        }
+        if (cut) {
+            T up[2], r[2];
+            wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+            T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+            T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+            T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+            pop[i] = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+        } else {
+            T m[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+            pop[i] = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+        }
+        if (pop[i] < T(0)) pop[i] = T(0);
+    }
+    T Fx = 0, Fy = 0;
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[st + c]); }
+    T rhoVar = pop[0] + pop[1] + pop[2] + pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    T invRho = T(1) / rhoVar;
+    T ux = (pop[1] - pop[2] + pop[5] - pop[6] - pop[7] + pop[8] + T(0.5) * Fx) * invRho;
+    T uy = (pop[3] - pop[4] + pop[5] + pop[6] - pop[7] - pop[8] + T(0.5) * Fy) * invRho;
+    T pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+    T piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+    T pixy = pop[5] + pop[7] - pop[6] - pop[8];
+    T Omega = T(1) / (T(a.nu) * T(3) + T(0.5));
+    T Ru2 = rhoVar * ux * ux;
+    T Rv2 = rhoVar * uy * uy;
+    T pixx_part = (pixx - piyy) / T(2);
+    T piyy_part = (piyy - pixx) / T(2);
+    const T one = 1;
+    pixy = (one - Omega) * pixy + ux * uy * rhoVar * Omega + (one - T(0.5) * Omega) * (Fy * ux + Fx * uy);
+    T pixx_n = rhoVar / T(3) + pixx_part * (one - Omega) + (one + Omega) / T(2) * Ru2
+             + (one - Omega) / T(2) * Rv2 + Fx * ux + (one - Omega) / T(2) * (Fx * ux - Fy * uy);
+    T piyy_n = rhoVar / T(3) + piyy_part * (one - Omega) + (one + Omega) / T(2) * Rv2
+             + (one - Omega) / T(2) * Ru2 + Fy * uy + (one - Omega) / T(2) * (Fy * uy - Fx * ux);
+    out[0] = rhoVar;
+    out[1] = ux + Fx * invRho / T(2);
+    out[2] = uy + Fy * invRho / T(2);
+    out[3] = pixx_n * invRho - cs2_2d<T>();
+    out[4] = piyy_n * invRho - cs2_2d<T>();
+    out[5] = pixy * invRho;
+}
+
+template <typename T, int DT>
+__device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z, T out[10])
+{
+    using S = Storage<DT>;
+    using ST = typename S::type;
+    const ST* src = static_cast<const ST*>(a.src);
+    const int64_t st = a.stride;
+    const int nx = a.nx, ny = a.ny;
+    const int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+    T mc[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) mc[f] = S::template load<T>(src, f * st + c, f, a.inv_scale[f]);
+    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, z);
+    const int mval_c = masked ? a.mask[c] : 0;
+    T rho_acc = 0, uxp = 0, uxm = 0, uyp = 0, uym = 0, uzp = 0, uzm = 0;
+    T pixx = 0, piyy = 0, pizz = 0, pixy_p = 0, pixy_m = 0, pixz_p = 0, pixz_m = 0, piyz_p = 0, piyz_m = 0;
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+        const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]), wi = d3w<T>(i);
+        const int64_t s = (int64_t)(x - D3CX[i]) + (int64_t)nx * ((int64_t)(y - D3CY[i]) + (int64_t)ny * (z - D3CZ[i]));
+        int cut = 0, sid = -1;
+        if (masked) {
+            const int mval_s = a.mask[s];
+            if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
+        }
+        T pop_i;
+        if (cut) {
+            T up[3], r[3];
+            wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+            T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
+            T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
+            T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
+            T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
+            T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
+            T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
+            pop_i = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, wi);
+        } else {
+            T m[10];
+#pragma unroll
+            for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+            pop_i = d3q27_recon<T>(m[0], m[1], m[2], m[3], m[4], m[7], m[8], m[5], m[9], m[6], dx, dy, dz, wi);
+            if (pop_i < T(0)) pop_i = T(0);
+        }
+        rho_acc = rho_acc + pop_i;
+        if (D3CX[i] > 0) { uxp = uxp + pop_i; pixx = pixx + pop_i; }
+        else if (D3CX[i] < 0) { uxm = uxm + pop_i; pixx = pixx + pop_i; }
+        if (D3CY[i] > 0) { uyp = uyp + pop_i; piyy = piyy + pop_i; }
+        else if (D3CY[i] < 0) { uym = uym + pop_i; piyy = piyy + pop_i; }
+        if (D3CZ[i] > 0) { uzp = uzp + pop_i; pizz = pizz + pop_i; }
+        else if (D3CZ[i] < 0) { uzm = uzm + pop_i; pizz = pizz + pop_i; }
+        const int dxy = D3CX[i] * D3CY[i], dxz = D3CX[i] * D3CZ[i], dyz = D3CY[i] * D3CZ[i];
+        if (dxy > 0) pixy_p = pixy_p + pop_i; else if (dxy < 0) pixy_m = pixy_m + pop_i;
+        if (dxz > 0) pixz_p = pixz_p + pop_i; else if (dxz < 0) pixz_m = pixz_m + pop_i;
+        if (dyz > 0) piyz_p = piyz_p + pop_i; else if (dyz < 0) piyz_m = piyz_m + pop_i;
+    }
+    T Fx = 0, Fy = 0, Fz = 0;
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[st + c]); Fz = T(a.force[2 * st + c]); }
+    T rhoVar = rho_acc;
+    T invRho = T(1) / rhoVar;
+    T ux = (uxp - uxm + T(0.5) * Fx) * invRho;
+    T uy = (uyp - uym + T(0.5) * Fy) * invRho;
+    T uz = (uzp - uzm + T(0.5) * Fz) * invRho;
+    T pixy = pixy_p - pixy_m, pixz = pixz_p - pixz_m, piyz = piyz_p - piyz_m;
+    T Omega = T(1) / (T(a.nu) * T(3) + T(0.5));
+    const T one = 1, two = 2, three = 3;
+    T pixx_part = (two * pixx - piyy - pizz) / three;
+    T piyy_part = (two * piyy - pixx - pizz) / three;
+    T pizz_part = (two * pizz - pixx - piyy) / three;
+    T RU2 = rhoVar * ux * ux, RV2 = rhoVar * uy * uy, RW2 = rhoVar * uz * uz;
+    T RUVW2 = (RU2 + RV2 + RW2) / three;
+    T pixx_n = rhoVar / three + pixx_part * (one - Omega) + RUVW2 + (two * RU2 * Omega) / three
+             - (RV2 * Omega) / three - (RW2 * Omega) / three + Fx * ux;
+    T piyy_n = rhoVar / three + piyy_part * (one - Omega) + RUVW2 - (RU2 * Omega) / three
+             + (two * RV2 * Omega) / three - (RW2 * Omega) / three + Fy * uy;
+    T pizz_n = rhoVar / three + pizz_part * (one - Omega) + RUVW2 - (RU2 * Omega) / three
+             - (RV2 * Omega) / three + (two * RW2 * Omega) / three + Fz * uz;
+    T pixy_n = pixy - pixy * Omega + ux * uy * rhoVar * Omega + (Fy * ux + Fx * uy) / two;
+    T pixz_n = pixz - pixz * Omega + ux * uz * rhoVar * Omega + (Fz * ux + Fx * uz) / two;
+    T piyz_n = piyz - piyz * Omega + uy * uz * rhoVar * Omega + (Fz * uy + Fy * uz) / two;
+    out[0] = rhoVar;
+    out[1] = ux + Fx * invRho / two;
+    out[2] = uy + Fy * invRho / two;
+    out[3] = uz + Fz * invRho / two;
+    out[4] = pixx_n * invRho - cs2_3d<T>();
+    out[5] = piyy_n * invRho - cs2_3d<T>();
+    out[6] = pizz_n * invRho - cs2_3d<T>();
+    out[7] = pixy_n * invRho;
+    out[8] = pixz_n * invRho;
+    out[9] = piyz_n * invRho;
+}
+
+}  // namespace hl

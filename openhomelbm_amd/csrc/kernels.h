@@ -19,17 +19,21 @@ TileCfg get_tiling(int lattice);
 hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
 // Returns hipErrorNotSupported when no tiled kernel exists for this case.
 hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
-// Sets a.skip (tile blocks next to the body mask) and a.region (the cells of
-// those blocks, for the gather kernel). Leaves a.region empty when no block
-// is skipped; leaves both untouched when the tiled kernel does not apply.
-void plan_tiled(int lattice, int dtype, EnvArgs& a);
+
 
 hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, const double m[10], hipStream_t s);
 hipError_t voxelize(int lattice, const EnvArgs& a, uint8_t* mask, const float* caps, int ncap, const int32_t box[6],
                     hipStream_t s);
+hipError_t voxelize_polygons(const EnvArgs& a, uint8_t* mask, const float* verts, const int32_t* counts, int npoly,
+                             int maxv, const int32_t box[6], hipStream_t s);
 // ws: workspace of at least forces_workspace_bytes() bytes.
 size_t forces_workspace_bytes(const EnvArgs& a, int lattice);
 hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
+// Near-body pass after a fast step kernel: re-runs, in the reference's
+// operation order with the wall rule, every cell whose interior projection
+// has a cut link (when store), and/or writes the per-solid force/torque of
+// the state read (when out != NULL; ws = forces workspace).
+hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* out, void* ws, hipStream_t s);
 size_t mass_workspace_bytes(const EnvArgs& a, int lattice);
 hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
 

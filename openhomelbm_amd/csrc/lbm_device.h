@@ -311,7 +311,6 @@ struct EnvArgs {
     float scale[10];
     float inv_scale[10];
     int32_t region[6];  // cells the gather kernel updates: x0, x1, y0, y1, z0, z1 (half-open)
-    int32_t skip[6];    // tile-unit block box the tiled kernel leaves to the gather kernel
 };
 
 __device__ __forceinline__ bool in_box_halo(const EnvArgs& a, int x, int y, int z)

@@ -10,7 +10,7 @@
 // differs from the reference's order at the 1e-7 level.
 //
 // Work decomposition (one block = a TX x TY column tile and a chunk of z):
-//   * 27 population slots per column in LDS (one plane's worth);
+//   * 27 population slots per column in LDS (one plane's worth, twice);
 //   * the block marches z upward. At plane z it rebuilds the sources of
 //     plane z+1; their cz = -1 ("down") populations go to LDS at once, their
 //     cz = 0 ("mid") ones are held in registers one step and their cz = +1
@@ -22,11 +22,12 @@
 //   * face cells gather at their projection onto the interior and apply the
 //     face conditions in the order x, y, z (plane 0 and nz-1 are written by
 //     the chunks that own planes 1 and nz-2);
-//   * blocks whose cells can have cut links (within one cell of the body
-//     mask box) are skipped here and updated by the gather kernel, which
-//     applies the mask wall rule in the reference's operation order; the
-//     tiled kernel itself carries no wall-rule code and so stays small in
-//     registers.
+//   * cells with cut links (next to a body mask) are streamed here as plain
+//     pulls and then overwritten by the near-body fix-up kernel, which
+//     applies the mask wall rule in the reference's operation order and
+//     accumulates the body forces; the tiled kernel itself carries no
+//     wall-rule code and so stays small in registers;
+//   * LDS is double-buffered, so each plane costs one block barrier.
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -184,12 +185,8 @@ __global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
 k_tiled_d3q27(EnvArgs a, int zc)
 {
     constexpr int NX = TX + 2, NY = TY + 2, NCOL = NX * NY;
-    __shared__ float sp[27][NCOL];
+    __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
 
-    // blocks next to the body mask are updated by the gather kernel (a.region)
-    if ((int)blockIdx.x >= a.skip[0] && (int)blockIdx.x < a.skip[1] && (int)blockIdx.y >= a.skip[2] &&
-        (int)blockIdx.y < a.skip[3] && (int)blockIdx.z >= a.skip[4] && (int)blockIdx.z < a.skip[5])
-        return;
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t plane = (int64_t)nx * ny;
@@ -229,6 +226,10 @@ k_tiled_d3q27(EnvArgs a, int zc)
     }
 
     for (int z = zb; z < ze; ++z) {
+        // buffer b is written here and read after the barrier; the next plane
+        // writes the other buffer, and the one after that passes this plane's
+        // barrier only once every read of b has been issued
+        float(*buf)[NCOL] = sp[(z - zb) & 1];
         if (src_ok) {
             float m[10], f[27];
 #pragma unroll
@@ -236,14 +237,14 @@ k_tiled_d3q27(EnvArgs a, int zc)
             if (z + 2 <= ze) load_moments<DT>(a, (int64_t)(z + 2) * plane + scol, nxt);
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                sp[kMid[k]][tid] = hold_mid[k];
-                sp[kUp[k]][tid] = hold_up0[k];
+                buf[kMid[k]][tid] = hold_mid[k];
+                buf[kUp[k]][tid] = hold_up0[k];
                 hold_up0[k] = hold_up1[k];
             }
             recon27(m, f);
 #pragma unroll
             for (int k = 0; k < 9; ++k) {
-                sp[kDown[k]][tid] = f[kDown[k]];
+                buf[kDown[k]][tid] = f[kDown[k]];
                 hold_up1[k] = f[kUp[k]];
                 hold_mid[k] = f[kMid[k]];
             }
@@ -252,7 +253,7 @@ k_tiled_d3q27(EnvArgs a, int zc)
         if (dst_ok) {
             const int64_t pc = (int64_t)z * plane + pcol;
             Acc acc;
-#define HL_PULL(I) accumulate<I>(acc, sp[I][cp - D3CX[I] - D3CY[I] * NX]);
+#define HL_PULL(I) accumulate<I>(acc, buf[I][cp - D3CX[I] - D3CY[I] * NX]);
             HL_PULL(0) HL_PULL(1) HL_PULL(2) HL_PULL(3) HL_PULL(4) HL_PULL(5) HL_PULL(6) HL_PULL(7) HL_PULL(8)
             HL_PULL(9) HL_PULL(10) HL_PULL(11) HL_PULL(12) HL_PULL(13) HL_PULL(14) HL_PULL(15) HL_PULL(16)
             HL_PULL(17) HL_PULL(18) HL_PULL(19) HL_PULL(20) HL_PULL(21) HL_PULL(22) HL_PULL(23) HL_PULL(24)
@@ -280,7 +281,6 @@ k_tiled_d3q27(EnvArgs a, int zc)
                 store_moments<DT>(a, (int64_t)(nz - 1) * plane + (int64_t)dy * nx + dx, out);
             }
         }
-        __syncthreads();
     }
 }
 
@@ -330,41 +330,6 @@ hipError_t dispatch_tile(const EnvArgs& a, hipStream_t s)
 }
 
 }  // namespace
-
-void plan_tiled(int lattice, int dtype, EnvArgs& a)
-{
-    if (lattice != HL_D3Q27 || dtype == HL_F64 || a.mask == nullptr) return;
-    const TileCfg c = g_tile3d;
-    const int nbx = (a.nx + c.tx - 1) / c.tx, nby = (a.ny + c.ty - 1) / c.ty, nbz = (a.nz - 2 + c.zc - 1) / c.zc;
-    // a block holds a cut link only if its cells (or their interior
-    // projections) come within one cell of the mask box
-    int bx0 = nbx, bx1 = 0, by0 = nby, by1 = 0, bz0 = nbz, bz1 = 0;
-    for (int b = 0; b < nbx; ++b) {
-        const int x0 = b * c.tx;
-        if (x0 + c.tx >= a.box[0] - 1 && x0 - 1 <= a.box[1]) { bx0 = bx0 < b ? bx0 : b; bx1 = b + 1; }
-    }
-    for (int b = 0; b < nby; ++b) {
-        const int y0 = b * c.ty;
-        if (y0 + c.ty >= a.box[2] - 1 && y0 - 1 <= a.box[3]) { by0 = by0 < b ? by0 : b; by1 = b + 1; }
-    }
-    for (int b = 0; b < nbz; ++b) {
-        const int zb = 1 + b * c.zc, ze = zb + c.zc < a.nz - 1 ? zb + c.zc : a.nz - 1;
-        if (zb <= a.box[5] && ze - 1 >= a.box[4] - 1) { bz0 = bz0 < b ? bz0 : b; bz1 = b + 1; }
-    }
-    if (bx0 >= bx1 || by0 >= by1 || bz0 >= bz1) {
-        a.region[1] = a.region[0];  // nothing to hand over
-        return;
-    }
-    a.skip[0] = bx0; a.skip[1] = bx1; a.skip[2] = by0; a.skip[3] = by1; a.skip[4] = bz0; a.skip[5] = bz1;
-    const int zlo = 1 + bz0 * c.zc;
-    const int zhi = 1 + bz1 * c.zc < a.nz - 1 ? 1 + bz1 * c.zc : a.nz - 1;
-    a.region[0] = bx0 * c.tx;
-    a.region[1] = bx1 * c.tx < a.nx ? bx1 * c.tx : a.nx;
-    a.region[2] = by0 * c.ty;
-    a.region[3] = by1 * c.ty < a.ny ? by1 * c.ty : a.ny;
-    a.region[4] = zlo == 1 ? 0 : zlo;          // plane 0 is written with plane 1
-    a.region[5] = zhi == a.nz - 1 ? a.nz : zhi;  // plane nz-1 with plane nz-2
-}
 
 hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s)
 {

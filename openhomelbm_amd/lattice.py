@@ -249,6 +249,37 @@ class Lattice:
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
 
+    def voxelize_polygons(self, verts: np.ndarray, counts: Sequence[int], box=None, stream=None) -> None:
+        """Rebuild the mask from closed 2D polygons (even-odd rule) on the GPU.
+
+        verts: [npoly, maxv, 2] lattice coordinates; counts: vertices per polygon.
+        Polygon k marks its cells with solid id k."""
+        if self.lattice != 9:
+            raise ValueError("polygon voxelization is for D2Q9 lattices")
+        v = np.ascontiguousarray(verts, dtype=np.float32).reshape(len(counts), -1, 2)
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        if v.shape[0] > self.nsolid:
+            raise ValueError("more polygons than solids")
+        if self.mask is None:
+            self.mask = torch.zeros(self.ncell, dtype=torch.uint8, device=self.device)
+        if box is None:
+            pts = np.concatenate([v[k, : c[k]] for k in range(len(c))]) if len(c) else np.zeros((0, 2))
+            if len(pts):
+                lo, hi = np.floor(pts.min(0)).astype(int) - 1, np.ceil(pts.max(0)).astype(int) + 2
+                box = (max(0, lo[0]), min(self.nx, hi[0]), max(0, lo[1]), min(self.ny, hi[1]), 0, 1)
+            else:
+                box = (0, 0, 0, 0, 0, 0)
+        write = union_box(self._mask_dirty, box)
+        self._poly_dev = torch.from_numpy(v).to(self.device)
+        self._poly_cnt = torch.from_numpy(c).to(self.device)
+        b = (C.c_int32 * 6)(*write)
+        e = self.env()
+        N.check(self.lib.hl_voxelize_polygons(C.byref(e), self.mask.data_ptr(), self._poly_dev.data_ptr(),
+                                              self._poly_cnt.data_ptr(), v.shape[0], v.shape[1], b,
+                                              _stream_ptr(stream)), "hl_voxelize_polygons")
+        self.mask_box = tuple(box)
+        self._mask_dirty = tuple(box)
+
     def set_poses(self, poses: np.ndarray) -> None:
         """Pose table [nsolid, 8 | 24] (see include/homelbm.h)."""
         p = np.ascontiguousarray(poses, dtype=np.float32).reshape(self.nsolid, self.pose_width)

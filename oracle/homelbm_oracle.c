@@ -113,6 +113,37 @@ void hlo_voxelize(const hlo_env* e, const float* caps, int32_t ncap, const int32
             }
 }
 
+/* Even-odd +x ray test of get_solid_boundary_img (envs/lbm/lbm_func.py:804-864,
+ * with border thickness 0): an edge (p1, p2) counts when it straddles the
+ * row (y1 > py) != (y2 > py) and its crossing x1 + (py - y1)(x2 - x1)/(y2 - y1)
+ * lies right of px; an odd count means inside. FP32, no contraction. */
+void hlo_voxelize_polygons(const hlo_env* e, const float* verts, const int32_t* counts, int32_t npoly, int32_t maxv,
+                           const int32_t box[6], uint8_t* mask)
+{
+    const int nx = e->nx;
+    for (int y = box[2]; y < box[3]; ++y)
+        for (int x = box[0]; x < box[1]; ++x) {
+            const float px = (float)x, py = (float)y;
+            uint8_t v = 0;
+            for (int k = 0; k < npoly && !v; ++k) {
+                const float* P = verts + 2 * (int64_t)maxv * k;
+                const int n = counts[k];
+                int crossings = 0;
+                for (int j = 0; j < n; ++j) {
+                    const float x1 = P[2 * j], y1 = P[2 * j + 1];
+                    const int j2 = (j + 1) % n;
+                    const float x2 = P[2 * j2], y2 = P[2 * j2 + 1];
+                    if ((y1 > py) != (y2 > py)) {
+                        const float ix = x1 + (py - y1) * (x2 - x1) / (y2 - y1);
+                        if (px < ix) ++crossings;
+                    }
+                }
+                if (crossings & 1) v = (uint8_t)(k + 1);
+            }
+            mask[(int64_t)x + (int64_t)nx * y] = v;
+        }
+}
+
 double hlo_mass_f32(const hlo_env* e, const float* src)
 {
     const int64_t n = (int64_t)e->nx * e->ny * e->nz;

@@ -71,6 +71,12 @@ double hlo_recon_f64(int lattice, const double* m, int i);
 void hlo_voxelize(const hlo_env* e, const float* caps, int32_t ncap,
                   const int32_t box[6], uint8_t* mask);
 
+/* Even-odd polygon voxelizer (2D). verts: [npoly][maxv][2] lattice
+ * coordinates of closed polygons, counts[npoly] vertices each. Writes mask
+ * over the half-open box; lowest polygon index wins. */
+void hlo_voxelize_polygons(const hlo_env* e, const float* verts, const int32_t* counts, int32_t npoly,
+                           int32_t maxv, const int32_t box[6], uint8_t* mask);
+
 /* FP64 sum of rho over all cells. */
 double hlo_mass_f32(const hlo_env* e, const float* src);
 

@@ -62,6 +62,7 @@ def lib():
         L.hlo_recon_f64.argtypes = [C.c_int, P, C.c_int]
         L.hlo_recon_f64.restype = C.c_double
         L.hlo_voxelize.argtypes = [C.POINTER(_Env), P, C.c_int32, P, P]
+        L.hlo_voxelize_polygons.argtypes = [C.POINTER(_Env), P, P, C.c_int32, C.c_int32, P, P]
         L.hlo_mass_f32.argtypes = [C.POINTER(_Env), P]
         L.hlo_mass_f32.restype = C.c_double
         _lib = L
@@ -204,6 +205,22 @@ def voxelize(env: Env, capsules: np.ndarray, box: Optional[Sequence[int]] = None
     assert mask.dtype == np.uint8 and mask.flags.c_contiguous and mask.size == env.ncell
     e = env.c_env()
     lib().hlo_voxelize(C.byref(e), caps.ctypes.data, caps.shape[0], b.ctypes.data, mask.ctypes.data)
+    return mask
+
+
+def voxelize_polygons(env: Env, verts: np.ndarray, counts: Sequence[int], box: Optional[Sequence[int]] = None,
+                      mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """verts: [npoly, maxv, 2] FP32 lattice coordinates; counts: vertices per polygon."""
+    v = np.ascontiguousarray(verts, dtype=np.float32)
+    c = np.ascontiguousarray(counts, dtype=np.int32)
+    if box is None:
+        box = (0, env.nx, 0, env.ny, 0, 1)
+    b = np.ascontiguousarray(box, dtype=np.int32)
+    if mask is None:
+        mask = np.zeros(env.shape, dtype=np.uint8)
+    e = env.c_env()
+    lib().hlo_voxelize_polygons(C.byref(e), v.ctypes.data, c.ctypes.data, v.shape[0], v.shape[1], b.ctypes.data,
+                                mask.ctypes.data)
     return mask
 
 

@@ -186,3 +186,20 @@ def test_invalid_box_raises():
     lat = Lattice(9, 16, 12, nsolid=1)
     with pytest.raises(N.HLError):
         lat.voxelize(np.zeros((1, 8), np.float32), box=(0, 99, 0, 12, 0, 1))
+
+
+def test_polygon_voxelizer_bit_exact():
+    rng = np.random.default_rng(3)
+    env = orc.Env(9, 64, 48)
+    counts = [7, 10]
+    verts = np.zeros((2, 10, 2), np.float32)
+    for k, n in enumerate(counts):
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        rad = rng.uniform(5, 10, n)
+        c = (20 + 20 * k, 24)
+        verts[k, :n] = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], -1)
+    ref = orc.voxelize_polygons(env, verts, counts)
+    lat = Lattice(9, 64, 48, nsolid=2)
+    lat.voxelize_polygons(verts, counts)
+    got = lat.mask.view(48, 64).cpu().numpy()
+    assert np.array_equal(got, ref) and ref.any()

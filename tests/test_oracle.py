@@ -412,3 +412,54 @@ def test_mass_fp64():
     env = orc.Env(27, 6, 5, 4)
     m = random_state(27, (4, 5, 6), 3).astype(np.float32)
     assert abs(orc.mass(env, m) - m[0].astype(np.float64).sum()) < 1e-10
+
+
+def test_polygon_voxelizer_square_closed_form():
+    """Axis-aligned square [10.5, 20.5] x [5.5, 12.5]: inside iff
+    11 <= x <= 20 and 6 <= y <= 12 (even-odd rule, lbm_func.py:846-859)."""
+    env = orc.Env(9, 32, 20)
+    sq = np.array([[[10.5, 5.5], [20.5, 5.5], [20.5, 12.5], [10.5, 12.5]]], np.float32)
+    m = orc.voxelize_polygons(env, sq, [4])
+    y, x = np.mgrid[0:20, 0:32]
+    assert np.array_equal(m, ((x >= 11) & (x <= 20) & (y >= 6) & (y <= 12)).astype(np.uint8))
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_polygon_voxelizer_matches_bruteforce(seed):
+    """Random star-shaped polygons: the FP32 even-odd mask equals an FP64
+    winding-number test away from the edges; the lowest polygon wins."""
+    rng = np.random.default_rng(seed)
+    env = orc.Env(9, 48, 40)
+    polys, counts = [], []
+    for k in range(3):
+        n = int(rng.integers(5, 12))
+        ang = np.sort(rng.uniform(0, 2 * np.pi, n))
+        rad = rng.uniform(4, 9, n)
+        c = rng.uniform(12, 30, 2)
+        polys.append(np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], -1))
+        counts.append(n)
+    verts = np.zeros((3, 12, 2), np.float32)
+    for k in range(3):
+        verts[k, : counts[k]] = polys[k]
+    m = orc.voxelize_polygons(env, verts, counts)
+    y, x = np.mgrid[0:40, 0:48].astype(float)
+    expect = np.zeros((40, 48), np.uint8)
+    ambiguous = np.zeros((40, 48), bool)
+    for k in range(2, -1, -1):
+        P = verts[k, : counts[k]].astype(float)
+        wind = np.zeros((40, 48))
+        for j in range(counts[k]):
+            a, b = P[j], P[(j + 1) % counts[k]]
+            t1 = np.arctan2(a[1] - y, a[0] - x)
+            t2 = np.arctan2(b[1] - y, b[0] - x)
+            d = (t2 - t1 + np.pi) % (2 * np.pi) - np.pi
+            wind += d
+            # distance to the edge
+            ab = b - a
+            t = np.clip(((x - a[0]) * ab[0] + (y - a[1]) * ab[1]) / (ab @ ab), 0, 1)
+            dist = np.hypot(x - (a[0] + t * ab[0]), y - (a[1] + t * ab[1]))
+            ambiguous |= dist < 1e-3
+        inside = np.abs(wind) > np.pi
+        expect[inside] = k + 1
+    assert np.array_equal(m[~ambiguous], expect[~ambiguous])
+    assert (m != 0).sum() > 20
