@@ -186,17 +186,14 @@ static Region fixup_region(const EnvArgs& a, int lattice)
     return r;
 }
 
-static int reduce_blocks(int64_t n)
+static int reduce_blocks(int64_t n, int block = kReduceBlock, int maxb = kMaxReduceBlocks)
 {
-    int64_t b = (n + kReduceBlock - 1) / kReduceBlock;
+    int64_t b = (n + block - 1) / block;
     if (b < 1) b = 1;
-    return (int)(b < kMaxReduceBlocks ? b : kMaxReduceBlocks);
+    return (int)(b < maxb ? b : maxb);
 }
 
-size_t forces_workspace_bytes(const EnvArgs& a, int lattice)
-{
-    return (size_t)kMaxReduceBlocks * HL_MAX_SOLIDS * 6 * sizeof(double);
-}
+
 
 // Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
 // torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
@@ -243,123 +240,177 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
     }
 }
 
-// Near-body fix-up. For every cell of the region whose projection onto the
-// interior has a cut link it (a) recomputes the update in the reference's
-// operation order with the wall rule (when `store`), overwriting what the
-// tiled kernel wrote there, and (b) accumulates the momentum-exchange force
-// and torque of interior cells into per-block FP64 partials (when
-// `partial`): per solid, a wavefront butterfly sum, then the block's waves in
-// index order. Cells without cut links are left untouched.
-template <typename T, int DT, int Q>
-__global__ void __launch_bounds__(kReduceBlock) k_fixup(EnvArgs a, Region rg, int store, double* partial)
+// ------------------------------------------------------------------------
+// Near-body fix-up.
+//
+// A cell needs the wall rule when its projection onto the interior has a cut
+// link (a link whose two ends differ in mask occupancy). Those cells are
+// listed once per mask (hl_step builds the list at the start of each call):
+//   k_cut_count  one wave per 64 region cells: ballot of "has a cut link";
+//   k_cut_scan   one block: exclusive scan of the per-wave counts;
+//   k_cut_write  each wave writes its cells in lane order at its offset,
+// so the list is in region order whatever the scheduling. Per step,
+// k_fixup runs a fixed grid of wavefronts over the list, two cells per wave
+// and one lane per lattice direction: each lane pulls its population (the
+// source cell's loads run in parallel across lanes); the group's first lane
+// sums them in ascending direction order and collides exactly as the
+// reference does, then writes the cell (overwriting the tiled kernel's
+// value). Each lane also owns the momentum-exchange force of its own link;
+// per solid the wave sums with a butterfly into a per-wave FP64 partial.
+
+__device__ __forceinline__ bool region_cell_cut(const EnvArgs& a, const Region& rg, int t, int Q)
 {
-    __shared__ double acc[kWaves][HL_MAX_SOLIDS][6];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (partial)
-        for (int k = threadIdx.x; k < kWaves * HL_MAX_SOLIDS * 6; k += kReduceBlock) (&acc[0][0][0])[k] = 0.0;
-    __syncthreads();
-    using ST = typename Storage<DT>::type;
-    const ST* src = static_cast<const ST*>(a.src);
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t n = rg.n();
-    const int64_t per_iter = (int64_t)gridDim.x * kReduceBlock;
-    const int64_t iters = (n + per_iter - 1) / per_iter;  // uniform trip count for the wave reductions
-    for (int64_t it = 0; it < iters; ++it) {
-        const int64_t t = it * per_iter + (int64_t)blockIdx.x * kReduceBlock + threadIdx.x;
-        const bool valid = t < n;
-        int x = 0, y = 0, z = 0, px = 0, py = 0, pz = 0;
-        uint32_t cut = 0;
-        int mval_c = 0;
-        if (valid) {
-            x = rg.x0 + (int)(t % rg.w);
-            y = rg.y0 + (int)((t / rg.w) % rg.h);
-            z = rg.z0 + (int)(t / ((int64_t)rg.w * rg.h));
+    const int x = rg.x0 + t % rg.w;
+    const int y = rg.y0 + (t / rg.w) % rg.h;
+    const int z = rg.z0 + t / (rg.w * rg.h);
+    const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
+    const int pz = Q == 9 ? 0 : min(max(z, 1), nz - 2);
+    if (!in_box_halo(a, px, py, pz)) return false;
+    const int64_t c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
+    const int mv_c = a.mask[c] != 0;
+    bool has = false;
+    for (int i = 1; i < Q; ++i) {
+        const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+        has |= (a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)] != 0) != mv_c;
+    }
+    return has;
+}
+
+__global__ void __launch_bounds__(64) k_cut_count(EnvArgs a, Region rg, int Q, uint64_t* ballots, int* counts)
+{
+    const int t = blockIdx.x * 64 + threadIdx.x;
+    const bool has = t < (int)rg.n() && region_cell_cut(a, rg, t, Q);
+    const uint64_t b = __ballot(has);
+    if (threadIdx.x == 0) {
+        ballots[blockIdx.x] = b;
+        counts[blockIdx.x] = __popcll(b);
+    }
+}
+
+// Exclusive scan of counts[0..n) into offsets, total into *total (one block).
+__global__ void __launch_bounds__(1024) k_cut_scan(const int* counts, int n, int* offsets, int* total)
+{
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = tid * per, b1 = min(b0 + per, n);
+    int s = 0;
+    for (int b = b0; b < b1; ++b) s += counts[b];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = tid > 0 ? part[tid - 1] : 0;
+    for (int b = b0; b < b1; ++b) {
+        offsets[b] = run;
+        run += counts[b];
+    }
+    if (tid == 1023) *total = part[1023];
+}
+
+__global__ void __launch_bounds__(64) k_cut_write(const uint64_t* ballots, const int* offsets, int nblk, int* list)
+{
+    const int blk = blockIdx.x, lane = threadIdx.x;
+    if (blk >= nblk) return;
+    const uint64_t b = ballots[blk];
+    if ((b >> lane) & 1ull) list[offsets[blk] + __popcll(b & ((1ull << lane) - 1ull))] = blk * 64 + lane;
+}
+
+template <typename T, int DT, int Q>
+__global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* list, const int* count, int store,
+                                               double* partial)
+{
+    __shared__ T pops[2][32];
+    __shared__ T sums[2][16];
+    __shared__ double acc[HL_MAX_SOLIDS * 6];
+    const int lane = threadIdx.x;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int ncut = *count;
+    const int h = lane >> 5, i = lane & 31;
+    if (partial)
+        for (int k = lane; k < a.nsolid * 6; k += 64) acc[k] = 0.0;
+    for (int base = blockIdx.x * 2; base < ncut; base += gridDim.x * 2) {
+        const int e = base + h;
+        const bool cell = e < ncut;
+        const bool act = cell && i < Q;
+        int x = 0, y = 0, z = 0, px = 0, py = 0, pz = 0, mval_c = 0, sid = -1;
+        int64_t c = 0;
+        T F[6] = {0, 0, 0, 0, 0, 0};
+        if (cell) {
+            const int tt = list[e];
+            x = rg.x0 + tt % rg.w;
+            y = rg.y0 + (tt / rg.w) % rg.h;
+            z = rg.z0 + tt / (rg.w * rg.h);
             px = min(max(x, 1), nx - 2);
             py = min(max(y, 1), ny - 2);
             pz = Q == 9 ? 0 : min(max(z, 1), nz - 2);
-            if (in_box_halo(a, px, py, pz)) {
-                const int64_t c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
-                mval_c = a.mask[c];
-#pragma unroll
-                for (int i = 1; i < Q; ++i) {
-                    const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
-                    const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
-                    if ((mv != 0) != (mval_c != 0)) cut |= 1u << i;
+            c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
+            mval_c = a.mask[c];
+        }
+        if (act) {
+            T mc[10];
+            load_cell<T, DT>(a, c, Q == 9 ? 6 : 10, mc);
+            const T pop = Q == 9 ? d2q9_pull<T, DT>(a, px, py, i, mc, true, mval_c)
+                                 : d3q27_pull<T, DT>(a, px, py, pz, i, mc, true, mval_c);
+            pops[h][i] = pop;
+            // force of this lane's link (interior cells only: a face cell repeats its projection)
+            if (partial && i > 0 && px == x && py == y && pz == z) {
+                const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
+                const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
+                if ((mv != 0) != (mval_c != 0)) {
+                    const int s2 = mval_c ? mval_c - 1 : mv - 1;
+                    if (s2 < a.nsolid) {
+                        sid = s2;
+                        link_force<T, DT, Q>(a, mc, x, y, z, i, sid, F);
+                    }
                 }
             }
         }
-        if (store && cut) {
-            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+        __syncthreads();
+        if (Q == 27 && store && cell && i < 16) sums[h][i] = d3q27_group_sum<T>(pops[h], i);
+        __syncthreads();
+        if (store && cell && i == 0) {
+            using ST = typename Storage<DT>::type;
             ST* dst = static_cast<ST*>(a.dst);
+            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
             if (Q == 9) {
-                T m[6];
-                d2q9_post<T, DT>(a, px, py, m);
+                T Fx = 0, Fy = 0, m[6];
+                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+                d2q9_collide<T>(a, pops[h], Fx, Fy, m);
                 if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
 #pragma unroll
                 for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
             } else {
-                T m[10];
-                d3q27_post<T, DT>(a, px, py, pz, m);
+                T Fx = 0, Fy = 0, Fz = 0, m[10];
+                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); Fz = T(a.force[2 * a.stride + c]); }
+                d3q27_collide_sums<T>(a, sums[h], Fx, Fy, Fz, m);
                 if (px != x || py != y || pz != z) bc_chain<T, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
 #pragma unroll
                 for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
             }
         }
-        if (!partial) continue;
-        // forces: interior cells only (a face cell repeats its projection)
-        uint64_t mine = 0;
-        const bool interior = valid && px == x && py == y && pz == z;
-        const int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
-        if (interior && cut) {
+        if (partial) {
+            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
+            while (uni) {
+                const int s = __builtin_ctzll(uni);
+                uni &= uni - 1;
 #pragma unroll
-            for (int i = 1; i < Q; ++i) {
-                if (!(cut & (1u << i))) continue;
-                const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
-                const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
-                const int sid = mval_c ? mval_c - 1 : mv - 1;
-                if (sid < a.nsolid) mine |= 1ull << sid;
-            }
-        }
-        uint64_t uni = wave_or(mine);
-        if (uni == 0) continue;
-        T mc[10];
-        if (mine) {
-#pragma unroll
-            for (int f = 0; f < (Q == 9 ? 6 : 10); ++f) mc[f] = Storage<DT>::template load<T>(src, f * a.stride + c, f, a.inv_scale[f]);
-        }
-        while (uni) {
-            const int s = __builtin_ctzll(uni);
-            uni &= uni - 1;
-            double sum[6] = {0, 0, 0, 0, 0, 0};
-            if ((mine >> s) & 1ull) {
-                for (int i = 1; i < Q; ++i) {
-                    if (!(cut & (1u << i))) continue;
-                    const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
-                    const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
-                    const int sid = mval_c ? mval_c - 1 : mv - 1;
-                    if (sid != s) continue;
-                    T F[6];
-                    link_force<T, DT, Q>(a, mc, x, y, z, i, sid, F);
-#pragma unroll
-                    for (int k = 0; k < 6; ++k) sum[k] += (double)F[k];
+                for (int q = 0; q < 6; ++q) {
+                    const double v = wave_sum(sid == s ? (double)F[q] : 0.0);
+                    if (lane == 0) acc[s * 6 + q] += v;
                 }
             }
-#pragma unroll
-            for (int k = 0; k < 6; ++k) sum[k] = wave_sum(sum[k]);
-            if (lane == 0) {
-#pragma unroll
-                for (int k = 0; k < 6; ++k) acc[wave][s][k] += sum[k];
-            }
         }
+        __syncthreads();  // pops are rewritten by the next pair
     }
     if (!partial) return;
     __syncthreads();
-    for (int k = threadIdx.x; k < a.nsolid * 6; k += kReduceBlock) {
-        const int s = k / 6, q = k % 6;
-        double v = 0.0;
-        for (int w = 0; w < kWaves; ++w) v += acc[w][s][q];
-        partial[(int64_t)blockIdx.x * a.nsolid * 6 + k] = v;
-    }
+    for (int k = lane; k < a.nsolid * 6; k += 64) partial[(int64_t)blockIdx.x * a.nsolid * 6 + k] = acc[k];
 }
 
 // out[k] = sum over blocks b of partial[b][k]: each lane sums a fixed strided
@@ -379,12 +430,73 @@ __global__ void k_zero(double* out, int n)
     if (k < n) out[k] = 0.0;
 }
 
-template <typename T, int DT>
-static hipError_t launch_fixup(int lattice, const EnvArgs& a, const Region& rg, int store, double* part, int nb,
-                               hipStream_t s)
+constexpr int kFixupWaves = 1024;
+
+struct FixupLayout {
+    size_t partial, ballots, counts, offsets, total, list, bytes;
+    int nblk;
+};
+
+static FixupLayout fixup_layout(const Region& rg)
 {
-    if (lattice == HL_D2Q9) hipLaunchKernelGGL((k_fixup<T, DT, 9>), nb, kReduceBlock, 0, s, a, rg, store, part);
-    else hipLaunchKernelGGL((k_fixup<T, DT, 27>), nb, kReduceBlock, 0, s, a, rg, store, part);
+    FixupLayout L;
+    const int64_t n = rg.n();
+    L.nblk = (int)((n + 63) / 64);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off = (off + bytes + 255) / 256 * 256; return o; };
+    L.partial = take((size_t)kFixupWaves * HL_MAX_SOLIDS * 6 * sizeof(double));
+    L.ballots = take((size_t)(L.nblk > 0 ? L.nblk : 1) * sizeof(uint64_t));
+    L.counts = take((size_t)(L.nblk > 0 ? L.nblk : 1) * sizeof(int));
+    L.offsets = take((size_t)(L.nblk > 0 ? L.nblk : 1) * sizeof(int));
+    L.total = take(sizeof(int));
+    L.list = take((size_t)(n > 0 ? n : 1) * sizeof(int));
+    L.bytes = off;
+    return L;
+}
+
+size_t forces_workspace_bytes(const EnvArgs& a, int lattice)
+{
+    return fixup_layout(fixup_region(a, lattice)).bytes;
+}
+
+void cut_bits_view(int lattice, EnvArgs& a, void* ws)
+{
+    const Region rg = fixup_region(a, lattice);
+    if (a.mask == nullptr || rg.n() == 0 || ws == nullptr) {
+        a.cut_bits = nullptr;
+        return;
+    }
+    const FixupLayout L = fixup_layout(rg);
+    a.cut_bits = reinterpret_cast<const uint64_t*>(static_cast<char*>(ws) + L.ballots);
+    a.cut_origin[0] = rg.x0; a.cut_origin[1] = rg.y0; a.cut_origin[2] = rg.z0;
+    a.cut_extent[0] = rg.w; a.cut_extent[1] = rg.h; a.cut_extent[2] = rg.d;
+}
+
+hipError_t cut_list(int lattice, const EnvArgs& a, void* ws, hipStream_t s)
+{
+    const Region rg = fixup_region(a, lattice);
+    if (a.mask == nullptr || rg.n() == 0) return hipSuccess;
+    if (rg.n() > (int64_t)1 << 30) return hipErrorInvalidValue;
+    const FixupLayout L = fixup_layout(rg);
+    char* base = static_cast<char*>(ws);
+    uint64_t* ballots = reinterpret_cast<uint64_t*>(base + L.ballots);
+    int* counts = reinterpret_cast<int*>(base + L.counts);
+    int* offsets = reinterpret_cast<int*>(base + L.offsets);
+    int* total = reinterpret_cast<int*>(base + L.total);
+    int* list = reinterpret_cast<int*>(base + L.list);
+    const int Q = lattice == HL_D2Q9 ? 9 : 27;
+    hipLaunchKernelGGL(k_cut_count, L.nblk, 64, 0, s, a, rg, Q, ballots, counts);
+    hipLaunchKernelGGL(k_cut_scan, 1, 1024, 0, s, counts, L.nblk, offsets, total);
+    hipLaunchKernelGGL(k_cut_write, L.nblk, 64, 0, s, ballots, offsets, L.nblk, list);
+    return hipGetLastError();
+}
+
+template <typename T, int DT>
+static hipError_t launch_fixup(int lattice, const EnvArgs& a, const Region& rg, const int* list, const int* count,
+                               int store, double* part, int nb, hipStream_t s)
+{
+    if (lattice == HL_D2Q9) hipLaunchKernelGGL((k_fixup<T, DT, 9>), nb, 64, 0, s, a, rg, list, count, store, part);
+    else hipLaunchKernelGGL((k_fixup<T, DT, 27>), nb, 64, 0, s, a, rg, list, count, store, part);
     return hipGetLastError();
 }
 
@@ -401,14 +513,19 @@ hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* o
         return hipSuccess;
     }
     if (!store && !want_forces) return hipSuccess;
-    const int nb = reduce_blocks(rg.n());
-    double* part = want_forces ? static_cast<double*>(ws) : nullptr;
+    const FixupLayout L = fixup_layout(rg);
+    char* base = static_cast<char*>(ws);
+    const int* total = reinterpret_cast<const int*>(base + L.total);
+    const int* list = reinterpret_cast<const int*>(base + L.list);
+    double* part = want_forces ? reinterpret_cast<double*>(base + L.partial) : nullptr;
+    int64_t pairs = (rg.n() + 1) / 2;
+    const int nb = (int)(pairs < kFixupWaves ? (pairs > 0 ? pairs : 1) : kFixupWaves);
     hipError_t e;
     switch (dtype) {
-    case HL_F32: e = launch_fixup<float, HL_F32>(lattice, a, rg, store, part, nb, s); break;
-    case HL_F64: e = launch_fixup<double, HL_F64>(lattice, a, rg, store, part, nb, s); break;
-    case HL_F16: e = launch_fixup<float, HL_F16>(lattice, a, rg, store, part, nb, s); break;
-    case HL_BF16: e = launch_fixup<float, HL_BF16>(lattice, a, rg, store, part, nb, s); break;
+    case HL_F32: e = launch_fixup<float, HL_F32>(lattice, a, rg, list, total, store, part, nb, s); break;
+    case HL_F64: e = launch_fixup<double, HL_F64>(lattice, a, rg, list, total, store, part, nb, s); break;
+    case HL_F16: e = launch_fixup<float, HL_F16>(lattice, a, rg, list, total, store, part, nb, s); break;
+    case HL_BF16: e = launch_fixup<float, HL_BF16>(lattice, a, rg, list, total, store, part, nb, s); break;
     default: return hipErrorInvalidValue;
     }
     if (e != hipSuccess || !want_forces) return e;
@@ -418,6 +535,8 @@ hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* o
 
 hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s)
 {
+    hipError_t e = cut_list(lattice, a, ws, s);
+    if (e != hipSuccess) return e;
     return fixup(lattice, dtype, a, false, out, ws, s);
 }
 

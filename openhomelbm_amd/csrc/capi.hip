@@ -127,6 +127,33 @@ int32_t workspace(size_t bytes, void** out)
     return HL_OK;
 }
 
+// Side stream (default priority) and fork/join events per device, for the
+// near-body fix-up that runs concurrently with the tiled kernel. Created on
+// first use, kept until the process ends. g_side_mutex serialises the
+// enqueue of steps that use them.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+std::mutex g_side_mutex;
+std::vector<SideStream> g_side;
+
+int32_t side_stream(SideStream** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if ((int)g_side.size() <= dev) g_side.resize(dev + 1);
+    SideStream& ss = g_side[dev];
+    if (!ss.stream) {
+        if ((e = hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        if ((e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        if ((e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
+    *out = &ss;
+    return HL_OK;
+}
+
 // Uniform-flow moments in working precision T, in the operation order of
 // set_uniform_flow_kernel (karman_lbm_env.py:15-53) and
 // set_uniform_flow_3d_kernel (karman_lbm_env_3d.py:15-64).
@@ -190,24 +217,57 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
     if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
     if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
     hipStream_t s = static_cast<hipStream_t>(stream);
+    // the near-body pass needs the cut-cell list when a mask is present and
+    // either the tiled kernel runs or forces are requested
+    const bool tiled = kernel != HL_KERNEL_GATHER && hl::tiling_supported(env->lattice, 32, 8, 1) &&
+                       env->dtype != HL_F64;
+    const bool want_forces = force_history && env->nsolid > 0;
     void* ws = nullptr;
-    if (force_history && env->nsolid > 0) {
-        rc = workspace(hl::forces_workspace_bytes(make_args(env, cur), env->lattice), &ws);
+    if (env->mask && nsteps > 0 && (tiled || want_forces)) {
+        hl::EnvArgs a0 = make_args(env, cur);
+        rc = workspace(hl::forces_workspace_bytes(a0, env->lattice), &ws);
+        if (rc) return rc;
+        hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
+        if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
+    }
+    std::unique_lock<std::mutex> side_lock(g_side_mutex, std::defer_lock);
+    SideStream* ss = nullptr;
+    if (tiled && ws) {
+        side_lock.lock();
+        rc = side_stream(&ss);
         if (rc) return rc;
     }
     for (int k = 0; k < nsteps; ++k) {
         hl::EnvArgs a = make_args(env, cur);
-        double* fout = ws ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
+        double* fout = want_forces ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
         hipError_t e = hipErrorNotSupported;
         bool fixed = false;
         if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) {
+            if (ss) {
+                // near-body fix-up (and forces) on the side stream, concurrent
+                // with the tiled kernel; they write disjoint cells
+                hl::cut_bits_view(env->lattice, a, ws);
+                if ((e = hipEventRecord(ss->fork, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+                if ((e = hipStreamWaitEvent(ss->stream, ss->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, ss->stream);
+                if (e != hipSuccess) return hip_fail(e, "fix-up kernel launch");
+                if ((e = hipEventRecord(ss->join, ss->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+            }
             e = hl::step_tiled(env->lattice, env->dtype, a, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
                 return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
-            if (e == hipSuccess) {
-                // cells next to bodies: reference-order wall rule, plus the forces
-                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, s);
+            if (ss) {
+                hipError_t e2 = hipStreamWaitEvent(s, ss->join, 0);
+                if (e2 != hipSuccess) return hip_fail(e2, "hipStreamWaitEvent");
+                if (e == hipErrorNotSupported) {
+                    // the tiled kernel declined this shape: redo the step with the gather kernel
+                    a.cut_bits = nullptr;
+                    e = hl::step_gather(env->lattice, env->dtype, a, s);
+                }
                 fixed = true;
+            } else if (e == hipSuccess) {
+                fixed = true;
+                if (fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
             }
         }
         if (e == hipErrorNotSupported) e = hl::step_gather(env->lattice, env->dtype, a, s);

@@ -11,46 +11,46 @@
 
 namespace hl {
 
+// Population i pulled into interior cell (x, y) (lbm_func.py:335-412): the
+// reconstruction from the source cell's moments, or on a cut link the
+// wall-rule population from the cell's own moments mc; clamped at 0 on both
+// branches as in the reference.
 template <typename T, int DT>
-__device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[6])
+__device__ __forceinline__ T d2q9_pull(const EnvArgs& a, int x, int y, int i, const T* mc, bool masked, int mval_c)
 {
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
     const int64_t st = a.stride;
-    const int nx = a.nx;
-    const int64_t c = (int64_t)x + (int64_t)nx * y;
-    T mc[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) mc[f] = S::template load<T>(src, f * st + c, f, a.inv_scale[f]);
-    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, 0);
-    const int mval_c = masked ? a.mask[c] : 0;
-    T pop[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const int64_t s = (int64_t)(x - D2CX[i]) + (int64_t)nx * (y - D2CY[i]);
-        int cut = 0, sid = -1;
-        if (masked) {
-            const int mval_s = a.mask[s];
-            if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
-        }
-        if (cut) {
-            T up[2], r[2];
-            wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
-            T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
-            T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
-            T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
-            pop[i] = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
-        } else {
-            T m[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
-            pop[i] = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
-        }
-        if (pop[i] < T(0)) pop[i] = T(0);
+    const int64_t s = (int64_t)(x - D2CX[i]) + (int64_t)a.nx * (y - D2CY[i]);
+    int cut = 0, sid = -1;
+    if (masked) {
+        const int mval_s = a.mask[s];
+        if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
     }
-    T Fx = 0, Fy = 0;
-    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[st + c]); }
+    T pop;
+    if (cut) {
+        T up[2], r[2];
+        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+        pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+    } else {
+        T m[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+        pop = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+    }
+    if (pop < T(0)) pop = T(0);
+    return pop;
+}
+
+// Moments and regularized collision of the 9 pulled populations
+// (lbm_func.py:414-467), same operation order.
+template <typename T>
+__device__ __forceinline__ void d2q9_collide(const EnvArgs& a, const T* pop, T Fx, T Fy, T out[6])
+{
     T rhoVar = pop[0] + pop[1] + pop[2] + pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
     T invRho = T(1) / rhoVar;
     T ux = (pop[1] - pop[2] + pop[5] - pop[6] - pop[7] + pop[8] + T(0.5) * Fx) * invRho;
@@ -78,68 +78,133 @@ __device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[
 }
 
 template <typename T, int DT>
-__device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z, T out[10])
+__device__ __forceinline__ void load_cell(const EnvArgs& a, int64_t c, int nf, T* mc)
+{
+    using S = Storage<DT>;
+    const typename S::type* src = static_cast<const typename S::type*>(a.src);
+    for (int f = 0; f < nf; ++f) mc[f] = S::template load<T>(src, f * a.stride + c, f, a.inv_scale[f]);
+}
+
+template <typename T, int DT>
+__device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[6])
+{
+    const int64_t c = (int64_t)x + (int64_t)a.nx * y;
+    T mc[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) mc[f] = Storage<DT>::template load<T>(static_cast<const typename Storage<DT>::type*>(a.src), f * a.stride + c, f, a.inv_scale[f]);
+    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, 0);
+    const int mval_c = masked ? a.mask[c] : 0;
+    T pop[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) pop[i] = d2q9_pull<T, DT>(a, x, y, i, mc, masked, mval_c);
+    T Fx = 0, Fy = 0;
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+    d2q9_collide<T>(a, pop, Fx, Fy, out);
+}
+
+// Population i pulled into interior cell (x, y, z) (lbm_func_3d.py:414-526):
+// the reconstruction from the source cell's moments, clamped at 0, or on a
+// cut link the unclamped wall-rule population from the cell's moments mc.
+template <typename T, int DT>
+__device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, int i, const T* mc, bool masked, int mval_c)
 {
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
     const int64_t st = a.stride;
-    const int nx = a.nx, ny = a.ny;
-    const int64_t c = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
-    T mc[10];
-#pragma unroll
-    for (int f = 0; f < 10; ++f) mc[f] = S::template load<T>(src, f * st + c, f, a.inv_scale[f]);
-    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, z);
-    const int mval_c = masked ? a.mask[c] : 0;
-    T rho_acc = 0, uxp = 0, uxm = 0, uyp = 0, uym = 0, uzp = 0, uzm = 0;
-    T pixx = 0, piyy = 0, pizz = 0, pixy_p = 0, pixy_m = 0, pixz_p = 0, pixz_m = 0, piyz_p = 0, piyz_m = 0;
-#pragma unroll
-    for (int i = 0; i < 27; ++i) {
-        const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]), wi = d3w<T>(i);
-        const int64_t s = (int64_t)(x - D3CX[i]) + (int64_t)nx * ((int64_t)(y - D3CY[i]) + (int64_t)ny * (z - D3CZ[i]));
-        int cut = 0, sid = -1;
-        if (masked) {
-            const int mval_s = a.mask[s];
-            if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
-        }
-        T pop_i;
-        if (cut) {
-            T up[3], r[3];
-            wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
-            T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
-            T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
-            T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
-            T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
-            T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
-            T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
-            pop_i = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, wi);
-        } else {
-            T m[10];
-#pragma unroll
-            for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
-            pop_i = d3q27_recon<T>(m[0], m[1], m[2], m[3], m[4], m[7], m[8], m[5], m[9], m[6], dx, dy, dz, wi);
-            if (pop_i < T(0)) pop_i = T(0);
-        }
-        rho_acc = rho_acc + pop_i;
-        if (D3CX[i] > 0) { uxp = uxp + pop_i; pixx = pixx + pop_i; }
-        else if (D3CX[i] < 0) { uxm = uxm + pop_i; pixx = pixx + pop_i; }
-        if (D3CY[i] > 0) { uyp = uyp + pop_i; piyy = piyy + pop_i; }
-        else if (D3CY[i] < 0) { uym = uym + pop_i; piyy = piyy + pop_i; }
-        if (D3CZ[i] > 0) { uzp = uzp + pop_i; pizz = pizz + pop_i; }
-        else if (D3CZ[i] < 0) { uzm = uzm + pop_i; pizz = pizz + pop_i; }
-        const int dxy = D3CX[i] * D3CY[i], dxz = D3CX[i] * D3CZ[i], dyz = D3CY[i] * D3CZ[i];
-        if (dxy > 0) pixy_p = pixy_p + pop_i; else if (dxy < 0) pixy_m = pixy_m + pop_i;
-        if (dxz > 0) pixz_p = pixz_p + pop_i; else if (dxz < 0) pixz_m = pixz_m + pop_i;
-        if (dyz > 0) piyz_p = piyz_p + pop_i; else if (dyz < 0) piyz_m = piyz_m + pop_i;
+    const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]), wi = d3w<T>(i);
+    const int64_t s = (int64_t)(x - D3CX[i]) + (int64_t)a.nx * ((int64_t)(y - D3CY[i]) + (int64_t)a.ny * (z - D3CZ[i]));
+    int cut = 0, sid = -1;
+    if (masked) {
+        const int mval_s = a.mask[s];
+        if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
     }
-    T Fx = 0, Fy = 0, Fz = 0;
-    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[st + c]); Fz = T(a.force[2 * st + c]); }
+    T pop_i;
+    if (cut) {
+        T up[3], r[3];
+        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+        T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
+        T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
+        T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
+        T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
+        T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
+        T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
+        pop_i = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, wi);
+    } else {
+        T m[10];
+#pragma unroll
+        for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+        pop_i = d3q27_recon<T>(m[0], m[1], m[2], m[3], m[4], m[7], m[8], m[5], m[9], m[6], dx, dy, dz, wi);
+        if (pop_i < T(0)) pop_i = T(0);
+    }
+    return pop_i;
+}
+
+// Sign-group moment sums of lbm_func_3d.py:528-562: sixteen accumulators,
+// each a sum in ascending direction order over the directions of its group.
+// k: 0 rho; 1/2 +x/-x; 3/4 +y/-y; 5/6 +z/-z; 7/8/9 pixx/piyy/pizz (c_a != 0);
+// 10/11 c_x c_y > 0 / < 0; 12/13 c_x c_z; 14/15 c_y c_z. Each group is an
+// independent sequential sum, so lanes may compute different k in parallel.
+__device__ __forceinline__ bool d3q27_in_group(int k, int i)
+{
+    const int cx = D3CX[i], cy = D3CY[i], cz = D3CZ[i];
+    switch (k) {
+    case 0: return true;
+    case 1: return cx > 0;
+    case 2: return cx < 0;
+    case 3: return cy > 0;
+    case 4: return cy < 0;
+    case 5: return cz > 0;
+    case 6: return cz < 0;
+    case 7: return cx != 0;
+    case 8: return cy != 0;
+    case 9: return cz != 0;
+    case 10: return cx * cy > 0;
+    case 11: return cx * cy < 0;
+    case 12: return cx * cz > 0;
+    case 13: return cx * cz < 0;
+    case 14: return cy * cz > 0;
+    default: return cy * cz < 0;
+    }
+}
+
+// Group membership as 27-bit masks (bit i set when direction i belongs).
+__device__ __forceinline__ uint32_t d3q27_group_bits(int k)
+{
+    uint32_t bits = 0;
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+        uint32_t b = 0;
+#pragma unroll
+        for (int i = 0; i < 27; ++i)
+            if (d3q27_in_group(kk, i)) b |= 1u << i;
+        if (kk == k) bits = b;
+    }
+    return bits;
+}
+
+template <typename T> __device__ __forceinline__ T d3q27_group_sum(const T* pop, int k)
+{
+    const uint32_t bits = d3q27_group_bits(k);
+    T acc = 0;
+    for (int i = 0; i < 27; ++i)
+        if ((bits >> i) & 1u) acc = acc + pop[i];
+    return acc;
+}
+
+// The 3D regularized collision (lbm_func_3d.py:568-604) from the sixteen
+// group sums, same operation order.
+template <typename T>
+__device__ __forceinline__ void d3q27_collide_sums(const EnvArgs& a, const T* g, T Fx, T Fy, T Fz, T out[10])
+{
+    const T rho_acc = g[0], uxp = g[1], uxm = g[2], uyp = g[3], uym = g[4], uzp = g[5], uzm = g[6];
+    const T pixx = g[7], piyy = g[8], pizz = g[9];
     T rhoVar = rho_acc;
     T invRho = T(1) / rhoVar;
     T ux = (uxp - uxm + T(0.5) * Fx) * invRho;
     T uy = (uyp - uym + T(0.5) * Fy) * invRho;
     T uz = (uzp - uzm + T(0.5) * Fz) * invRho;
-    T pixy = pixy_p - pixy_m, pixz = pixz_p - pixz_m, piyz = piyz_p - piyz_m;
+    T pixy = g[10] - g[11], pixz = g[12] - g[13], piyz = g[14] - g[15];
     T Omega = T(1) / (T(a.nu) * T(3) + T(0.5));
     const T one = 1, two = 2, three = 3;
     T pixx_part = (two * pixx - piyy - pizz) / three;
@@ -166,6 +231,38 @@ __device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z
     out[7] = pixy_n * invRho;
     out[8] = pixz_n * invRho;
     out[9] = piyz_n * invRho;
+}
+
+template <typename T>
+__device__ __forceinline__ void d3q27_collide(const EnvArgs& a, const T* pop, T Fx, T Fy, T Fz, T out[10])
+{
+    T g[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        T acc = 0;
+#pragma unroll
+        for (int i = 0; i < 27; ++i)
+            if (d3q27_in_group(k, i)) acc = acc + pop[i];
+        g[k] = acc;
+    }
+    d3q27_collide_sums<T>(a, g, Fx, Fy, Fz, out);
+}
+
+template <typename T, int DT>
+__device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z, T out[10])
+{
+    const int64_t c = (int64_t)x + (int64_t)a.nx * ((int64_t)y + (int64_t)a.ny * z);
+    T mc[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) mc[f] = Storage<DT>::template load<T>(static_cast<const typename Storage<DT>::type*>(a.src), f * a.stride + c, f, a.inv_scale[f]);
+    const bool masked = a.mask != nullptr && in_box_halo(a, x, y, z);
+    const int mval_c = masked ? a.mask[c] : 0;
+    T pop[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) pop[i] = d3q27_pull<T, DT>(a, x, y, z, i, mc, masked, mval_c);
+    T Fx = 0, Fy = 0, Fz = 0;
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); Fz = T(a.force[2 * a.stride + c]); }
+    d3q27_collide<T>(a, pop, Fx, Fy, Fz, out);
 }
 
 }  // namespace hl

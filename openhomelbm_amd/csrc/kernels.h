@@ -29,10 +29,14 @@ hipError_t voxelize_polygons(const EnvArgs& a, uint8_t* mask, const float* verts
 // ws: workspace of at least forces_workspace_bytes() bytes.
 size_t forces_workspace_bytes(const EnvArgs& a, int lattice);
 hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
-// Near-body pass after a fast step kernel: re-runs, in the reference's
-// operation order with the wall rule, every cell whose interior projection
-// has a cut link (when store), and/or writes the per-solid force/torque of
-// the state read (when out != NULL; ws = forces workspace).
+// Lists, in region order, the cells whose interior projection has a cut link
+// (into ws, sized by forces_workspace_bytes); valid until the mask changes.
+hipError_t cut_list(int lattice, const EnvArgs& a, void* ws, hipStream_t s);
+// Points a.cut_bits at the list's cell bitmap in ws (null when no mask).
+void cut_bits_view(int lattice, EnvArgs& a, void* ws);
+// Near-body pass over that list: re-runs each listed cell in the reference's
+// operation order with the wall rule (when store), and/or writes the
+// per-solid force/torque of the state read (when out != NULL).
 hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* out, void* ws, hipStream_t s);
 size_t mass_workspace_bytes(const EnvArgs& a, int lattice);
 hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);

@@ -311,7 +311,22 @@ struct EnvArgs {
     float scale[10];
     float inv_scale[10];
     int32_t region[6];  // cells the gather kernel updates: x0, x1, y0, y1, z0, z1 (half-open)
+    // cells whose update the near-body fix-up owns: bit t of cut_bits marks
+    // region cell t = (x - cx0) + cw * ((y - cy0) + ch * (z - cz0)); null = none
+    const uint64_t* cut_bits;
+    int32_t cut_origin[3];
+    int32_t cut_extent[3];
 };
+
+__device__ __forceinline__ bool fixup_owns(const EnvArgs& a, int x, int y, int z)
+{
+    if (!a.cut_bits) return false;
+    const int rx = x - a.cut_origin[0], ry = y - a.cut_origin[1], rz = z - a.cut_origin[2];
+    if (rx < 0 || ry < 0 || rz < 0 || rx >= a.cut_extent[0] || ry >= a.cut_extent[1] || rz >= a.cut_extent[2])
+        return false;
+    const int t = rx + a.cut_extent[0] * (ry + a.cut_extent[1] * rz);
+    return (a.cut_bits[t >> 6] >> (t & 63)) & 1ull;
+}
 
 __device__ __forceinline__ bool in_box_halo(const EnvArgs& a, int x, int y, int z)
 {

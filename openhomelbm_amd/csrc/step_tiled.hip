@@ -22,12 +22,15 @@
 //   * face cells gather at their projection onto the interior and apply the
 //     face conditions in the order x, y, z (plane 0 and nz-1 are written by
 //     the chunks that own planes 1 and nz-2);
-//   * cells with cut links (next to a body mask) are streamed here as plain
-//     pulls and then overwritten by the near-body fix-up kernel, which
-//     applies the mask wall rule in the reference's operation order and
-//     accumulates the body forces; the tiled kernel itself carries no
-//     wall-rule code and so stays small in registers;
-//   * LDS is double-buffered, so each plane costs one block barrier.
+//   * cells with cut links (next to a body mask) are not written here: the
+//     near-body fix-up kernel, running concurrently on a second stream,
+//     updates them with the mask wall rule in the reference's operation
+//     order and accumulates the body forces; the tiled kernel itself carries
+//     no wall-rule code and so stays small in registers;
+//   * LDS is double-buffered and the loop is software-pipelined: while the
+//     cells of plane z pull from one buffer, the sources of plane z+2 fill
+//     the other, so each plane costs one block barrier and the two halves
+//     of the work can overlap within a wavefront.
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -180,7 +183,7 @@ __device__ __forceinline__ void store_moments(const EnvArgs& a, int64_t c, const
 
 }  // namespace
 
-template <int DT, int TX, int TY>
+template <int DT, int TX, int TY, bool FORCE>
 __global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
 k_tiled_d3q27(EnvArgs a, int zc)
 {
@@ -207,11 +210,68 @@ k_tiled_d3q27(EnvArgs a, int zc)
     const int64_t pcol = (int64_t)py * nx + px;
 
     const float omega = 1.0f / (a.nu * 3.0f + 0.5f);
+    // does this tile touch the fix-up region at all (block-uniform)?
+    const bool near = a.cut_bits != nullptr && x0 + TX > a.cut_origin[0] && x0 < a.cut_origin[0] + a.cut_extent[0] &&
+                      y0 + TY > a.cut_origin[1] && y0 < a.cut_origin[1] + a.cut_extent[1];
 
     float hold_mid[9], hold_up0[9], hold_up1[9];
-    float nxt[10];
+    float pfa[10], pfb[10];  // prefetched source moments, alternating by plane
 
-    // prologue: up pops of plane zb-1 and mid/up pops of plane zb
+    // Source work for plane k: rebuild its 27 populations, write to buf the
+    // down pops of plane k, the held mid pops of plane k-1 and the held up
+    // pops of plane k-2, then shift the holds.
+    auto source_plane = [&](float(*buf)[NCOL], const float* m) {
+        float f[27];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            buf[kMid[k]][tid] = hold_mid[k];
+            buf[kUp[k]][tid] = hold_up0[k];
+            hold_up0[k] = hold_up1[k];
+        }
+        recon27(m, f);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            buf[kDown[k]][tid] = f[kDown[k]];
+            hold_up1[k] = f[kUp[k]];
+            hold_mid[k] = f[kMid[k]];
+        }
+    };
+
+    // Pull, collide and store the cells of plane z from buf.
+    auto dest_plane = [&](float(*buf)[NCOL], int z) {
+        const int64_t pc = (int64_t)z * plane + pcol;
+        Acc acc;
+#define HL_PULL(I) accumulate<I>(acc, buf[I][cp - D3CX[I] - D3CY[I] * NX]);
+        HL_PULL(0) HL_PULL(1) HL_PULL(2) HL_PULL(3) HL_PULL(4) HL_PULL(5) HL_PULL(6) HL_PULL(7) HL_PULL(8)
+        HL_PULL(9) HL_PULL(10) HL_PULL(11) HL_PULL(12) HL_PULL(13) HL_PULL(14) HL_PULL(15) HL_PULL(16)
+        HL_PULL(17) HL_PULL(18) HL_PULL(19) HL_PULL(20) HL_PULL(21) HL_PULL(22) HL_PULL(23) HL_PULL(24)
+        HL_PULL(25) HL_PULL(26)
+#undef HL_PULL
+        float F[3] = {0.0f, 0.0f, 0.0f};
+        if (FORCE) { F[0] = a.force[pc]; F[1] = a.force[a.stride + pc]; F[2] = a.force[2 * a.stride + pc]; }
+        float out[10];
+        collide27(acc, F, omega, out);
+        // faces: x, then y (this plane), then z (planes 0 and nz-1 from planes 1 and nz-2)
+        if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
+        else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
+        if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
+        else if (dy == 0) bc_face_closed(out, a.bc_type[3], a.bc_value[3], 1);
+        // cells the near-body fix-up owns are left to it (it runs concurrently)
+        if (!(near && fixup_owns(a, dx, dy, z))) store_moments<DT>(a, (int64_t)z * plane + (int64_t)dy * nx + dx, out);
+        if (z == 1 && !(near && fixup_owns(a, dx, dy, 0))) {
+            float face[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) face[k] = out[k];
+            bc_face_closed(face, a.bc_type[5], a.bc_value[5], 2);
+            store_moments<DT>(a, (int64_t)dy * nx + dx, face);
+        }
+        if (z == nz - 2 && !(near && fixup_owns(a, dx, dy, nz - 1))) {
+            bc_face_closed(out, a.bc_type[4], a.bc_value[4], 2);
+            store_moments<DT>(a, (int64_t)(nz - 1) * plane + (int64_t)dy * nx + dx, out);
+        }
+    };
+
+    // prologue: planes zb-1 (up pops), zb (mid/up pops), zb+1 (into buffer 0)
     if (src_ok) {
         float m[10], f[27];
         load_moments<DT>(a, (int64_t)(zb - 1) * plane + scol, m);
@@ -222,70 +282,35 @@ k_tiled_d3q27(EnvArgs a, int zc)
         recon27(m, f);
 #pragma unroll
         for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; hold_up1[k] = f[kUp[k]]; }
-        load_moments<DT>(a, (int64_t)(zb + 1) * plane + scol, nxt);
+        load_moments<DT>(a, (int64_t)(zb + 1) * plane + scol, m);
+        if (zb + 2 <= ze) load_moments<DT>(a, (int64_t)(zb + 2) * plane + scol, pfa);
+        source_plane(sp[0], m);
     }
+    __syncthreads();
 
-    for (int z = zb; z < ze; ++z) {
-        // buffer b is written here and read after the barrier; the next plane
-        // writes the other buffer, and the one after that passes this plane's
-        // barrier only once every read of b has been issued
-        float(*buf)[NCOL] = sp[(z - zb) & 1];
-        if (src_ok) {
-            float m[10], f[27];
-#pragma unroll
-            for (int k = 0; k < 10; ++k) m[k] = nxt[k];
-            if (z + 2 <= ze) load_moments<DT>(a, (int64_t)(z + 2) * plane + scol, nxt);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                buf[kMid[k]][tid] = hold_mid[k];
-                buf[kUp[k]][tid] = hold_up0[k];
-                hold_up0[k] = hold_up1[k];
-            }
-            recon27(m, f);
-#pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                buf[kDown[k]][tid] = f[kDown[k]];
-                hold_up1[k] = f[kUp[k]];
-                hold_mid[k] = f[kMid[k]];
-            }
+    // Software pipeline: step z pulls plane z from buffer (z-zb)&1 while
+    // rebuilding the sources of plane z+2 (prefetched one step earlier) into
+    // the other buffer, then prefetches plane z+3; one barrier per plane. The
+    // loop is unrolled by two so each prefetch lands in the registers it is
+    // consumed from (no loop-carried copies that would wait on the loads).
+    auto step = [&](int z, const float* use, float* into) {
+        const int cur = (z - zb) & 1;
+        if (src_ok && z + 1 < ze) {
+            source_plane(sp[cur ^ 1], use);
+            // issued after `use` is consumed, so waits on `use` never cover these
+            if (z + 3 <= ze) load_moments<DT>(a, (int64_t)(z + 3) * plane + scol, into);
         }
+        if (dst_ok) dest_plane(sp[cur], z);
         __syncthreads();
-        if (dst_ok) {
-            const int64_t pc = (int64_t)z * plane + pcol;
-            Acc acc;
-#define HL_PULL(I) accumulate<I>(acc, buf[I][cp - D3CX[I] - D3CY[I] * NX]);
-            HL_PULL(0) HL_PULL(1) HL_PULL(2) HL_PULL(3) HL_PULL(4) HL_PULL(5) HL_PULL(6) HL_PULL(7) HL_PULL(8)
-            HL_PULL(9) HL_PULL(10) HL_PULL(11) HL_PULL(12) HL_PULL(13) HL_PULL(14) HL_PULL(15) HL_PULL(16)
-            HL_PULL(17) HL_PULL(18) HL_PULL(19) HL_PULL(20) HL_PULL(21) HL_PULL(22) HL_PULL(23) HL_PULL(24)
-            HL_PULL(25) HL_PULL(26)
-#undef HL_PULL
-            float F[3] = {0.0f, 0.0f, 0.0f};
-            if (a.force) { F[0] = a.force[pc]; F[1] = a.force[a.stride + pc]; F[2] = a.force[2 * a.stride + pc]; }
-            float out[10];
-            collide27(acc, F, omega, out);
-            // faces: x, then y (this plane), then z (planes 0 and nz-1 from planes 1 and nz-2)
-            if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
-            else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
-            if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
-            else if (dy == 0) bc_face_closed(out, a.bc_type[3], a.bc_value[3], 1);
-            store_moments<DT>(a, (int64_t)z * plane + (int64_t)dy * nx + dx, out);
-            if (z == 1) {
-                float face[10];
-#pragma unroll
-                for (int k = 0; k < 10; ++k) face[k] = out[k];
-                bc_face_closed(face, a.bc_type[5], a.bc_value[5], 2);
-                store_moments<DT>(a, (int64_t)dy * nx + dx, face);
-            }
-            if (z == nz - 2) {
-                bc_face_closed(out, a.bc_type[4], a.bc_value[4], 2);
-                store_moments<DT>(a, (int64_t)(nz - 1) * plane + (int64_t)dy * nx + dx, out);
-            }
-        }
+    };
+    for (int z = zb; z < ze; z += 2) {
+        step(z, pfa, pfb);
+        if (z + 1 < ze) step(z + 1, pfb, pfa);
     }
 }
 
 // Tile shape of the D3Q27 kernel (set by hl_set_tiling / the autotuner).
-static TileCfg g_tile3d{32, 8, 16};
+static TileCfg g_tile3d{64, 8, 16};
 
 bool tiling_supported(int lattice, int tx, int ty, int zc)
 {
@@ -313,7 +338,10 @@ hipError_t launch_tiled(const EnvArgs& a, int zc, hipStream_t s)
     if (zc < 1) return hipErrorInvalidValue;
     constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
     dim3 grid((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (a.nz - 2 + zc - 1) / zc);
-    hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY>), grid, dim3(threads), 0, s, a, zc);
+    // the body-force field is a template switch: a run-time branch around its
+    // loads would make the compiler wait on the in-flight prefetch at the merge
+    if (a.force) hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, true>), grid, dim3(threads), 0, s, a, zc);
+    else hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, false>), grid, dim3(threads), 0, s, a, zc);
     return hipGetLastError();
 }
 
