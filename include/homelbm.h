@@ -122,6 +122,56 @@ int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* vert
 /* FP64 sum of rho over all cells of state `cur` into device out[0]. */
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batches: many resident D2Q9 FP32 lattices, of uneven sizes, advanced
+ * together with one launch per kernel kind per step (work-lists of
+ * (environment, tile) items). Replaces the nworld dimension of LBM_Solver
+ * (envs/lbm/lbm_solver.py:41-141: precompute_transformed_segments,
+ * stream_and_collide, apply_bc and Swap_Mom launched over (nworld, nx, ny),
+ * every world the same size) and its per-world init_force clears. Each
+ * environment's buffer parity and time index live in device memory and are
+ * advanced by the step itself, so a captured multi-step graph can be
+ * replayed. Results are bitwise equal to stepping each environment alone.
+ * ---------------------------------------------------------------------- */
+typedef struct hl_batch hl_batch;
+
+/* Build a batch over n descriptors (copied; the buffers they point to stay
+ * owned by the caller). Every environment starts with parity 0 (state in
+ * moments[0]) and time 0. */
+int32_t hl_batch_create(const hl_env* envs, int32_t n, hl_batch** out);
+void hl_batch_destroy(hl_batch* b);
+
+/* Bodies of environment env: device mask (0 outside the half-open box);
+ * optional device capsule table [nposes][ncap][8] re-voxelized into the box
+ * at the start of every step from entry time % nposes; optional device pose
+ * table [nposes][nsolid][8] (NULL = the descriptor's static poses). The
+ * prescribed-kinematics tables replace advance_prescribed_solid_motion_kernel
+ * (envs/lbm/karman/karman_lbm_env.py:65-77) and the pose-to-solid kernels
+ * (envs/lbm/lbm_fluid_env_func.py:72-105). Invalidates a captured graph. */
+int32_t hl_batch_set_bodies(hl_batch* b, int32_t env, const uint8_t* mask, const int32_t box[6], const float* caps,
+                            int32_t ncap, const float* poses, int32_t nposes);
+
+/* Host copies of every environment's buffer parity (0/1) and time index. */
+int32_t hl_batch_set_state(hl_batch* b, const int32_t* parity, const int32_t* time);
+int32_t hl_batch_get_state(hl_batch* b, int32_t* parity, int32_t* time);
+
+/* Advance nsteps. active: device uint8 [n] (NULL = all); inactive
+ * environments are untouched. force_history: NULL or device FP64
+ * [nsteps][n][16][6] receiving each step's force/torque of the state it reads
+ * (as hl_forces). per_env != 0 launches the same kernels one environment at
+ * a time (the baseline schedule; same results). */
+int32_t hl_batch_step(hl_batch* b, const uint8_t* active, int32_t nsteps, double* force_history, int32_t per_env,
+                      void* stream);
+
+/* Capture nsteps batched steps (arguments as hl_batch_step) in a graph owned
+ * by the batch, and replay it on a stream. Replaces the Warp graph capture
+ * of LBM_Solver.step (lbm_solver.py:107-141). */
+int32_t hl_batch_capture(hl_batch* b, const uint8_t* active, int32_t nsteps, double* force_history);
+int32_t hl_batch_replay(hl_batch* b, void* stream);
+
+/* Environments, step tiles and step tiles that reduce forces (any may be NULL). */
+int32_t hl_batch_size(const hl_batch* b, int32_t* nenv, int32_t* step_tiles, int32_t* force_tiles);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,7 +17,9 @@ HL_MAX_SOLIDS = 64
 
 #: every entry point declared in include/homelbm.h
 EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces", "hl_init_uniform",
-           "hl_voxelize", "hl_voxelize_polygons", "hl_mass", "hl_set_tiling", "hl_get_tiling")
+           "hl_voxelize", "hl_voxelize_polygons", "hl_mass", "hl_set_tiling", "hl_get_tiling",
+           "hl_batch_create", "hl_batch_destroy", "hl_batch_set_bodies", "hl_batch_set_state",
+           "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size")
 
 
 class HLEnv(C.Structure):
@@ -74,8 +76,20 @@ def load():
     L.hl_mass.argtypes = [env_p, I32, P, P]
     L.hl_set_tiling.argtypes = [I32, I32, I32, I32]
     L.hl_get_tiling.argtypes = [I32, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]
+    PI32 = C.POINTER(I32)
+    L.hl_batch_create.argtypes = [env_p, I32, C.POINTER(C.c_void_p)]
+    L.hl_batch_destroy.argtypes = [P]
+    L.hl_batch_destroy.restype = None
+    L.hl_batch_set_bodies.argtypes = [P, I32, P, PI32, P, I32, P, I32]
+    L.hl_batch_set_state.argtypes = [P, PI32, PI32]
+    L.hl_batch_get_state.argtypes = [P, PI32, PI32]
+    L.hl_batch_step.argtypes = [P, P, I32, P, I32, P]
+    L.hl_batch_capture.argtypes = [P, P, I32, P]
+    L.hl_batch_replay.argtypes = [P, P]
+    L.hl_batch_size.argtypes = [P, PI32, PI32, PI32]
     for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
-                 "hl_set_tiling", "hl_get_tiling"):
+                 "hl_set_tiling", "hl_get_tiling", "hl_batch_create", "hl_batch_set_bodies", "hl_batch_set_state",
+                 "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

@@ -195,50 +195,6 @@ static int reduce_blocks(int64_t n, int block = kReduceBlock, int maxb = kMaxRed
 
 
 
-// Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
-// torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
-template <typename T, int DT, int Q>
-__device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x, int y, int z, int i, int sid, T F[6])
-{
-    if (Q == 9) {
-        T up[2], r[2];
-        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
-        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
-        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
-        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
-        T pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
-        if (pop < T(0)) pop = T(0);
-        const int inv = D2INV[i];
-        T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
-        T Fx = cur_f * (T(D2CX[inv]) - up[0]) - pop * (T(D2CX[i]) - up[0]);
-        T Fy = cur_f * (T(D2CY[inv]) - up[1]) - pop * (T(D2CY[i]) - up[1]);
-        T grid_length = T(1) / T(a.nx);
-        F[0] = Fx; F[1] = Fy; F[2] = 0; F[3] = 0; F[4] = 0;
-        F[5] = (r[0] * Fy - r[1] * Fx) * grid_length;
-    } else {
-        const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
-        T up[3], r[3];
-        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
-        T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
-        T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
-        T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
-        T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
-        T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
-        T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
-        T pop = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, d3w<T>(i));
-        const int inv = D3INV[i];
-        const T ex = T(D3CX[inv]), ey = T(D3CY[inv]), ez = T(D3CZ[inv]);
-        T cur_f = d3q27_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[7], mc[8], mc[5], mc[9], mc[6], ex, ey, ez,
-                                 d3w<T>(inv));
-        T Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
-        T Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
-        T Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
-        F[0] = Fx; F[1] = Fy; F[2] = Fz;
-        F[3] = r[1] * Fz - r[2] * Fy;
-        F[4] = r[2] * Fx - r[0] * Fz;
-        F[5] = r[0] * Fy - r[1] * Fx;
-    }
-}
 
 // ------------------------------------------------------------------------
 // Near-body fix-up.

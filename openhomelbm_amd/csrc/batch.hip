@@ -1,0 +1,648 @@
+// Batched D2Q9 engine: many resident environments of uneven grid sizes
+// advanced together, one launch per kernel kind per step for all of them.
+//
+// Work-lists map each block to an (environment, tile). Per-environment state
+// that changes during an advance (ping-pong parity, kinematics time index)
+// lives in device memory and is advanced by the last kernel of each step, so
+// a multi-step advance needs no host involvement and can be captured in a
+// graph and replayed.
+//
+// Per step, for all active environments:
+//   k_bvox      rebuild the body mask in the body box from the capsule table
+//               at the environment's time t
+//   k_bstep     the D2Q9 update of one 64x4 tile staged with its halo in LDS
+//               (reference operation order, mask wall rule); tiles that
+//               meet the body also reduce the per-solid momentum-exchange
+//               force of the state they read into an FP64 partial
+//   k_badvance  per environment: fixed-order sum of its tiles' partials into
+//               the force history, then flip the parity and advance time.
+// The same kernels can be launched one environment at a time (the
+// per-environment baseline); every cell's arithmetic and every sum's order
+// are the same, so the two schedules give bitwise equal results.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gather_post.h"
+#include "homelbm.h"
+#include "kernels.h"
+#include "lbm_device.h"
+
+namespace hl {
+
+constexpr int kBatchMaxSolids = 16;
+// step tile: kTX x kTY cells, 256 threads; lane = column, each wave kRows rows
+constexpr int kTX = 64, kTY = 4;
+constexpr int kThreads = 256, kWaves = kThreads / 64, kRows = kTY / kWaves;
+constexpr int kLX = kTX + 3, kLY = kTY + 3;  // staged region: 2 cells below, 1 above
+constexpr int kStage = (kLX * kLY + kThreads - 1) / kThreads;
+constexpr int kVoxW = 16;         // voxelizer item: 16 x 16 cells
+constexpr int kPartialWidth = kBatchMaxSolids * 3;  // Fx, Fy, Tz per solid
+
+struct BatchEnv {
+    EnvArgs base;         // arguments for parity 0 (moments[0] -> moments[1])
+    const float* caps;    // [T][ncap][8] capsule table (null: static mask)
+    const float* poses;   // [T][nsolid][8] pose table (null: static poses in base.solids)
+    int32_t ncap, T;
+    int32_t vox_begin, vox_count;    // voxelizer items
+    int32_t step_begin, step_count;  // step tiles
+    int32_t slot_begin, slot_count;  // force partial slots (step tiles meeting the body)
+};
+
+struct BatchItem {  // env, origin and extent of a rectangle of cells; partial slot or -1
+    int32_t env, x0, y0, w, h, slot;
+};
+
+struct BatchArgs {
+    const BatchEnv* envs;
+    const BatchItem* step_items;
+    const BatchItem* vox_items;
+    int32_t* parity;        // per env
+    int32_t* time;          // per env
+    const uint8_t* active;  // per env (null = all)
+    double* partial;        // [slots][kPartialWidth]
+};
+
+__device__ __forceinline__ bool env_active(const BatchArgs& b, int e) { return b.active == nullptr || b.active[e]; }
+
+// Per-block copy of the environment's arguments in LDS, with the source,
+// destination and poses of its current step.
+__device__ __forceinline__ void load_env(const BatchArgs& b, int e, EnvArgs& ea)
+{
+    static_assert(sizeof(EnvArgs) % 4 == 0 && sizeof(EnvArgs) / 4 <= 256, "EnvArgs copy");
+    const BatchEnv& be = b.envs[e];
+    if (threadIdx.x < sizeof(EnvArgs) / 4)
+        reinterpret_cast<uint32_t*>(&ea)[threadIdx.x] = reinterpret_cast<const uint32_t*>(&be.base)[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (b.parity[e] & 1) {
+            const void* t = ea.src;
+            ea.src = ea.dst;
+            ea.dst = const_cast<void*>(t);
+        }
+        if (be.poses && be.T > 0) ea.solids = be.poses + (int64_t)(b.time[e] % be.T) * ea.nsolid * 8;
+    }
+    __syncthreads();
+}
+
+// Capsule voxelizer over one item (the same evaluation as the single-lattice
+// voxelizer): mask = solid id + 1 of the lowest-index capsule whose segment
+// lies within its radius of the cell centre, else 0.
+__global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0)
+{
+    const BatchItem it = b.vox_items[item0 + blockIdx.x];
+    if (!env_active(b, it.env)) return;
+    const BatchEnv& be = b.envs[it.env];
+    if (!be.caps || be.T <= 0) return;
+    const float* caps = be.caps + (int64_t)(b.time[it.env] % be.T) * be.ncap * 8;
+    const int k = threadIdx.x;
+    if (k >= it.w * it.h) return;
+    const int x = it.x0 + k % it.w, y = it.y0 + k / it.w;
+    const float px = (float)x, py = (float)y, pz = 0.0f;
+    uint8_t v = 0;
+    for (int j = 0; j < be.ncap; ++j) {
+        const float* C = caps + 8 * j;
+        float abx = C[3] - C[0], aby = C[4] - C[1], abz = C[5] - C[2];
+        float apx = px - C[0], apy = py - C[1], apz = pz - C[2];
+        float len2 = abx * abx + aby * aby + abz * abz;
+        float tt = 0.0f;
+        if (len2 > 0.0f) {
+            tt = (apx * abx + apy * aby + apz * abz) / len2;
+            tt = tt < 0.0f ? 0.0f : (tt > 1.0f ? 1.0f : tt);
+        }
+        float dx = apx - tt * abx, dy = apy - tt * aby, dz = apz - tt * abz;
+        float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 <= C[6] * C[6]) { v = (uint8_t)((int)C[7] + 1); break; }
+    }
+    const_cast<uint8_t*>(be.base.mask)[(int64_t)x + (int64_t)be.base.nx * y] = v;
+}
+
+__device__ __forceinline__ double bwave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
+{
+    __shared__ EnvArgs ea;
+    __shared__ float tile[6][kLY][kLX];
+    __shared__ uint8_t mtile[kLY][kLX + 1];
+    __shared__ float pose[kBatchMaxSolids * 8];
+    __shared__ double acc[kWaves][kPartialWidth];
+    const BatchItem it = b.step_items[item0 + blockIdx.x];
+    const int e = it.env;
+    if (!env_active(b, e)) return;
+    const int tid = threadIdx.x;
+    const bool near = it.slot >= 0;
+    const BatchEnv& be = b.envs[e];
+    const int par = b.parity[e] & 1;
+    const int nx = be.base.nx, ny = be.base.ny, nsolid = be.base.nsolid;
+    const int64_t st = be.base.stride;
+    const float* src_p = static_cast<const float*>(par ? be.base.dst : be.base.src);
+    const auto src = as_global(src_p);
+    const auto mask = as_global(be.base.mask);
+    // the staged region [x0-2, x0+kTX] x [y0-2, y0+kTY] reaches the body box
+    // (the mask is 0 outside it): stage the mask and evaluate the wall rule
+    const bool masked = mask != nullptr && it.x0 - 2 <= be.base.box[1] && it.x0 + kTX >= be.base.box[0] - 1 &&
+                        it.y0 - 2 <= be.base.box[3] && it.y0 + kTY >= be.base.box[2] - 1;
+    const float* poses = be.poses && be.T > 0 ? be.poses + (int64_t)(b.time[e] % be.T) * nsolid * 8 : be.base.solids;
+    // stage the tile with a halo of two cells below and one above in x and y
+    // (so every face cell's interior projection finds its sources here); all
+    // loads are issued before the first wait
+    float v[kStage][6];
+    int mv[kStage];
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+        const int k = tid + r * kThreads;
+        const int lx = k % kLX, ly = k / kLX;
+        const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
+        const bool in = k < kLX * kLY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const int64_t c = (int64_t)(in ? gx : 0) + (int64_t)nx * (in ? gy : 0);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) v[r][f] = in ? src[f * st + c] : 0.0f;
+        mv[r] = (masked && in) ? mask[c] : 0;
+    }
+    float pv = 0.0f;
+    if (masked && poses && tid < nsolid * 8) pv = poses[tid];
+    static_assert(sizeof(EnvArgs) % 4 == 0 && sizeof(EnvArgs) / 4 <= kThreads, "EnvArgs copy");
+    if (tid < sizeof(EnvArgs) / 4)
+        reinterpret_cast<uint32_t*>(&ea)[tid] = reinterpret_cast<const uint32_t*>(&be.base)[tid];
+    if (near)
+        for (int k = tid; k < kWaves * kPartialWidth; k += kThreads) (&acc[0][0])[k] = 0.0;
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+        const int k = tid + r * kThreads;
+        if (k < kLX * kLY) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) tile[f][k / kLX][k % kLX] = v[r][f];
+            mtile[k / kLX][k % kLX] = (uint8_t)mv[r];
+        }
+    }
+    if (tid < kBatchMaxSolids * 8) pose[tid] = pv;
+    __syncthreads();
+    if (tid == 0) {
+        ea.src = src_p;
+        ea.dst = const_cast<void*>(par ? be.base.src : be.base.dst);
+        ea.solids = pose;  // wall velocities read the staged pose table
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int x = it.x0 + lane;
+    const auto dst = as_global(static_cast<float*>(ea.dst));
+    uint32_t cutbits[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int y = it.y0 + wave * kRows + r;
+        cutbits[r] = 0;
+        if (x >= nx || y >= ny) continue;
+        // a face cell takes its interior projection's update, then the face chain
+        const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
+        const int tx = px - it.x0 + 2, ty = py - it.y0 + 2;
+        const int mval_c = masked ? mtile[ty][tx] : 0;
+        float mc[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
+        // stream_and_collide (lbm_func.py:322-467) from the staged moments
+        float pop[9];
+        uint32_t cb = 0;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            int cut = 0, sid = -1;
+            if (masked) {
+                const int mval_s = mtile[ty - D2CY[i]][tx - D2CX[i]];
+                if ((mval_s != 0) != (mval_c != 0)) {
+                    cut = 1;
+                    sid = mval_c ? mval_c - 1 : mval_s - 1;
+                    cb |= 1u << i;
+                }
+            }
+            float ms[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) ms[f] = tile[f][ty - D2CY[i]][tx - D2CX[i]];
+            pop[i] = d2q9_link_pop<float>(ea, px, py, i, mc, cut, sid, ms);
+        }
+        float m[6];
+        d2q9_collide<float>(ea, pop, 0.0f, 0.0f, m);
+        if (px != x || py != y) bc_chain<float, 2>(m, x, y, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
+        else cutbits[r] = cb;  // forces come from interior cells only
+        const int64_t c = (int64_t)x + (int64_t)nx * y;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) dst[f * st + c] = m[f];
+    }
+    if (!near) return;
+    // momentum exchange of the tile's cut links per solid: each lane's cells
+    // and links in order, a wavefront butterfly, then the block's waves in order
+    uint32_t sids = 0;
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int ty = wave * kRows + r + 2, tx = lane + 2;
+        for (int i = 1; i < 9; ++i)
+            if ((cutbits[r] >> i) & 1u) {
+                const int mvc = mtile[ty][tx], mvs = mtile[ty - D2CY[i]][tx - D2CX[i]];
+                const int s = mvc ? mvc - 1 : mvs - 1;
+                if (s < nsolid) sids |= 1u << s;
+            }
+    }
+    uint32_t uni = sids;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) uni |= (uint32_t)__shfl_xor((int)uni, off, 64);
+    while (uni) {
+        const int s = __builtin_ctz(uni);
+        uni &= uni - 1;
+        double sum[3] = {0.0, 0.0, 0.0};
+        if ((sids >> s) & 1u) {
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                if (!cutbits[r]) continue;
+                const int ty = wave * kRows + r + 2, tx = lane + 2;
+                const int y = it.y0 + ty - 2;
+                const int mvc = mtile[ty][tx];
+                float mc[6];
+#pragma unroll
+                for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
+                for (int i = 1; i < 9; ++i) {
+                    if (!((cutbits[r] >> i) & 1u)) continue;
+                    const int mvs = mtile[ty - D2CY[i]][tx - D2CX[i]];
+                    if ((mvc ? mvc - 1 : mvs - 1) != s) continue;
+                    float F[6];
+                    link_force<float, HL_F32, 9>(ea, mc, x, y, 0, i, s, F);
+                    sum[0] += (double)F[0];
+                    sum[1] += (double)F[1];
+                    sum[2] += (double)F[5];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) sum[q] = bwave_sum(sum[q]);
+        if (lane == 0)
+            for (int q = 0; q < 3; ++q) acc[wave][s * 3 + q] = sum[q];
+    }
+    __syncthreads();
+    if (tid < kPartialWidth) {
+        double t = 0.0;
+        for (int w = 0; w < kWaves; ++w) t += acc[w][tid];
+        as_global(b.partial)[(int64_t)it.slot * kPartialWidth + tid] = t;
+    }
+}
+
+// Per environment (one block): the fixed-order sum of its partial slots into
+// the force history [k][env][16][6] (Fx, Fy, 0, 0, 0, Tz), then flip the
+// parity and advance the time.
+__global__ void __launch_bounds__(64) k_badvance(BatchArgs b, int env0, double* history, int nenv_total, int k)
+{
+    __shared__ double vals[kPartialWidth];
+    const int e = env0 + blockIdx.x;
+    if (!env_active(b, e)) return;
+    const BatchEnv& be = b.envs[e];
+    const int q = threadIdx.x;
+    if (history) {
+        if (q < kPartialWidth) {
+            const double* p = b.partial + (int64_t)be.slot_begin * kPartialWidth + q;
+            const int n = be.slot_count;
+            double v = 0.0;
+            int j = 0;
+            for (; j + 8 <= n; j += 8) {  // loads issued together, added in slot order
+                double t[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) t[u] = p[(int64_t)(j + u) * kPartialWidth];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v += t[u];
+            }
+            for (; j < n; ++j) v += p[(int64_t)j * kPartialWidth];
+            vals[q] = v;
+        }
+        __syncthreads();
+        double* h = history + ((int64_t)k * nenv_total + e) * kBatchMaxSolids * 6;
+        for (int o = q; o < kBatchMaxSolids * 6; o += 64) {
+            const int s = o / 6, comp = o % 6;
+            h[o] = comp == 0 ? vals[s * 3] : comp == 1 ? vals[s * 3 + 1] : comp == 5 ? vals[s * 3 + 2] : 0.0;
+        }
+    }
+    if (q == 0) {
+        b.parity[e] ^= 1;
+        b.time[e] += 1;
+    }
+}
+
+}  // namespace hl
+
+// ----------------------------------------------------------------- C ABI
+
+struct hl_batch {
+    int n = 0;
+    std::vector<hl::BatchEnv> envs;  // host mirror of the device table
+    std::vector<hl::BatchItem> step_items, vox_items;
+    int nslots = 0;
+    hl::BatchEnv* d_envs = nullptr;
+    hl::BatchItem *d_step = nullptr, *d_vox = nullptr;
+    size_t cap_envs = 0, cap_step = 0, cap_vox = 0, cap_partial = 0;
+    int32_t *d_parity = nullptr, *d_time = nullptr;
+    size_t cap_parity = 0, cap_time = 0;
+    double* d_partial = nullptr;
+    bool dirty = true;  // table or item lists changed since the last upload
+    hipStream_t cap_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+using hl::report;
+
+int32_t hip_report(hipError_t e, const char* where)
+{
+    std::string m = std::string(where) + ": " + hipGetErrorString(e);
+    return report(HL_E_HIP, m.c_str());
+}
+
+template <typename T> hipError_t upload(T** dptr, size_t* cap, const T* data, size_t count)
+{
+    const size_t need = count ? count : 1;
+    if (!*dptr || *cap < need) {
+        if (*dptr) {
+            hipError_t e = hipFree(*dptr);
+            if (e != hipSuccess) return e;
+        }
+        *dptr = nullptr;
+        *cap = 0;
+        hipError_t e = hipMalloc(dptr, need * sizeof(T));
+        if (e != hipSuccess) return e;
+        *cap = need;
+    }
+    if (!count) return hipSuccess;
+    return hipMemcpy(*dptr, data, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Step tiles cover every environment; tiles that meet the force region
+// (interior cells within one cell of the mask box, where every cut link has
+// its fluid end) get a partial slot. Voxelizer items tile the mask box.
+void rebuild_items(hl_batch* b)
+{
+    b->step_items.clear();
+    b->vox_items.clear();
+    int slot = 0;
+    for (int e = 0; e < b->n; ++e) {
+        hl::BatchEnv& be = b->envs[e];
+        const hl::EnvArgs& a = be.base;
+        const bool body = a.mask && a.box[0] < a.box[1] && a.box[2] < a.box[3];
+        const int fx0 = std::max(a.box[0] - 1, 1), fx1 = std::min(a.box[1] + 1, a.nx - 1);
+        const int fy0 = std::max(a.box[2] - 1, 1), fy1 = std::min(a.box[3] + 1, a.ny - 1);
+        const bool forces = body && a.nsolid > 0 && fx0 < fx1 && fy0 < fy1;
+        be.step_begin = (int)b->step_items.size();
+        be.slot_begin = slot;
+        for (int y = 0; y < a.ny; y += hl::kTY)
+            for (int x = 0; x < a.nx; x += hl::kTX) {
+                const bool meets = forces && x < fx1 && x + hl::kTX > fx0 && y < fy1 && y + hl::kTY > fy0;
+                b->step_items.push_back({e, x, y, hl::kTX, hl::kTY, meets ? slot++ : -1});
+            }
+        be.step_count = (int)b->step_items.size() - be.step_begin;
+        be.slot_count = slot - be.slot_begin;
+        be.vox_begin = (int)b->vox_items.size();
+        if (body && be.caps)
+            for (int y = a.box[2]; y < a.box[3]; y += hl::kVoxW)
+                for (int x = a.box[0]; x < a.box[1]; x += hl::kVoxW)
+                    b->vox_items.push_back(
+                        {e, x, y, std::min(hl::kVoxW, a.box[1] - x), std::min(hl::kVoxW, a.box[3] - y), -1});
+        be.vox_count = (int)b->vox_items.size() - be.vox_begin;
+    }
+    b->nslots = slot;
+    b->dirty = true;
+}
+
+int32_t sync_device(hl_batch* b)
+{
+    if (!b->dirty) return HL_OK;
+    hipError_t e;
+    if ((e = upload(&b->d_envs, &b->cap_envs, b->envs.data(), b->envs.size())) != hipSuccess ||
+        (e = upload(&b->d_step, &b->cap_step, b->step_items.data(), b->step_items.size())) != hipSuccess ||
+        (e = upload(&b->d_vox, &b->cap_vox, b->vox_items.data(), b->vox_items.size())) != hipSuccess)
+        return hip_report(e, "batch work-list upload");
+    const size_t np = (size_t)std::max(b->nslots, 1) * hl::kPartialWidth;
+    if (!b->d_partial || b->cap_partial < np) {
+        if (b->d_partial && (e = hipFree(b->d_partial)) != hipSuccess) return hip_report(e, "hipFree");
+        b->d_partial = nullptr;
+        b->cap_partial = 0;
+        if ((e = hipMalloc(&b->d_partial, np * sizeof(double))) != hipSuccess)
+            return hip_report(e, "batch partial-sum allocation");
+        b->cap_partial = np;
+    }
+    b->dirty = false;
+    return HL_OK;
+}
+
+hl::BatchArgs batch_args(const hl_batch* b, const uint8_t* active)
+{
+    hl::BatchArgs a;
+    a.envs = b->d_envs;
+    a.step_items = b->d_step;
+    a.vox_items = b->d_vox;
+    a.parity = b->d_parity;
+    a.time = b->d_time;
+    a.active = active;
+    a.partial = b->d_partial;
+    return a;
+}
+
+int32_t enqueue(hl_batch* b, const uint8_t* active, int32_t nsteps, double* history, int32_t per_env, hipStream_t s)
+{
+    const hl::BatchArgs a = batch_args(b, active);
+    const int nvox = (int)b->vox_items.size(), nstep = (int)b->step_items.size();
+    for (int k = 0; k < nsteps; ++k) {
+        if (!per_env) {
+            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0);
+            hipLaunchKernelGGL(hl::k_bstep, dim3(nstep), dim3(hl::kThreads), 0, s, a, 0);
+            hipLaunchKernelGGL(hl::k_badvance, dim3(b->n), dim3(64), 0, s, a, 0, history, b->n, k);
+        } else {
+            // the same kernels, one environment per launch
+            for (int e = 0; e < b->n; ++e) {
+                const hl::BatchEnv& be = b->envs[e];
+                if (be.vox_count) hipLaunchKernelGGL(hl::k_bvox, dim3(be.vox_count), dim3(256), 0, s, a, be.vox_begin);
+                hipLaunchKernelGGL(hl::k_bstep, dim3(be.step_count), dim3(hl::kThreads), 0, s, a, be.step_begin);
+                hipLaunchKernelGGL(hl::k_badvance, dim3(1), dim3(64), 0, s, a, e, history, b->n, k);
+            }
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_report(e, "batch kernel launch");
+    }
+    return HL_OK;
+}
+
+void drop_graph(hl_batch* b)
+{
+    if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t hl_batch_create(const hl_env* envs, int32_t n, hl_batch** out)
+{
+    if (!out) return report(HL_E_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!envs || n <= 0) return report(HL_E_INVALID, "a batch needs at least one environment");
+    for (int e = 0; e < n; ++e) {
+        int32_t rc = hl::check_env(&envs[e]);
+        if (rc) return rc;
+        const hl_env& x = envs[e];
+        if (x.lattice != HL_D2Q9 || x.dtype != HL_F32 || x.force || x.nsolid > hl::kBatchMaxSolids) {
+            std::string m = "batch environment " + std::to_string(e) +
+                            ": batches hold D2Q9 FP32 lattices with no body-force field and at most 16 solids";
+            return report(HL_E_UNSUPPORTED, m.c_str());
+        }
+    }
+    hl_batch* b = new (std::nothrow) hl_batch();
+    if (!b) return report(HL_E_NOMEM, "out of host memory");
+    b->n = n;
+    b->envs.resize(n);
+    for (int e = 0; e < n; ++e) {
+        hl::BatchEnv& be = b->envs[e];
+        std::memset(&be, 0, sizeof(be));
+        be.base = hl::env_args(&envs[e], 0);
+        be.base.cut_bits = nullptr;
+    }
+    rebuild_items(b);
+    std::vector<int32_t> zeros(n, 0);
+    hipError_t he;
+    int32_t rc = sync_device(b);
+    if (!rc && ((he = upload(&b->d_parity, &b->cap_parity, zeros.data(), zeros.size())) != hipSuccess ||
+                (he = upload(&b->d_time, &b->cap_time, zeros.data(), zeros.size())) != hipSuccess))
+        rc = hip_report(he, "batch state allocation");
+    if (rc) {
+        hl_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return HL_OK;
+}
+
+void hl_batch_destroy(hl_batch* b)
+{
+    if (!b) return;
+    drop_graph(b);
+    if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+    hipFree(b->d_envs);
+    hipFree(b->d_step);
+    hipFree(b->d_vox);
+    hipFree(b->d_parity);
+    hipFree(b->d_time);
+    hipFree(b->d_partial);
+    delete b;
+}
+
+int32_t hl_batch_set_bodies(hl_batch* b, int32_t env, const uint8_t* mask, const int32_t box[6], const float* caps,
+                            int32_t ncap, const float* poses, int32_t nposes)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (env < 0 || env >= b->n) return report(HL_E_INVALID, "environment index out of range");
+    hl::BatchEnv& be = b->envs[env];
+    hl::EnvArgs& a = be.base;
+    if (!mask || !box) return report(HL_E_INVALID, "mask and box are required");
+    if (box[0] < 0 || box[1] > a.nx || box[2] < 0 || box[3] > a.ny || box[0] > box[1] || box[2] > box[3])
+        return report(HL_E_INVALID, "body box outside the grid");
+    if (ncap < 0 || nposes < 0) return report(HL_E_INVALID, "negative table length");
+    if (ncap > 0 && (!caps || nposes <= 0)) return report(HL_E_INVALID, "a capsule table needs caps and nposes > 0");
+    if (poses && nposes <= 0) return report(HL_E_INVALID, "a pose table needs nposes > 0");
+    if (a.nsolid > 0 && !poses && !a.solids) return report(HL_E_INVALID, "no pose table for the solids");
+    a.mask = mask;
+    a.box[0] = box[0]; a.box[1] = box[1]; a.box[2] = box[2]; a.box[3] = box[3];
+    a.box[4] = 0;
+    a.box[5] = 1;
+    be.caps = ncap > 0 ? caps : nullptr;
+    be.ncap = ncap;
+    be.poses = poses;
+    be.T = nposes;
+    drop_graph(b);
+    rebuild_items(b);
+    return sync_device(b);
+}
+
+int32_t hl_batch_set_state(hl_batch* b, const int32_t* parity, const int32_t* time)
+{
+    if (!b || !parity || !time) return report(HL_E_INVALID, "batch, parity and time are required");
+    for (int e = 0; e < b->n; ++e)
+        if ((parity[e] != 0 && parity[e] != 1) || time[e] < 0) return report(HL_E_INVALID, "parity must be 0/1, time >= 0");
+    hipError_t e;
+    if ((e = hipMemcpy(b->d_parity, parity, b->n * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_time, time, b->n * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_report(e, "batch state upload");
+    return HL_OK;
+}
+
+int32_t hl_batch_get_state(hl_batch* b, int32_t* parity, int32_t* time)
+{
+    if (!b || !parity || !time) return report(HL_E_INVALID, "batch, parity and time are required");
+    hipError_t e;
+    if ((e = hipMemcpy(parity, b->d_parity, b->n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(time, b->d_time, b->n * sizeof(int32_t), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_report(e, "batch state download");
+    return HL_OK;
+}
+
+int32_t hl_batch_step(hl_batch* b, const uint8_t* active, int32_t nsteps, double* force_history, int32_t per_env,
+                      void* stream)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (nsteps < 0) return report(HL_E_INVALID, "nsteps must be >= 0");
+    int32_t rc = sync_device(b);
+    if (rc) return rc;
+    return enqueue(b, active, nsteps, force_history, per_env, static_cast<hipStream_t>(stream));
+}
+
+int32_t hl_batch_capture(hl_batch* b, const uint8_t* active, int32_t nsteps, double* force_history)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (nsteps <= 0) return report(HL_E_INVALID, "nsteps must be > 0");
+    int32_t rc = sync_device(b);
+    if (rc) return rc;
+    drop_graph(b);
+    hipError_t e;
+    if (!b->cap_stream && (e = hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking)) != hipSuccess)
+        return hip_report(e, "hipStreamCreate");
+    if ((e = hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+        return hip_report(e, "hipStreamBeginCapture");
+    rc = enqueue(b, active, nsteps, force_history, 0, b->cap_stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->cap_stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_report(e, "hipStreamEndCapture");
+    b->graph = g;
+    if ((e = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0)) != hipSuccess) {
+        drop_graph(b);
+        return hip_report(e, "hipGraphInstantiate");
+    }
+    return HL_OK;
+}
+
+int32_t hl_batch_replay(hl_batch* b, void* stream)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (!b->graph_exec) return report(HL_E_INVALID, "no captured graph (hl_batch_capture first)");
+    if (b->dirty) return report(HL_E_INVALID, "bodies changed since the capture; capture again");
+    hipError_t e = hipGraphLaunch(b->graph_exec, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_report(e, "hipGraphLaunch");
+    return HL_OK;
+}
+
+int32_t hl_batch_size(const hl_batch* b, int32_t* nenv, int32_t* step_tiles, int32_t* force_tiles)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (nenv) *nenv = b->n;
+    if (step_tiles) *step_tiles = (int32_t)b->step_items.size();
+    if (force_tiles) *force_tiles = b->nslots;
+    return HL_OK;
+}
+
+}  // extern "C"

@@ -201,6 +201,12 @@ template <typename T> void uniform_moments(int lattice, T ux, T uy, T uz, double
 
 }  // namespace
 
+namespace hl {
+int32_t report(int32_t code, const char* msg) { return fail(code, "%s", msg); }
+int32_t check_env(const hl_env* e) { return validate(e); }
+EnvArgs env_args(const hl_env* e, int cur) { return make_args(e, cur); }
+}  // namespace hl
+
 extern "C" {
 
 int32_t hl_version(void) { return 100; }

@@ -11,10 +11,30 @@
 
 namespace hl {
 
-// Population i pulled into interior cell (x, y) (lbm_func.py:335-412): the
-// reconstruction from the source cell's moments, or on a cut link the
-// wall-rule population from the cell's own moments mc; clamped at 0 on both
-// branches as in the reference.
+// Population i pulled into interior cell (x, y) (lbm_func.py:335-412) once
+// the link is classified: on a cut link (solid sid) the wall-rule population
+// from the cell's own moments mc, else the reconstruction from the source
+// cell's moments ms; clamped at 0 on both branches as in the reference.
+template <typename T>
+__device__ __forceinline__ T d2q9_link_pop(const EnvArgs& a, int x, int y, int i, const T* mc, int cut, int sid,
+                                           const T* ms)
+{
+    T pop;
+    if (cut) {
+        T up[2], r[2];
+        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+        pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+    } else {
+        pop = d2q9_recon<T>(ms[0], ms[1], ms[2], ms[3], ms[4], ms[5], i);
+    }
+    if (pop < T(0)) pop = T(0);
+    return pop;
+}
+
+// The same with the source moments read from global memory.
 template <typename T, int DT>
 __device__ __forceinline__ T d2q9_pull(const EnvArgs& a, int x, int y, int i, const T* mc, bool masked, int mval_c)
 {
@@ -28,22 +48,12 @@ __device__ __forceinline__ T d2q9_pull(const EnvArgs& a, int x, int y, int i, co
         const int mval_s = a.mask[s];
         if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
     }
-    T pop;
-    if (cut) {
-        T up[2], r[2];
-        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
-        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
-        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
-        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
-        pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
-    } else {
-        T m[6];
+    T m[6];
+    if (!cut) {
 #pragma unroll
         for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
-        pop = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
     }
-    if (pop < T(0)) pop = T(0);
-    return pop;
+    return d2q9_link_pop<T>(a, x, y, i, mc, cut, sid, m);
 }
 
 // Moments and regularized collision of the 9 pulled populations
@@ -263,6 +273,51 @@ __device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z
     T Fx = 0, Fy = 0, Fz = 0;
     if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); Fz = T(a.force[2 * a.stride + c]); }
     d3q27_collide<T>(a, pop, Fx, Fy, Fz, out);
+}
+
+// Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
+// torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
+template <typename T, int DT, int Q>
+__device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x, int y, int z, int i, int sid, T F[6])
+{
+    if (Q == 9) {
+        T up[2], r[2];
+        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+        T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+        T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+        T pop = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+        if (pop < T(0)) pop = T(0);
+        const int inv = D2INV[i];
+        T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+        T Fx = cur_f * (T(D2CX[inv]) - up[0]) - pop * (T(D2CX[i]) - up[0]);
+        T Fy = cur_f * (T(D2CY[inv]) - up[1]) - pop * (T(D2CY[i]) - up[1]);
+        T grid_length = T(1) / T(a.nx);
+        F[0] = Fx; F[1] = Fy; F[2] = 0; F[3] = 0; F[4] = 0;
+        F[5] = (r[0] * Fy - r[1] * Fx) * grid_length;
+    } else {
+        const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
+        T up[3], r[3];
+        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+        T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
+        T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
+        T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
+        T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
+        T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
+        T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
+        T pop = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, d3w<T>(i));
+        const int inv = D3INV[i];
+        const T ex = T(D3CX[inv]), ey = T(D3CY[inv]), ez = T(D3CZ[inv]);
+        T cur_f = d3q27_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[7], mc[8], mc[5], mc[9], mc[6], ex, ey, ez,
+                                 d3w<T>(inv));
+        T Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
+        T Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
+        T Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
+        F[0] = Fx; F[1] = Fy; F[2] = Fz;
+        F[3] = r[1] * Fz - r[2] * Fy;
+        F[4] = r[2] * Fx - r[0] * Fz;
+        F[5] = r[0] * Fy - r[1] * Fx;
+    }
 }
 
 }  // namespace hl

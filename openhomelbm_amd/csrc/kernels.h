@@ -39,6 +39,13 @@ void cut_bits_view(int lattice, EnvArgs& a, void* ws);
 // per-solid force/torque of the state read (when out != NULL).
 hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* out, void* ws, hipStream_t s);
 size_t mass_workspace_bytes(const EnvArgs& a, int lattice);
+
+// Shared with the batch entry points (defined in capi.hip): set the calling
+// thread's hl_last_error message and return code; validate a descriptor;
+// kernel arguments of a descriptor reading state cur.
+int32_t report(int32_t code, const char* msg);
+int32_t check_env(const hl_env* e);
+EnvArgs env_args(const hl_env* e, int cur);
 hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
 
 }  // namespace hl

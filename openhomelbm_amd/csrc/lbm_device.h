@@ -294,6 +294,17 @@ __device__ __forceinline__ void wall_velocity_3d(const float* solids, int nsolid
     r[0] = d0; r[1] = d1; r[2] = d2;
 }
 
+// Global-memory view of a generic pointer. Pointers read from memory (not
+// kernel arguments) are generic to the compiler, so their accesses become
+// flat instructions, which count against the LDS wait counter as well and
+// make LDS reads wait for outstanding stores. Casting them lets the compiler
+// emit global loads and stores.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p)
+{
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // Kernel-side view of an hl_env for one precision (by value, so it can be a
 // kernel argument and a graph node parameter).
 struct EnvArgs {

@@ -116,3 +116,154 @@ def build_lattice(w: Workload, dtype: str = "f32"):
         else:
             f[1:1 + d] += noise.to(f.dtype)
     return lat
+
+
+# ----------------------------------------------------------------- eels
+
+EEL_SIZES = ((192, 96), (256, 128), (384, 160), (512, 192))
+EEL_SEGMENTS = 8
+EEL_RADIUS = 4.0
+EEL_U = 0.05
+EEL_VMAX = 0.1
+
+
+@dataclass
+class Eel:
+    """An 8-capsule planar chain with sinusoidal CPG joint angles.
+
+    Joint j (1..7, between segments j-1 and j) follows
+        theta_j(t) = A * env_j * sin(omega * t - j * phi),  env_j = 0.05 + 0.95 * j / 7,
+    the envelope of the reference's eel_wave preset
+    (tools/lbm2d_realtime_control.py:337-370); segment 0 keeps heading 0 (the
+    chain trails downstream of its head). omega = 2 pi f / 1000 per lattice
+    step, slowed (time_scale < 1) so that no body point moves faster than
+    EEL_VMAX lattice units per step: taken literally, in-phase chains reach
+    tail speeds above 1 (faster than the lattice sound speed).
+    """
+
+    nx: int
+    ny: int
+    amplitude: float
+    frequency: float   # cycles per 1000 steps, before time_scale
+    phase_lag: float
+    head: Tuple[float, float]
+    seg_len: float
+    radius: float = EEL_RADIUS
+    time_scale: float = 1.0
+
+    @property
+    def length(self) -> float:
+        return self.seg_len * EEL_SEGMENTS
+
+    @property
+    def nu(self) -> float:
+        """Re = 300 on the chain length at U = 0.05."""
+        return EEL_U * self.length / 300.0
+
+    def joints(self, t) -> np.ndarray:
+        """FP64 joint positions [..., 9, 2] at (possibly fractional) times t."""
+        t = np.asarray(t, dtype=np.float64)[..., None]
+        j = np.arange(EEL_SEGMENTS, dtype=np.float64)
+        env = 0.05 + 0.95 * j / (EEL_SEGMENTS - 1)
+        omega = 2.0 * np.pi * self.frequency / 1000.0 * self.time_scale
+        theta = self.amplitude * env * np.sin(omega * t - j * self.phase_lag)
+        theta[..., 0] = 0.0
+        heading = np.cumsum(theta, axis=-1)
+        seg = self.seg_len * np.stack([np.cos(heading), np.sin(heading)], axis=-1)
+        p = np.zeros(t.shape[:-1] + (EEL_SEGMENTS + 1, 2))
+        p[..., 0, :] = self.head
+        p[..., 1:, :] = self.head + np.cumsum(seg, axis=-2)
+        return p
+
+    def tables(self, t0: int, count: int):
+        """FP32 capsule [count, 8, 8] and pose [count, 8, 8] tables for steps
+        t0 .. t0+count-1 (computed in FP64, rounded once). Capsule j runs from
+        joint j to joint j+1 with solid id j; pose j is (joint j, cos, sin of
+        the segment heading) at t and at t - 1."""
+        t = np.arange(t0, t0 + count, dtype=np.float64)
+        now, last = self.joints(t), self.joints(t - 1.0)
+        caps = np.zeros((count, EEL_SEGMENTS, 8))
+        caps[..., 0:2] = now[:, :-1]
+        caps[..., 3:5] = now[:, 1:]
+        caps[..., 6] = self.radius
+        caps[..., 7] = np.arange(EEL_SEGMENTS)
+        poses = np.zeros((count, EEL_SEGMENTS, 8))
+        for k, p in ((0, now), (4, last)):
+            d = p[:, 1:] - p[:, :-1]
+            h = np.arctan2(d[..., 1], d[..., 0])
+            poses[..., k:k + 2] = p[:, :-1]
+            poses[..., k + 2] = np.cos(h)
+            poses[..., k + 3] = np.sin(h)
+        return caps.astype(np.float32), poses.astype(np.float32)
+
+    def box(self, t0: int, count: int):
+        """Half-open cell box covering the chain over the steps, padded by 1."""
+        p = self.joints(np.arange(t0, t0 + count, dtype=np.float64))
+        lo = p.reshape(-1, 2).min(0) - self.radius
+        hi = p.reshape(-1, 2).max(0) + self.radius
+        x0, y0 = int(np.floor(lo[0])) - 1, int(np.floor(lo[1])) - 1
+        x1, y1 = int(np.ceil(hi[0])) + 2, int(np.ceil(hi[1])) + 2
+        return (max(0, x0), min(self.nx, x1), max(0, y0), min(self.ny, y1), 0, 1)
+
+    def max_speed(self, period_samples: int = 512) -> float:
+        """Largest joint displacement per lattice step over one CPG period."""
+        omega = 2.0 * np.pi * self.frequency / 1000.0 * self.time_scale
+        period = 2.0 * np.pi / omega
+        t = np.linspace(0.0, period, period_samples)
+        return float(np.abs(self.joints(t + 1.0) - self.joints(t)).max())
+
+
+def eel_batch(n: int = 64, seed: int = 3003) -> List[Eel]:
+    """n eels with seeded grid sizes and CPG parameters: size from
+    {192x96, 256x128, 384x160, 512x192}, chain length 0.3 nx, radius 4,
+    A ~ U[0.2, 0.6] rad, f ~ U[0.5, 2] per 1000 steps, phi ~ U[0, pi/2].
+    Head at (0.25 nx, ny / 2)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    eels = []
+    for _ in range(n):
+        nx, ny = EEL_SIZES[int(rng.integers(0, len(EEL_SIZES)))]
+        A = float(rng.uniform(0.2, 0.6))
+        f = float(rng.uniform(0.5, 2.0))
+        phi = float(rng.uniform(0.0, np.pi / 2))
+        e = Eel(nx=nx, ny=ny, amplitude=A, frequency=f, phase_lag=phi, head=(0.25 * nx, 0.5 * ny),
+                seg_len=0.3 * nx / EEL_SEGMENTS)
+        v = e.max_speed()
+        if v > EEL_VMAX:
+            e.time_scale = EEL_VMAX / v
+        eels.append(e)
+    return eels
+
+
+def eel_state(e: Eel, index: int, seed: int = 3003) -> np.ndarray:
+    """FP32 initial moments [6, ny, nx]: rho = 1, u = (0.05, 0) + 1e-3 noise
+    (seeded per environment), S = 0, the reference's uniform-flow values."""
+    rng = np.random.Generator(np.random.PCG64([seed, index]))
+    m = np.zeros((6, e.ny, e.nx), np.float64)
+    m[0] = 1.0
+    m[1] = EEL_U
+    m[1:3] += 1e-3 * (2.0 * rng.random((2, e.ny, e.nx)) - 1.0)
+    return m.astype(np.float32)
+
+
+def build_eel_batch(n: int = 64, seed: int = 3003, steps: int = 2000):
+    """GPU Batch of n eel environments with kinematics tables for `steps` steps.
+
+    Faces: velocity inlet (0.05, 0) on the left, zero-gradient outlet, free-slip
+    top and bottom. Returns (batch, lattices, eels)."""
+    from .batch import Batch
+    from .lattice import Lattice
+
+    eels = eel_batch(n, seed)
+    lats = []
+    for i, e in enumerate(eels):
+        L = Lattice(9, e.nx, e.ny, nsolid=EEL_SEGMENTS)
+        L.nu = e.nu
+        L.bc_type = [0, 2, 1, 2, 1, 1]
+        L.bc_value = [[EEL_U, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+        L.set_state(eel_state(e, i, seed))
+        lats.append(L)
+    b = Batch(lats)
+    for i, e in enumerate(eels):
+        caps, poses = e.tables(0, steps)
+        b.set_kinematics(i, caps, poses, e.box(0, steps))
+    return b, lats, eels

@@ -68,3 +68,21 @@ def test_bad_bc_type_rejected():
     e.bc_type[2] = 7
     assert lib.hl_step(C.byref(e), 0, 1, 0, None, None) == -1
     assert "bc_type" in lib.hl_last_error().decode()
+
+
+def test_batch_rejects_unsupported_environments():
+    lib = N.load()
+    e = N.HLEnv()
+    e.lattice, e.dtype, e.nx, e.ny, e.nz = 27, 0, 8, 8, 8
+    e.stride = 512
+    e.moments[0] = 0x1000
+    e.moments[1] = 0x2000
+    e.nu = 0.1
+    h = C.c_void_p()
+    assert lib.hl_batch_create(C.byref(e), 1, C.byref(h)) == -2
+    assert "D2Q9 FP32" in lib.hl_last_error().decode()
+    assert not h.value
+    assert lib.hl_batch_create(C.byref(e), 0, C.byref(h)) == -1
+    assert lib.hl_batch_step(None, None, 1, None, 0, None) == -1
+    assert lib.hl_batch_replay(None, None) == -1
+    assert "NULL" in lib.hl_last_error().decode()
