@@ -39,6 +39,8 @@ def parse():
     p.add_argument("--kernel", default="auto", choices=["auto", "gather", "tiled"])
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="only run a plain device copy of this many MiB, 5 times (counter calibration)")
     return p.parse_args()
 
 
@@ -138,6 +140,17 @@ def main():
 
     from openhomelbm_amd import _native as N
     from openhomelbm_amd import inputs
+
+    if args.copy_mib > 0:
+        # reference kernel of known traffic: reads and writes copy_mib MiB per launch
+        n = args.copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_mib} MiB x 5")
+        return
 
     if args.workload == "d2q9_eels64":
         r = bench_batched(args.steps, args.warmup, max(min(args.steps, 200), 1))

@@ -1,0 +1,133 @@
+"""Autotuner for the tiled D3Q27 kernel's launch shape.
+
+The tiled kernel's tile (TX x TY columns staged in LDS) and z-chunk (ZC
+planes marched per workgroup) trade halo re-reads, LDS footprint (hence
+workgroups per CU) and the number of workgroups in flight; the best choice
+depends on the lattice size. The tuner times every supported shape on a
+lattice of the size class being tuned (the state is saved and restored) and
+records the fastest per size class in a JSON table. Lattice.step applies the
+table's entry for its size class before it launches.
+
+Results do not depend on the tile shape (the kernel's arithmetic per cell is
+the same), so tuning changes speed only.
+"""
+from __future__ import annotations
+
+import json
+import os
+from typing import Dict, Iterable, Optional, Tuple
+
+from . import _native as N
+
+#: tile shapes the tiled kernel is compiled for
+TILES = ((64, 8), (64, 4), (32, 8), (32, 16), (16, 16))
+ZCHUNKS = (8, 16, 32, 64)
+#: shipped table (measured on MI355X); a user table path may be set with
+#: OPENHOMELBM_TUNING
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+_table: Optional[Dict[str, Tuple[int, int, int]]] = None
+_active: Dict[int, Tuple[int, int, int]] = {}
+
+
+def size_class(lattice: int, nx: int, ny: int, nz: int = 1) -> str:
+    """Size class key: lattice and cell count bucket (powers of 4 cells)."""
+    n = nx * ny * nz
+    b = 0
+    while (1 << (2 * (b + 1))) <= n:
+        b += 1
+    return f"d{2 if lattice == 9 else 3}q{lattice}:4^{b}"
+
+
+def table_path() -> str:
+    return os.environ.get("OPENHOMELBM_TUNING", DEFAULT_TABLE)
+
+
+def load_table(path: Optional[str] = None) -> Dict[str, Tuple[int, int, int]]:
+    global _table
+    p = path or table_path()
+    t: Dict[str, Tuple[int, int, int]] = {}
+    if os.path.exists(p):
+        with open(p) as f:
+            raw = json.load(f)
+        for k, v in raw.get("tiles", {}).items():
+            t[k] = tuple(int(x) for x in v["tile"])
+    _table = t
+    return t
+
+
+def save_table(entries: Dict[str, dict], path: Optional[str] = None) -> str:
+    p = path or table_path()
+    raw = {"tiles": {}}
+    if os.path.exists(p):
+        with open(p) as f:
+            raw = json.load(f)
+    raw.setdefault("tiles", {}).update(entries)
+    with open(p, "w") as f:
+        json.dump(raw, f, indent=1, sort_keys=True)
+    load_table(p)
+    return p
+
+
+def tuned_tile(lattice: int, nx: int, ny: int, nz: int) -> Optional[Tuple[int, int, int]]:
+    if _table is None:
+        load_table()
+    return _table.get(size_class(lattice, nx, ny, nz))
+
+
+def apply(lat) -> None:
+    """Set the tiled kernel's shape for this lattice's size class (if tuned)."""
+    if lat.lattice != 27:
+        return
+    t = tuned_tile(lat.lattice, lat.nx, lat.ny, lat.nz)
+    if t is None or _active.get(27) == t:
+        return
+    N.set_tiling(27, *t)
+    _active[27] = t
+
+
+def candidates() -> Iterable[Tuple[int, int, int]]:
+    for tx, ty in TILES:
+        for zc in ZCHUNKS:
+            yield (tx, ty, zc)
+
+
+def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[int, int, int]]] = None,
+         save: bool = True, path: Optional[str] = None, log=None) -> dict:
+    """Time the tiled kernel on `lat` for every shape; keep the fastest.
+
+    The lattice's state is restored afterwards. Returns the size-class entry
+    {"tile": [tx, ty, zc], "us_per_step": t, "timings": {...}}."""
+    import torch
+
+    if lat.lattice != 27:
+        raise ValueError("only the D3Q27 kernel has tunable tile shapes")
+    saved = lat.buf.clone()
+    cur = lat.cur
+    timings = {}
+    try:
+        for t in (shapes or candidates()):
+            N.set_tiling(27, *t)
+            _active[27] = t
+            lat.step(warmup, N.HL_KERNEL_TILED, _tuned=False)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            e0.record()
+            lat.step(steps, N.HL_KERNEL_TILED, _tuned=False)
+            e1.record()
+            torch.cuda.synchronize()
+            timings["x".join(map(str, t))] = e0.elapsed_time(e1) * 1e3 / steps
+            if log:
+                log(f"[autotune] tile {t}: {timings['x'.join(map(str, t))]:.1f} us/step")
+    finally:
+        lat.buf.copy_(saved)
+        lat.cur = cur
+    best = min(timings, key=timings.get)
+    tile = [int(x) for x in best.split("x")]
+    N.set_tiling(27, *tile)
+    _active[27] = tuple(tile)
+    entry = {"tile": tile, "us_per_step": round(timings[best], 2),
+             "timings": {k: round(v, 2) for k, v in timings.items()}}
+    if save:
+        save_table({size_class(lat.lattice, lat.nx, lat.ny, lat.nz): entry}, path)
+    return entry

@@ -193,8 +193,17 @@ k_tiled_d3q27(EnvArgs a, int zc)
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const int64_t plane = (int64_t)nx * ny;
-    const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY;
-    const int zb = 1 + blockIdx.z * zc;
+    // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin in
+    // dispatch order, so dispatch slot L runs on XCD L % 8. Give XCD k the
+    // contiguous run of tiles [k * per, (k + 1) * per) (x fastest, then y,
+    // then z) so that tiles sharing halo cache lines share one L2.
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int ntile = gx * gy * gridDim.z;
+    const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int per = ntile / 8;
+    const int T = L < per * 8 ? (L % 8) * per + L / 8 : L;
+    const int x0 = (T % gx) * TX, y0 = ((T / gx) % gy) * TY;
+    const int zb = 1 + (T / (gx * gy)) * zc;
     const int ze = min(zb + zc, nz - 1);  // destination planes [zb, ze)
 
     // source column of this thread (tile + halo)

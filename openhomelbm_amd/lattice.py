@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _native as N
+from . import autotune
 
 _TORCH_DTYPE = {N.HL_F32: torch.float32, N.HL_F64: torch.float64, N.HL_F16: torch.float16,
                 N.HL_BF16: torch.bfloat16}
@@ -192,9 +193,12 @@ class Lattice:
 
     # -------------------------------------------------------------- step
     def step(self, nsteps: int = 1, kernel: int = N.HL_KERNEL_AUTO, force_history: Optional[torch.Tensor] = None,
-             stream=None) -> None:
+             stream=None, _tuned: bool = True) -> None:
         """Advance nsteps. force_history (device FP64 [nsteps, nsolid, 6]) receives
-        the force/torque of the state each step reads."""
+        the force/torque of the state each step reads. The tiled kernel uses the
+        autotuned shape of this lattice's size class when one is recorded."""
+        if _tuned and self.lattice == 27:
+            autotune.apply(self)
         e = self.env()
         fh = None
         if force_history is not None:
