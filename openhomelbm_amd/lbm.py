@@ -1,0 +1,368 @@
+"""D2Q9 solver API of the reference (envs/lbm/lbm_core.py HomeFlow and
+envs/lbm/lbm_solver.py LBM_Solver) over the HIP library.
+
+Same class names, constructor signatures, methods and field names. Fields
+are LatticeField / HostArray handles with .numpy() / .assign() instead of
+Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
+...) works as with the reference. Differences, all deliberate:
+
+* Bodies are closed polygons (configure_solid, lbm_core.py:172-282). Each
+  step voxelizes them into an integer mask with the reference's even-odd
+  inside test, and the wall rule runs on mask links (mask(x) != mask(x-c_i))
+  instead of segment-link intersections (get_cutcell, lbm_func.py:545-607).
+  Capsule bodies (add_capsule) are an extension.
+* Forces and torques are reduced in FP64 in a fixed order and are
+  deterministic; solid_forcex / solid_forcey / torque hold them as FP32 like
+  the reference's atomics.
+* create_solid_from_mujoco needs MuJoCo, which is not available here; it
+  raises with that message.
+* step() is asynchronous like the reference's graph launch; reading a field
+  synchronizes.
+"""
+from __future__ import annotations
+
+import math
+from typing import Any, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _native as N
+from .arrays import ForceField, HostArray, LatticeField, as_numpy
+from .lattice import Lattice
+
+ML_FLUID, ML_SOLID, ML_WALL = 0, 1, 2
+
+_W = (4 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 9, 1 / 36, 1 / 36, 1 / 36, 1 / 36)
+_CX = (0, 1, -1, 0, 0, 1, -1, -1, 1)
+_CY = (0, 0, 0, 1, -1, 1, 1, -1, -1)
+_INV = (0, 2, 1, 4, 3, 7, 8, 5, 6)
+
+
+class HomeFlow:
+    """D2Q9 state of one world (envs/lbm/lbm_core.py:18-282).
+
+    Construct, then Initialize(nx, ny, n_objects) to allocate on the GPU."""
+
+    def __init__(self, dtype: str = "f32", device: Optional[torch.device] = None):
+        self._dtype = dtype
+        self._device = device
+        self._lat: Optional[Lattice] = None
+
+    # ------------------------------------------------------------ setup
+    def Initialize(self, nx: int, ny: int, n_objects: int = 1) -> None:
+        """Allocate the world (lbm_core.py:90-170): rho = 1, u = 0, S = 0,
+        bc_type (1, 1, 1, 1), vis_shear 0.01."""
+        if n_objects < 0 or n_objects > N.HL_MAX_SOLIDS:
+            raise ValueError(f"n_objects must be in [0, {N.HL_MAX_SOLIDS}]")
+        self.time_step = 1.0
+        self.nx, self.ny = int(nx), int(ny)
+        self.grid_length = 1.0 / nx
+        self.velocity_scale = self.time_step / self.grid_length
+        self._lat = Lattice(9, self.nx, self.ny, dtype=self._dtype, nsolid=n_objects, device=self._device)
+        L = self._lat
+        L.nu = 0.01
+        L.bc_type = [1, 1, 1, 1, 1, 1]
+        L.init_uniform((0.0, 0.0))
+        self.w_d2q9 = np.array(_W, np.float32)
+        self.cx_d2q9 = np.array(_CX, np.float32)
+        self.cy_d2q9 = np.array(_CY, np.float32)
+        self.indexd2q9Inv_gpu = np.array(_INV, np.int32)
+        self.small_size = 5.0
+        self.n_objects = n_objects
+        n = max(n_objects, 1)
+        self.max_segments_per_object = 10
+        self._solid_dirty = True
+        dirty = self._mark_dirty
+        self._solid = {
+            "solid_position": HostArray(np.zeros((n, 2)), on_change=dirty),
+            "solid_angle": HostArray(np.zeros(n), on_change=dirty),
+            "solid_scale": HostArray(np.zeros(n), on_change=dirty),
+            "solid_mass_center": HostArray(np.zeros((n, 2)), on_change=dirty),
+            "solid_max_radius": HostArray(np.zeros(n)),
+            "mass": HostArray(np.zeros(n)),
+            "MoI": HostArray(np.zeros(n)),
+            "solid_line": HostArray(np.zeros((n, self.max_segments_per_object, 2)), on_change=dirty),
+            "solid_line_num": HostArray(np.zeros(n, np.int32), dtype=np.int32, on_change=dirty),
+        }
+        self._capsules: dict = {}
+        self._last_pose: Optional[np.ndarray] = None
+        self._forces = torch.zeros((1, max(n_objects, 1), 6), dtype=torch.float64, device=L.device)
+        self._box = (0, 0, 0, 0, 0, 0)
+
+    def _mark_dirty(self):
+        self._solid_dirty = True
+
+    @property
+    def lattice(self) -> Lattice:
+        if self._lat is None:
+            raise RuntimeError("HomeFlow.Initialize(nx, ny, n_objects) has not been called")
+        return self._lat
+
+    # ------------------------------------------------------------ fields
+    rho = property(lambda s: LatticeField(s.lattice, (0,)))
+    u = property(lambda s: LatticeField(s.lattice, (1, 2)))
+    Sxx = property(lambda s: LatticeField(s.lattice, (3,)))
+    Syy = property(lambda s: LatticeField(s.lattice, (4,)))
+    Sxy = property(lambda s: LatticeField(s.lattice, (5,)))
+    rho_post = property(lambda s: LatticeField(s.lattice, (0,), "post"))
+    u_post = property(lambda s: LatticeField(s.lattice, (1, 2), "post"))
+    Sxx_post = property(lambda s: LatticeField(s.lattice, (3,), "post"))
+    Syy_post = property(lambda s: LatticeField(s.lattice, (4,), "post"))
+    Sxy_post = property(lambda s: LatticeField(s.lattice, (5,), "post"))
+    forcex = property(lambda s: ForceField(s.lattice, 0))
+    forcey = property(lambda s: ForceField(s.lattice, 1))
+
+    @property
+    def flag(self) -> HostArray:
+        """ML_WALL on every face cell, ML_FLUID inside (InitBoundary, lbm_func.py:17-23)."""
+        f = np.zeros((self.nx, self.ny), np.int32)
+        f[0, :] = f[-1, :] = f[:, 0] = f[:, -1] = ML_WALL
+        return HostArray(f, dtype=np.int32)
+
+    @property
+    def u_img(self) -> HostArray:
+        """Velocity magnitude of the present state (the render buffer's default content)."""
+        u = self.u.torch()
+        return HostArray(torch.sqrt((u * u).sum(-1)).cpu().numpy())
+
+    @property
+    def vis_shear(self) -> float:
+        return float(self.lattice.nu)
+
+    @vis_shear.setter
+    def vis_shear(self, v) -> None:
+        v = float(v)
+        if not v > 0.0 or not math.isfinite(v):
+            raise ValueError(f"vis_shear must be > 0, got {v}")
+        self.lattice.nu = v
+
+    @property
+    def bc_type(self):
+        return tuple(int(t) for t in self.lattice.bc_type[:4])
+
+    @bc_type.setter
+    def bc_type(self, v) -> None:
+        t = [int(x) for x in as_numpy(v).reshape(-1)]
+        if len(t) != 4:
+            raise ValueError(f"bc_type needs 4 entries (left, top, right, bottom), got {len(t)}")
+        if any(x not in (0, 1, 2) for x in t):
+            raise ValueError("bc_type entries must be 0 (velocity), 1 (zero gradient) or 2 (free slip)")
+        self.lattice.bc_type = t + [1, 1]
+
+    @property
+    def bc_value(self) -> HostArray:
+        return HostArray(np.array([v[:2] for v in self.lattice.bc_value[:4]]), on_change=None)
+
+    @bc_value.setter
+    def bc_value(self, v) -> None:
+        a = as_numpy(v, np.float64)
+        if a.shape != (4, 2):
+            raise ValueError(f"bc_value needs shape (4, 2), got {a.shape}")
+        self.lattice.bc_value = [[float(a[k, 0]), float(a[k, 1]), 0.0] for k in range(4)] + [[0.0] * 3] * 2
+
+    # solid tables: whole-array assignment replaces the table
+    def __getattr__(self, name):
+        solid = self.__dict__.get("_solid")
+        if solid is not None and name in solid:
+            return solid[name]
+        raise AttributeError(name)
+
+    def __setattr__(self, name, value):
+        solid = self.__dict__.get("_solid")
+        if solid is not None and name in solid:
+            old = solid[name]
+            solid[name] = HostArray(value, dtype=old.dtype, on_change=old._on_change)
+            self._solid_dirty = True
+            return
+        object.__setattr__(self, name, value)
+
+    @property
+    def solid_forcex(self) -> HostArray:
+        return HostArray(self._forces[0, : self.n_objects, 0].cpu().numpy())
+
+    @property
+    def solid_forcey(self) -> HostArray:
+        return HostArray(self._forces[0, : self.n_objects, 1].cpu().numpy())
+
+    @property
+    def torque(self) -> HostArray:
+        return HostArray(self._forces[0, : self.n_objects, 5].cpu().numpy())
+
+    @property
+    def solid_line_transformed(self) -> HostArray:
+        return HostArray(self._transformed()[0])
+
+    # ------------------------------------------------------------ solids
+    def configure_solid(self, solid_id: int, lines: Any, position: Any, angle: float = 0.0, scale: float = 0.3,
+                        mass: float = 100000.0, mass_center: Optional[Any] = None) -> None:
+        """Closed polygon body (lbm_core.py:172-282): vertices in body units,
+        placed at `position` (lattice units) with angle and scale relative to nx."""
+        if solid_id < 0 or solid_id >= self.n_objects:
+            raise ValueError(f"solid_id {solid_id} out of range [0, {self.n_objects})")
+        pts = as_numpy(lines, np.float32).reshape(-1, 2)
+        n = len(pts)
+        if n < 3:
+            raise ValueError("a polygon needs at least 3 vertices")
+        S = self._solid
+        if n > self.max_segments_per_object:
+            grown = np.zeros((S["solid_line"].shape[0], n, 2), np.float32)
+            grown[:, : self.max_segments_per_object] = S["solid_line"].numpy()
+            S["solid_line"] = HostArray(grown, on_change=self._mark_dirty)
+            self.max_segments_per_object = n
+        mc = np.zeros(2, np.float32) if mass_center is None else as_numpy(mass_center, np.float32).reshape(2)
+        line = S["solid_line"].numpy()
+        line[solid_id] = 0.0
+        line[solid_id, :n] = pts
+        S["solid_line"].assign(line)
+        S["solid_line_num"][solid_id] = n
+        S["solid_position"][solid_id] = as_numpy(position, np.float32).reshape(2)
+        S["solid_mass_center"][solid_id] = mc
+        S["solid_angle"][solid_id] = float(angle)
+        S["solid_scale"][solid_id] = float(scale)
+        S["mass"][solid_id] = float(mass)
+        S["MoI"][solid_id] = float(mass) * (float(scale) * float(self.nx)) ** 2.0 / 8.0
+        dist = np.sqrt(((pts.astype(np.float64) - mc) ** 2).sum(1)).max()
+        S["solid_max_radius"][solid_id] = (dist + math.sqrt(2.0)) ** 2
+        self._capsules.pop(solid_id, None)
+        self._solid_dirty = True
+
+    def add_capsule(self, solid_id: int, a: Sequence[float], b: Sequence[float], radius: float,
+                    position: Optional[Sequence[float]] = None, angle: float = 0.0) -> None:
+        """Capsule body (extension): segment a-b (body frame, lattice units)
+        with the given radius, placed at position with angle."""
+        if solid_id < 0 or solid_id >= self.n_objects:
+            raise ValueError(f"solid_id {solid_id} out of range [0, {self.n_objects})")
+        if not radius > 0:
+            raise ValueError("capsule radius must be > 0")
+        self._capsules[solid_id] = (np.asarray(a, np.float64), np.asarray(b, np.float64), float(radius))
+        S = self._solid
+        S["solid_line_num"][solid_id] = 0
+        if position is not None:
+            S["solid_position"][solid_id] = as_numpy(position, np.float32).reshape(2)
+        S["solid_angle"][solid_id] = float(angle)
+        self._solid_dirty = True
+
+    def _transformed(self):
+        """Polygon vertices in lattice coordinates, FP32, in the operation order
+        of precompute_transformed_segments (lbm_func.py:509-542)."""
+        S = self._solid
+        n = S["solid_line"].shape[0]
+        line, num = S["solid_line"].numpy(), S["solid_line_num"].numpy()
+        out = np.zeros_like(line, dtype=np.float32)
+        f32 = np.float32
+        for k in range(n):
+            if num[k] == 0:
+                continue
+            scale = f32(S["solid_scale"][k]) * f32(self.nx)
+            ca, sa = f32(np.cos(f32(S["solid_angle"][k]))), f32(np.sin(f32(S["solid_angle"][k])))
+            pp = (line[k, : num[k]] - S["solid_mass_center"][k].astype(f32)) * scale
+            pos = S["solid_position"][k].astype(f32)
+            out[k, : num[k], 0] = pp[:, 0] * ca - pp[:, 1] * sa + pos[0]
+            out[k, : num[k], 1] = pp[:, 0] * sa + pp[:, 1] * ca + pos[1]
+        return out, num
+
+    def _poses(self) -> np.ndarray:
+        S = self._solid
+        n = max(self.n_objects, 1)
+        p = np.zeros((n, 4), np.float32)
+        p[:, 0:2] = S["solid_position"].numpy()[:n]
+        ang = S["solid_angle"].numpy()[:n].astype(np.float64)
+        p[:, 2], p[:, 3] = np.cos(ang), np.sin(ang)
+        return p
+
+    def _prepare(self, stream=None) -> None:
+        """Rebuild mask and pose table from the solid tables when they changed;
+        the previous step's pose becomes the 'last' pose (u_p = frame difference)."""
+        L = self.lattice
+        if self.n_objects == 0:
+            return
+        now = self._poses()
+        last = now if self._last_pose is None else self._last_pose
+        L.set_poses(np.concatenate([now, last], axis=1)[: self.n_objects])
+        self._last_pose = now
+        if not self._solid_dirty:
+            return
+        verts, num = self._transformed()
+        polys = [k for k in range(self.n_objects) if num[k] >= 3]
+        caps = []
+        for k, (a, b, r) in self._capsules.items():
+            c, s = now[k, 2], now[k, 3]
+            R = np.array([[c, -s], [s, c]], np.float64)
+            pa = R @ a[:2] + now[k, :2]
+            pb = R @ b[:2] + now[k, :2]
+            caps.append([pa[0], pa[1], 0.0, pb[0], pb[1], 0.0, r, k])
+        if polys and caps:
+            raise ValueError("a world's bodies are either polygons (configure_solid) or capsules (add_capsule)")
+        if polys:
+            vv = np.zeros((self.n_objects, max(int(num.max()), 3), 2), np.float32)
+            cnt = np.zeros(self.n_objects, np.int32)
+            for k in polys:
+                vv[k, : num[k]] = verts[k, : num[k]]
+                cnt[k] = num[k]
+            L.voxelize_polygons(vv, cnt, stream=stream)
+        elif caps:
+            L.voxelize(np.array(caps, np.float32), stream=stream)
+        self._solid_dirty = False
+
+    def step(self, stream=None) -> None:
+        """One time step with the per-solid force of the state it reads."""
+        self._prepare(stream)
+        L = self.lattice
+        L.step(1, N.HL_KERNEL_AUTO, force_history=self._forces if self.n_objects > 0 else None, stream=stream)
+
+
+class LBM_Solver:
+    """Worlds of one size stepped together (envs/lbm/lbm_solver.py:21-141)."""
+
+    def __init__(self, nx: int, ny: int, solid_num: int = 1, nworld: int = 1, device: Optional[Any] = None,
+                 dtype: str = "f32") -> None:
+        if nworld < 1:
+            raise ValueError("nworld must be >= 1")
+        self.nx, self.ny = nx, ny
+        self.solid_num = solid_num
+        self.nworld = nworld
+        self.captured = False
+        self.captured_graph = None
+        self.total_steps = 500000
+        self.per_frame_t = 500
+        self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+        self.flows = [HomeFlow(dtype=dtype, device=self.device) for _ in range(nworld)]
+        for flow in self.flows:
+            flow.Initialize(nx, ny, n_objects=solid_num)
+        self.flows_wp = self.flows
+        self.mujoco_mappings = {}
+        self.mujoco_origins_wp = None
+        self.lbm_origins_wp = None
+        self.scales_wp = None
+        self.solid_ids_wp = None
+        self.solid_id_to_index = {}
+
+    def step(self) -> None:
+        """Advance every world one step (asynchronous). The reference captures a
+        graph on the first call; here every call launches the fused kernels, so
+        resetting `captured` is harmless."""
+        for flow in self.flows:
+            flow.step()
+        self.captured = True
+
+    def create_solid_from_mujoco(self, solid_id, model, data, body_or_geom_name, lbm_position=None, lbm_scale=0.3,
+                                 n_samples=20, is_body=True, **kw):
+        raise NotImplementedError("create_solid_from_mujoco needs MuJoCo, which is not available on this platform; "
+                                  "build the outline with HomeFlow.configure_solid (polygon) or add_capsule")
+
+    def finalize_mappings(self, solid_ids: Sequence[int]) -> None:
+        """Coordinate-mapping tables for the coupling (lbm_solver.py:211-240)."""
+        origins, lbm, scales = [], [], []
+        for idx, sid in enumerate(solid_ids):
+            if sid not in self.mujoco_mappings:
+                raise ValueError(f"Solid {sid} not initialized")
+            m = self.mujoco_mappings[sid]
+            origins.append(m["mujoco_origin"])
+            lbm.append(m["lbm_origin"])
+            scales.append(m["scale"])
+            self.solid_id_to_index[sid] = idx
+        self.mujoco_origins_wp = np.array(origins, np.float32)
+        self.lbm_origins_wp = np.array(lbm, np.float32)
+        self.scales_wp = np.array(scales, np.float32)
+        self.solid_ids_wp = np.array(solid_ids, np.int32)

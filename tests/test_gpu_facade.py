@@ -1,0 +1,174 @@
+"""The reference-compatible solver API (LBM_Solver / HomeFlow, LBM_Solver3D /
+HomeFlow3D) on the GPU: field index order and ping-pong semantics, body
+set-up, and results against the oracle stepped with the same mask and poses."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from openhomelbm_amd.lbm import HomeFlow, LBM_Solver
+from openhomelbm_amd.lbm3d import LBM_Solver3D, quat_to_matrix
+from tests.helpers import random_state, tensor_rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _square(h):
+    return np.array([[-h, -h], [h, -h], [h, h], [-h, h]], np.float32)
+
+
+def _setup_2d(nx=48, ny=40, nworld=2):
+    s = LBM_Solver(nx, ny, solid_num=1, nworld=nworld)
+    for f in s.flows:
+        f.vis_shear = 0.02
+        f.bc_type = [0, 1, 1, 1]
+        f.bc_value = [[0.05, 0.0], [0, 0], [0, 0], [0, 0]]
+    m0 = random_state(9, (ny, nx), seed=5).astype(np.float32)
+    f = s.flows[0]
+    f.rho.assign(m0[0].T)
+    f.u.assign(np.stack([m0[1].T, m0[2].T], -1))
+    f.Sxx.assign(m0[3].T)
+    f.Syy.assign(m0[4].T)
+    f.Sxy.assign(m0[5].T)
+    # the *_post buffer too, so both states start equal
+    f.lattice.buf[f.lattice.cur ^ 1].copy_(f.lattice.buf[f.lattice.cur])
+    f.configure_solid(0, _square(0.1), position=(18.0, 19.5), angle=0.3, scale=0.25)
+    return s, m0
+
+
+def _oracle_2d(f, m, poses, steps):
+    env = orc.Env(9, f.nx, f.ny, nu=0.02, bc_type=(0, 1, 1, 1), bc_value=((0.05, 0, 0),) + ((0, 0, 0),) * 5)
+    verts, num = f._transformed()
+    mask = orc.voxelize_polygons(env, verts[:1, : num[0]], [num[0]])
+    env.mask = mask
+    hist = []
+    for k in range(steps):
+        env.solids = poses[k]
+        hist.append(orc.compute_forces(env, m))
+        m = orc.step(env, m)
+    return m, np.array(hist), mask
+
+
+def test_homeflow_field_order_and_post():
+    s, m0 = _setup_2d()
+    f = s.flows[0]
+    assert f.rho.numpy().shape == (48, 40) and f.u.numpy().shape == (48, 40, 2)
+    np.testing.assert_array_equal(f.rho.numpy(), m0[0].T)
+    before = f.u.numpy()
+    s.step()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(f.u_post.numpy(), before)  # *_post = previous state after the swap
+    assert f.flag.numpy()[0, 5] == 2 and f.flag.numpy()[5, 5] == 0
+
+
+def test_lbm_solver_static_polygon_matches_oracle():
+    s, m0 = _setup_2d()
+    f = s.flows[0]
+    pose = np.array([[18.0, 19.5, np.cos(0.3), np.sin(0.3), 18.0, 19.5, np.cos(0.3), np.sin(0.3)]], np.float32)
+    forces = []
+    for _ in range(3):
+        s.step()
+        forces.append([f.solid_forcex.numpy()[0], f.solid_forcey.numpy()[0], f.torque.numpy()[0]])
+    ref, hist, mask = _oracle_2d(f, m0, [pose] * 3, 3)
+    got = f.lattice.get_state(np.float32)
+    assert np.array_equal(f.lattice.mask.cpu().numpy().reshape(40, 48), mask)
+    assert mask.any()
+    assert np.array_equal(got, ref)
+    np.testing.assert_allclose(np.array(forces), hist[:, 0, [0, 1, 5]], rtol=1e-6, atol=1e-9)
+
+
+def test_lbm_solver_moving_polygon_uses_frame_difference():
+    s, m0 = _setup_2d()
+    f = s.flows[0]
+    v0, n0 = f._transformed()
+    s.step()
+    f.solid_position = [[18.3, 19.4]]  # whole-array assignment, as the reference's callers do
+    v1, n1 = f._transformed()
+    s.step()
+    torch.cuda.synchronize()
+    c, sn = np.float32(np.cos(0.3)), np.float32(np.sin(0.3))
+    p0 = np.array([[18.0, 19.5, c, sn, 18.0, 19.5, c, sn]], np.float32)
+    p1 = np.array([[18.3, 19.4, c, sn, 18.0, 19.5, c, sn]], np.float32)
+    env = orc.Env(9, 48, 40, nu=0.02, bc_type=(0, 1, 1, 1), bc_value=((0.05, 0, 0),) + ((0, 0, 0),) * 5)
+    m = m0
+    for pose, (v, n) in ((p0, (v0, n0)), (p1, (v1, n1))):
+        env.mask = orc.voxelize_polygons(env, v[:1, : n[0]], [n[0]])
+        env.solids = pose
+        m = orc.step(env, m)
+    assert not np.array_equal(v0, v1)
+    assert np.array_equal(f.lattice.get_state(np.float32), m)
+
+
+def test_2d_api_errors():
+    s = LBM_Solver(16, 12, solid_num=1)
+    f = s.flows[0]
+    with pytest.raises(ValueError):
+        f.configure_solid(3, _square(0.1), (5, 5))
+    with pytest.raises(ValueError):
+        f.bc_value = np.zeros((3, 2))
+    with pytest.raises(ValueError):
+        f.vis_shear = 0.0
+    with pytest.raises(ValueError):
+        f.bc_type = [0, 1, 1]
+    with pytest.raises(NotImplementedError):
+        s.create_solid_from_mujoco(0, None, None, "body")
+    with pytest.raises(ValueError):
+        s.finalize_mappings([0])
+
+
+def _setup_3d():
+    s = LBM_Solver3D(28, 22, 24, solid_num=2)
+    f = s.flows[0]
+    f.vis_shear = 0.03
+    f.bc_type = [0, 1, 1, 1, 1, 1]
+    f.bc_value = [[0.05, 0, 0]] + [[0, 0, 0]] * 5
+    s.create_solid_capsule(0, (-0.1, 0, 0), (0.1, 0, 0), 0.1, lbm_position=(10, 11, 12), lbm_scale=1.0,
+                           init_quaternion=(np.cos(0.2), 0, 0, np.sin(0.2)))
+    s.create_solid_sphere(1, 0.1, lbm_position=(19, 10.5, 12.5), lbm_scale=1.0)
+    m0 = random_state(27, (24, 22, 28), seed=8).astype(np.float32)
+    f.lattice.set_state(m0)
+    return s, m0
+
+
+def test_lbm_solver3d_matches_oracle():
+    s, m0 = _setup_3d()
+    f = s.flows[0]
+    s.step()
+    torch.cuda.synchronize()
+    env = orc.Env(27, 28, 22, 24, nu=0.03, bc_type=(0, 1, 1, 1, 1, 1), bc_value=((0.05, 0, 0),) + ((0, 0, 0),) * 5)
+    env.mask = orc.voxelize(env, f.capsules)
+    assert np.array_equal(f.lattice.mask.cpu().numpy().reshape(24, 22, 28), env.mask)
+    assert set(np.unique(env.mask)) == {0, 1, 2}
+    P = np.zeros((2, 24), np.float32)
+    for k, (pos, q) in enumerate((((10, 11, 12), (np.cos(0.2), 0, 0, np.sin(0.2))), ((19, 10.5, 12.5), (1, 0, 0, 0)))):
+        P[k, 0:3] = pos
+        P[k, 3:12] = quat_to_matrix(q).reshape(9)
+        P[k, 12:24] = P[k, 0:12]
+    env.solids = P
+    F = orc.compute_forces(env, m0)
+    ref = orc.step(env, m0)
+    err = tensor_rel_l2(27, f.lattice.get_state(np.float64), ref)
+    assert err["all"] <= 1e-6, err
+    got_f, got_t = s.get_forces_and_torques(0)
+    scale = 1.0 * 28
+    np.testing.assert_allclose(got_f, F[:, 0:3] / scale, rtol=1e-5, atol=1e-9)
+    np.testing.assert_allclose(got_t, F[:, 3:6] / scale ** 2, rtol=1e-5, atol=1e-9)
+
+
+def test_lbm_solver3d_pose_update_and_reset():
+    s, m0 = _setup_3d()
+    f = s.flows[0]
+    s.step()
+    s.update_solids_batch(0, [1], np.array([[0.1, 0.0, 0.0]]), np.array([[1, 0, 0, 0]]))
+    pos = f.solid_position.numpy()[1]
+    np.testing.assert_allclose(pos, (19 + 0.1 * 28, 10.5, 12.5), rtol=1e-6)
+    s.step()
+    torch.cuda.synchronize()
+    assert np.isfinite(f.rho.numpy()).all()
+    s.reset_world(0)
+    torch.cuda.synchronize()
+    assert np.all(f.rho.numpy() == 1.0) and np.all(f.u.numpy() == 0.0)
+    s.finalize_mappings([0, 1])
+    assert s.scales_wp.shape == (2,)
+    with pytest.raises(NotImplementedError):
+        s.create_solid_from_mesh(0, None)
