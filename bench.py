@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--workload", default="d3q27_sphere", choices=["d3q27_sphere", "d2q9_karman", "d2q9_eels64"])
     p.add_argument("--batched-steps", type=int, default=400,
                    help="steps of the 64-eel batched run reported beside the D3Q27 line (0 = skip)")
+    p.add_argument("--precision-steps", type=int, default=5000,
+                   help="steps of the FP64/FP32/FP16/BF16 eel sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
     p.add_argument("--kernel", default="auto", choices=["auto", "gather", "tiled"])
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
@@ -215,6 +217,15 @@ def main():
         batched = bench_batched(args.batched_steps, 20, min(args.batched_steps, 100))
         log(f"[bench] batched {batched}")
 
+    prec = None
+    if args.precision_steps > 0 and args.workload == "d3q27_sphere":
+        from openhomelbm_amd import precision
+
+        t0p = time.perf_counter()
+        prec = precision.sweep(steps=args.precision_steps, log=log)
+        prec["seconds"] = round(time.perf_counter() - t0p, 1)
+        log(f"[bench] precision sweep {prec}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_seconds)
@@ -236,7 +247,7 @@ def main():
         "config": {"workload": w.name, "lattice": "D3Q27" if w.lattice == 27 else "D2Q9",
                    "grid": [w.nx, w.ny, w.nz], "nu": w.nu, "u_in": w.u_in[0], "kernel": args.kernel,
                    "step": "stream/collide/boundary kernel + FP64 per-solid force reduction",
-                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched},
+                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "precision": prec},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_us": round(kernel_ms * 1e3, 2)},

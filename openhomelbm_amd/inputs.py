@@ -267,3 +267,98 @@ def build_eel_batch(n: int = 64, seed: int = 3003, steps: int = 2000):
         caps, poses = e.tables(0, steps)
         b.set_kinematics(i, caps, poses, e.box(0, steps))
     return b, lats, eels
+
+
+@dataclass
+class Eel3D:
+    """Articulated capsule chain swimming in the x-y plane of a D3Q27 lattice
+    (the precision-sweep body): nseg capsules of equal length, joint angles
+    theta_j = A env_j sin(omega t - j phi) as Eel, time slowed so that no body
+    point moves faster than EEL_VMAX per step."""
+
+    nx: int
+    ny: int
+    nz: int
+    amplitude: float
+    frequency: float
+    phase_lag: float
+    length: float = 160.0
+    radius: float = 8.0
+    nseg: int = 10
+    time_scale: float = 1.0
+
+    @property
+    def head(self):
+        return (0.25 * self.nx, 0.5 * self.ny, 0.5 * self.nz)
+
+    @property
+    def seg_len(self) -> float:
+        return self.length / self.nseg
+
+    def nu(self, re: float = 500.0) -> float:
+        return EEL_U * self.length / re
+
+    def joints(self, t) -> np.ndarray:
+        t = np.asarray(t, dtype=np.float64)[..., None]
+        j = np.arange(self.nseg, dtype=np.float64)
+        env = 0.05 + 0.95 * j / (self.nseg - 1)
+        omega = 2.0 * np.pi * self.frequency / 1000.0 * self.time_scale
+        theta = self.amplitude * env * np.sin(omega * t - j * self.phase_lag)
+        theta[..., 0] = 0.0
+        heading = np.cumsum(theta, axis=-1)
+        seg = self.seg_len * np.stack([np.cos(heading), np.sin(heading)], axis=-1)
+        p = np.zeros(t.shape[:-1] + (self.nseg + 1, 2))
+        p[..., 0, :] = self.head[:2]
+        p[..., 1:, :] = np.asarray(self.head[:2]) + np.cumsum(seg, axis=-2)
+        return p
+
+    def max_speed(self, samples: int = 512) -> float:
+        omega = 2.0 * np.pi * self.frequency / 1000.0 * self.time_scale
+        t = np.linspace(0.0, 2.0 * np.pi / omega, samples)
+        return float(np.abs(self.joints(t + 1.0) - self.joints(t)).max())
+
+    def tables(self, t0: int, count: int):
+        """FP32 capsule [count, nseg, 8] and 3D pose [count, nseg, 24] tables:
+        capsule j from joint j to joint j+1 at z = nz/2, solid id j; pose j =
+        (joint j, rotation about z by the segment heading) at t and t - 1."""
+        t = np.arange(t0, t0 + count, dtype=np.float64)
+        now, last = self.joints(t), self.joints(t - 1.0)
+        zc = self.head[2]
+        caps = np.zeros((count, self.nseg, 8))
+        caps[..., 0:2] = now[:, :-1]
+        caps[..., 2] = zc
+        caps[..., 3:5] = now[:, 1:]
+        caps[..., 5] = zc
+        caps[..., 6] = self.radius
+        caps[..., 7] = np.arange(self.nseg)
+        poses = np.zeros((count, self.nseg, 24))
+        for k, p in ((0, now), (12, last)):
+            d = p[:, 1:] - p[:, :-1]
+            h = np.arctan2(d[..., 1], d[..., 0])
+            c, s = np.cos(h), np.sin(h)
+            poses[..., k + 0:k + 2] = p[:, :-1]
+            poses[..., k + 2] = zc
+            R = poses[..., k + 3:k + 12]
+            R[..., 0], R[..., 1], R[..., 3], R[..., 4], R[..., 8] = c, -s, s, c, 1.0
+        return caps.astype(np.float32), poses.astype(np.float32)
+
+    def box(self, t0: int, count: int):
+        p = self.joints(np.arange(t0, t0 + count, dtype=np.float64)).reshape(-1, 2)
+        lo, hi = p.min(0) - self.radius, p.max(0) + self.radius
+        zc = self.head[2]
+        return (max(0, int(np.floor(lo[0])) - 1), min(self.nx, int(np.ceil(hi[0])) + 2),
+                max(0, int(np.floor(lo[1])) - 1), min(self.ny, int(np.ceil(hi[1])) + 2),
+                max(0, int(np.floor(zc - self.radius)) - 1), min(self.nz, int(np.ceil(zc + self.radius)) + 2))
+
+
+def eel3d(seed: int = 4004, nx: int = 384, ny: int = 192, nz: int = 192, length: float = 160.0,
+          radius: float = 8.0) -> Eel3D:
+    """The precision-sweep eel: 10 capsules, length 160, radius 8 on 384x192x192,
+    A ~ U[0.2, 0.6], f ~ U[0.5, 2] per 1000 steps, phi ~ U[0, pi/2] (seeded)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    e = Eel3D(nx=nx, ny=ny, nz=nz, amplitude=float(rng.uniform(0.2, 0.6)), frequency=float(rng.uniform(0.5, 2.0)),
+              phase_lag=float(rng.uniform(0.0, np.pi / 2)), length=length, radius=radius)
+    v = e.max_speed()
+    if v > EEL_VMAX:
+        e.time_scale = EEL_VMAX / v
+    return e

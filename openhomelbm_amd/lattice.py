@@ -253,6 +253,21 @@ class Lattice:
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
 
+    def voxelize_device(self, caps: torch.Tensor, box, stream=None) -> None:
+        """Rebuild the mask from a device capsule table [n, 8] (FP32) inside a
+        box that holds every capsule (no host transfer; for per-step motion)."""
+        if caps.dtype != torch.float32 or caps.device != self.device or not caps.is_contiguous() or caps.shape[-1] != 8:
+            raise ValueError("caps must be a contiguous FP32 device tensor [n, 8]")
+        if self.mask is None:
+            self.mask = torch.zeros(self.ncell, dtype=torch.uint8, device=self.device)
+        write = union_box(self._mask_dirty, box)
+        b = (C.c_int32 * 6)(*write)
+        e = self.env()
+        N.check(self.lib.hl_voxelize(C.byref(e), self.mask.data_ptr(), caps.data_ptr(), caps.numel() // 8, b,
+                                     _stream_ptr(stream)), "hl_voxelize")
+        self.mask_box = tuple(box)
+        self._mask_dirty = tuple(box)
+
     def voxelize_polygons(self, verts: np.ndarray, counts: Sequence[int], box=None, stream=None) -> None:
         """Rebuild the mask from closed 2D polygons (even-odd rule) on the GPU.
 
