@@ -35,6 +35,8 @@ def parse():
     p.add_argument("--workload", default="d3q27_sphere", choices=["d3q27_sphere", "d2q9_karman", "d2q9_eels64"])
     p.add_argument("--batched-steps", type=int, default=400,
                    help="steps of the 64-eel batched run reported beside the D3Q27 line (0 = skip)")
+    p.add_argument("--serving-seconds", type=float, default=1.0,
+                   help="seconds per arrival rate of the serving sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--precision-steps", type=int, default=5000,
                    help="steps of the FP64/FP32/FP16/BF16 eel sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
@@ -217,6 +219,12 @@ def main():
         batched = bench_batched(args.batched_steps, 20, min(args.batched_steps, 100))
         log(f"[bench] batched {batched}")
 
+    serve = None
+    if args.serving_seconds > 0 and args.workload == "d3q27_sphere":
+        from openhomelbm_amd import serving
+
+        serve = serving.sweep(duration=args.serving_seconds, log=log)
+
     prec = None
     if args.precision_steps > 0 and args.workload == "d3q27_sphere":
         from openhomelbm_amd import precision
@@ -247,7 +255,8 @@ def main():
         "config": {"workload": w.name, "lattice": "D3Q27" if w.lattice == 27 else "D2Q9",
                    "grid": [w.nx, w.ny, w.nz], "nu": w.nu, "u_in": w.u_in[0], "kernel": args.kernel,
                    "step": "stream/collide/boundary kernel + FP64 per-solid force reduction",
-                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "precision": prec},
+                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve,
+                   "precision": prec},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel_us": round(kernel_ms * 1e3, 2)},

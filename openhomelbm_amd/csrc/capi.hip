@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -94,63 +95,65 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
     return a;
 }
 
-// Grow-only reduction workspace, one per device, owned by the library until
-// the process ends. Sized on first use; call a reduction once outside any
-// stream capture so the allocation does not happen while capturing.
-struct Workspace {
-    void* ptr = nullptr;
+// Library-owned scratch per (device, caller stream): a grow-only reduction
+// workspace (cut-cell list, force partials) and, for the near-body pass that
+// runs concurrently with the tiled kernel, a side stream (default priority)
+// with fork/join events. Calls on different caller streams never share
+// scratch, so their work may overlap; calls on one stream are ordered by that
+// stream (concurrent calls from several host threads on one stream are not
+// supported). Created on first use, kept until the process ends. Call a step or
+// reduction once outside any stream capture so the allocation does not happen
+// while capturing.
+struct StreamCtx {
+    std::mutex mutex;  // held for the whole call that uses this context
+    void* ws = nullptr;
     size_t bytes = 0;
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
-std::mutex g_ws_mutex;
-std::vector<Workspace> g_ws;
+std::mutex g_ctx_mutex;
+std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
 
-int32_t workspace(size_t bytes, void** out)
+int32_t stream_ctx(hipStream_t s, StreamCtx** out)
 {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    if ((int)g_ws.size() <= dev) g_ws.resize(dev + 1);
-    Workspace& w = g_ws[dev];
-    if (w.bytes < bytes) {
-        if (w.ptr) {
-            e = hipFree(w.ptr);
-            if (e != hipSuccess) return hip_fail(e, "hipFree(workspace)");
-            w.ptr = nullptr;
-            w.bytes = 0;
-        }
-        e = hipMalloc(&w.ptr, bytes);
-        if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
-        w.bytes = bytes;
-    }
-    *out = w.ptr;
+    std::lock_guard<std::mutex> lock(g_ctx_mutex);
+    *out = &g_ctx[{dev, s}];  // std::map nodes are stable
     return HL_OK;
 }
 
-// Side stream (default priority) and fork/join events per device, for the
-// near-body fix-up that runs concurrently with the tiled kernel. Created on
-// first use, kept until the process ends. g_side_mutex serialises the
-// enqueue of steps that use them.
-struct SideStream {
-    hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-};
-std::mutex g_side_mutex;
-std::vector<SideStream> g_side;
-
-int32_t side_stream(SideStream** out)
+// Grows c's workspace to at least `bytes`. Work already enqueued on the
+// caller's stream (and the side stream) may still read the old buffer, so
+// both are drained before it is freed.
+int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
 {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    if ((int)g_side.size() <= dev) g_side.resize(dev + 1);
-    SideStream& ss = g_side[dev];
-    if (!ss.stream) {
-        if ((e = hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
-        if ((e = hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-        if ((e = hipEventCreateWithFlags(&ss.join, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    hipError_t e;
+    if (c->bytes < bytes) {
+        if (c->ws) {
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            if (c->side && (e = hipStreamSynchronize(c->side)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            if ((e = hipFree(c->ws)) != hipSuccess) return hip_fail(e, "hipFree(workspace)");
+            c->ws = nullptr;
+            c->bytes = 0;
+        }
+        e = hipMalloc(&c->ws, bytes);
+        if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
+        c->bytes = bytes;
     }
-    *out = &ss;
+    *out = c->ws;
+    return HL_OK;
+}
+
+int32_t side_stream(StreamCtx* c)
+{
+    hipError_t e;
+    if (!c->side) {
+        if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        if ((e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        if ((e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    }
     return HL_OK;
 }
 
@@ -228,20 +231,23 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
     const bool tiled = kernel != HL_KERNEL_GATHER && hl::tiling_supported(env->lattice, 32, 8, 1) &&
                        env->dtype != HL_F64;
     const bool want_forces = force_history && env->nsolid > 0;
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
     if (env->mask && nsteps > 0 && (tiled || want_forces)) {
         hl::EnvArgs a0 = make_args(env, cur);
-        rc = workspace(hl::forces_workspace_bytes(a0, env->lattice), &ws);
+        rc = workspace(ctx, s, hl::forces_workspace_bytes(a0, env->lattice), &ws);
         if (rc) return rc;
         hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
         if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
     }
-    std::unique_lock<std::mutex> side_lock(g_side_mutex, std::defer_lock);
-    SideStream* ss = nullptr;
+    StreamCtx* ss = nullptr;
     if (tiled && ws) {
-        side_lock.lock();
-        rc = side_stream(&ss);
+        rc = side_stream(ctx);
         if (rc) return rc;
+        ss = ctx;
     }
     for (int k = 0; k < nsteps; ++k) {
         hl::EnvArgs a = make_args(env, cur);
@@ -254,10 +260,10 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
                 // with the tiled kernel; they write disjoint cells
                 hl::cut_bits_view(env->lattice, a, ws);
                 if ((e = hipEventRecord(ss->fork, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-                if ((e = hipStreamWaitEvent(ss->stream, ss->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
-                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, ss->stream);
+                if ((e = hipStreamWaitEvent(ss->side, ss->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
+                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, ss->side);
                 if (e != hipSuccess) return hip_fail(e, "fix-up kernel launch");
-                if ((e = hipEventRecord(ss->join, ss->stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+                if ((e = hipEventRecord(ss->join, ss->side)) != hipSuccess) return hip_fail(e, "hipEventRecord");
             }
             e = hl::step_tiled(env->lattice, env->dtype, a, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
@@ -291,10 +297,15 @@ int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
     if (!out) return fail(HL_E_INVALID, "out is NULL");
     if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
     hl::EnvArgs a = make_args(env, cur);
-    void* ws = nullptr;
-    rc = workspace(hl::forces_workspace_bytes(a, env->lattice), &ws);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
     if (rc) return rc;
-    hipError_t e = hl::forces(env->lattice, env->dtype, a, out, ws, static_cast<hipStream_t>(stream));
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    void* ws = nullptr;
+    rc = workspace(ctx, s, hl::forces_workspace_bytes(a, env->lattice), &ws);
+    if (rc) return rc;
+    hipError_t e = hl::forces(env->lattice, env->dtype, a, out, ws, s);
     if (e != hipSuccess) return hip_fail(e, "forces kernel launch");
     return HL_OK;
 }
@@ -368,10 +379,15 @@ int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
     if (!out) return fail(HL_E_INVALID, "out is NULL");
     if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
     hl::EnvArgs a = make_args(env, cur);
-    void* ws = nullptr;
-    rc = workspace(hl::mass_workspace_bytes(a, env->lattice), &ws);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
     if (rc) return rc;
-    hipError_t e = hl::mass(env->lattice, env->dtype, a, out, ws, static_cast<hipStream_t>(stream));
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    void* ws = nullptr;
+    rc = workspace(ctx, s, hl::mass_workspace_bytes(a, env->lattice), &ws);
+    if (rc) return rc;
+    hipError_t e = hl::mass(env->lattice, env->dtype, a, out, ws, s);
     if (e != hipSuccess) return hip_fail(e, "mass kernel launch");
     return HL_OK;
 }
