@@ -48,6 +48,20 @@ def parse():
     return p.parse_args()
 
 
+def measured_traffic(tile):
+    """HBM bytes per launch of the tiled kernel at this tile shape, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or None."""
+    import re
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_tiled_d3q27_sphere_256x128x128.md")
+    if not os.path.exists(path):
+        return None
+    key = "x".join(str(t) for t in tile)
+    m = re.search(r"^traffic_bytes_per_launch\[" + key + r"\]: (\d+)$", open(path).read(), re.M)
+    return int(m.group(1)) if m else None
+
+
 def cpu_baseline(w, seconds: float):
     """Time the oracle's OpenMP FP32 step on the host cores (bounded sample)."""
     from oracle import oracle as orc  # the checker / baseline, never the product
@@ -206,6 +220,9 @@ def main():
     if not np.isfinite(lat.fields()[0].float().sum().item()):
         raise RuntimeError("non-finite density after the benchmark run")
 
+    tile = N.get_tiling(27)
+    traffic = measured_traffic(tile) if (w.lattice == 27 and args.dtype == "f32" and w.name.endswith("256x128x128")
+                                         and args.kernel != "gather") else None
     mlups = w.ncell * K / (ms_total * 1e-3) / 1e6
     b_cell = lat.bytes_per_cell_update
     traffic_alg = b_cell * w.ncell
@@ -258,8 +275,9 @@ def main():
                    "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve,
                    "precision": prec},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel_us": round(kernel_ms * 1e3, 2)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel_us": round(kernel_ms * 1e3, 2),
+                     "tile": list(tile) if w.lattice == 27 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

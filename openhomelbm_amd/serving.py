@@ -383,7 +383,7 @@ def poisson_arrivals(rate: float, duration: float, env_ids: Sequence[int], seed:
         out.append((t, int(rng.choice(env_ids)), int(rng.choice(ks))))
 
 
-def sweep(rates: Sequence[float] = (500.0, 2000.0, 8000.0), duration: float = 1.0, n_envs: int = 32,
+def sweep(rates: Sequence[float] = (250.0, 500.0, 1000.0, 2000.0), duration: float = 1.0, n_envs: int = 32,
           hbm_budget: float = 200e9, seed: int = 5005, log=None) -> dict:
     """Serve Poisson request streams at each rate over one resident pool;
     report throughput and p50/p99 request latency per rate."""
