@@ -37,6 +37,8 @@ def parse():
                    help="steps of the 64-eel batched run reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--serving-seconds", type=float, default=1.0,
                    help="seconds per arrival rate of the serving sweep reported beside the D3Q27 line (0 = skip)")
+    p.add_argument("--karman-steps", type=int, default=40000,
+                   help="steps of the FP32 vs FP64 Karman drag/Strouhal check reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--precision-steps", type=int, default=5000,
                    help="steps of the FP64/FP32/FP16/BF16 eel sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
@@ -242,6 +244,15 @@ def main():
 
         serve = serving.sweep(duration=args.serving_seconds, log=log)
 
+    karm = None
+    if args.karman_steps > 0 and args.workload == "d3q27_sphere":
+        from openhomelbm_amd import karman
+
+        t0k = time.perf_counter()
+        karm = karman.compare(steps=args.karman_steps, window=min(25000, args.karman_steps // 2), log=log)
+        karm["seconds"] = round(time.perf_counter() - t0k, 1)
+        log(f"[bench] karman {karm}")
+
     prec = None
     if args.precision_steps > 0 and args.workload == "d3q27_sphere":
         from openhomelbm_amd import precision
@@ -272,7 +283,7 @@ def main():
         "config": {"workload": w.name, "lattice": "D3Q27" if w.lattice == 27 else "D2Q9",
                    "grid": [w.nx, w.ny, w.nz], "nu": w.nu, "u_in": w.u_in[0], "kernel": args.kernel,
                    "step": "stream/collide/boundary kernel + FP64 per-solid force reduction",
-                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve,
+                   "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve, "karman": karm,
                    "precision": prec},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
