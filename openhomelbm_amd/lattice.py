@@ -146,10 +146,13 @@ class Lattice:
             e.force = self.force.data_ptr()
         e.solids = self.solids.data_ptr() if self.nsolid > 0 else None
         e.nu = float(self.nu)
+        # faces: 2D left, top, right, bottom; 3D left, right, top, bottom, front, back
+        # (shorter lists leave the remaining faces zero-gradient / zero)
         for k in range(6):
-            e.bc_type[k] = int(self.bc_type[k])
+            e.bc_type[k] = int(self.bc_type[k]) if k < len(self.bc_type) else 1
+            v = self.bc_value[k] if k < len(self.bc_value) else ()
             for a in range(3):
-                e.bc_value[k][a] = float(self.bc_value[k][a])
+                e.bc_value[k][a] = float(v[a]) if a < len(v) else 0.0
         for f in range(10):
             e.scale[f] = float(self.scale[f])
         return e
