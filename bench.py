@@ -64,6 +64,28 @@ def measured_traffic(tile):
     return int(m.group(1)) if m else None
 
 
+def copy_bandwidth(mib: int = 2048, reps: int = 10) -> float:
+    """Device-to-device copy rate (GB/s, read + write bytes) of torch's copy
+    kernel on a buffer far larger than the Infinity Cache: the practical HBM
+    ceiling beside the 8 TB/s vendor peak."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b
+    return 2.0 * n * 4 / sec / 1e9
+
+
 def cpu_baseline(w, seconds: float):
     """Time the oracle's OpenMP FP32 step on the host cores (bounded sample)."""
     from oracle import oracle as orc  # the checker / baseline, never the product
@@ -233,6 +255,9 @@ def main():
         f"({achieved / HBM_PEAK_GBS * 100:.1f}% of HBM peak); mass drift {(mass1 - mass0) / mass0:.3e}; "
         f"drag {f_last[0]:.5f}")
 
+    copy_gbs = copy_bandwidth()
+    log(f"[bench] device copy {copy_gbs:.0f} GB/s (read + write)")
+
     batched = None
     if args.batched_steps > 0 and args.workload == "d3q27_sphere":
         batched = bench_batched(args.batched_steps, 20, min(args.batched_steps, 100))
@@ -287,7 +312,8 @@ def main():
                    "precision": prec},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel_us": round(kernel_ms * 1e3, 2),
+                     "kernel_us": round(kernel_ms * 1e3, 2), "copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4),
                      "tile": list(tile) if w.lattice == 27 else None},
         "cpu_baseline": cpu,
     }

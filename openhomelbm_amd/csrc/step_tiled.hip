@@ -31,6 +31,10 @@
 //     cells of plane z pull from one buffer, the sources of plane z+2 fill
 //     the other, so each plane costs one block barrier and the two halves
 //     of the work can overlap within a wavefront.
+// Contraction is off: every fused multiply-add below is written out, so the
+// arithmetic of a cell is the same in every tile-shape instantiation.
+#pragma clang fp contract(off)
+
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -46,102 +50,131 @@ constexpr int kDown[9] = {6, 10, 12, 15, 17, 20, 21, 24, 26}; // cz = -1
 // grouped by direction pairs: f_{+c} = w rho (E + O), f_{-c} = w rho (E - O)
 // with even part E = A + 4.5 (c.u)^2 + 1.5 sum_a c_a^2 S_aa
 // + 3 sum_{a<b} c_a c_b S_ab, A = 1 - 1.5 u.u - 0.5 tr S, and odd part
-// O = 3 c.u. Populations are clamped at 0 as on the reference's plain
+// O = 3 c.u, evaluated as f_{+/-c} = wrho E +/- (3 wrho) c.u with fused
+// multiply-adds. Populations are clamped at 0 as on the reference's plain
 // streaming branch (lbm_func_3d.py:525-526).
 __device__ __forceinline__ void recon27(const float* m, float* f)
 {
     const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
     const float sxx = m[4], syy = m[5], szz = m[6], sxy = m[7], sxz = m[8], syz = m[9];
-    const float ux2 = ux * ux, uy2 = uy * uy, uz2 = uz * uz;
     const float trS = sxx + syy + szz;
-    const float A = 1.0f - 1.5f * (ux2 + uy2 + uz2) - 0.5f * trS;
+    const float usq = fmaf(ux, ux, fmaf(uy, uy, uz * uz));
+    const float A = fmaf(-1.5f, usq, fmaf(-0.5f, trS, 1.0f));
     const float r0 = (float)(8.0 / 27.0) * rho, r1 = (float)(2.0 / 27.0) * rho;
     const float r2 = (float)(1.0 / 54.0) * rho, r3 = (float)(1.0 / 216.0) * rho;
+    const float o1 = 3.0f * r1, o2 = 3.0f * r2, o3 = 3.0f * r3;
+    // pair (p, q) = (+c, -c) with projection s = c.u and even part E
+    auto pair = [&](int p, int q, float r, float o, float E, float s) {
+        const float e = r * E;
+        f[p] = fmaf(o, s, e);
+        f[q] = fmaf(-o, s, e);
+    };
     f[0] = r0 * A;
-    {   // axes
-        float E = A + 4.5f * ux2 + 1.5f * sxx, O = 3.0f * ux;
-        f[1] = r1 * (E + O); f[2] = r1 * (E - O);
-        E = A + 4.5f * uy2 + 1.5f * syy; O = 3.0f * uy;
-        f[3] = r1 * (E + O); f[4] = r1 * (E - O);
-        E = A + 4.5f * uz2 + 1.5f * szz; O = 3.0f * uz;
-        f[5] = r1 * (E + O); f[6] = r1 * (E - O);
-    }
-    {   // edges
-        const float Pxy = A + 1.5f * (sxx + syy), Pxz = A + 1.5f * (sxx + szz), Pyz = A + 1.5f * (syy + szz);
-        float s = ux + uy, E = Pxy + 3.0f * sxy + 4.5f * s * s, O = 3.0f * s;
-        f[7] = r2 * (E + O); f[8] = r2 * (E - O);
-        s = ux - uy; E = Pxy - 3.0f * sxy + 4.5f * s * s; O = 3.0f * s;
-        f[13] = r2 * (E + O); f[14] = r2 * (E - O);
-        s = ux + uz; E = Pxz + 3.0f * sxz + 4.5f * s * s; O = 3.0f * s;
-        f[9] = r2 * (E + O); f[10] = r2 * (E - O);
-        s = ux - uz; E = Pxz - 3.0f * sxz + 4.5f * s * s; O = 3.0f * s;
-        f[15] = r2 * (E + O); f[16] = r2 * (E - O);
-        s = uy + uz; E = Pyz + 3.0f * syz + 4.5f * s * s; O = 3.0f * s;
-        f[11] = r2 * (E + O); f[12] = r2 * (E - O);
-        s = uy - uz; E = Pyz - 3.0f * syz + 4.5f * s * s; O = 3.0f * s;
-        f[17] = r2 * (E + O); f[18] = r2 * (E - O);
-    }
-    {   // corners
-        const float R = A + 1.5f * trS;
-        float s = ux + uy + uz, E = R + 3.0f * (sxy + sxz + syz) + 4.5f * s * s, O = 3.0f * s;
-        f[19] = r3 * (E + O); f[20] = r3 * (E - O);
-        s = ux + uy - uz; E = R + 3.0f * (sxy - sxz - syz) + 4.5f * s * s; O = 3.0f * s;
-        f[21] = r3 * (E + O); f[22] = r3 * (E - O);
-        s = ux - uy + uz; E = R + 3.0f * (sxz - sxy - syz) + 4.5f * s * s; O = 3.0f * s;
-        f[23] = r3 * (E + O); f[24] = r3 * (E - O);
-        s = uy + uz - ux; E = R + 3.0f * (syz - sxy - sxz) + 4.5f * s * s; O = 3.0f * s;
-        f[25] = r3 * (E + O); f[26] = r3 * (E - O);
-    }
+    pair(1, 2, r1, o1, fmaf(4.5f * ux, ux, fmaf(1.5f, sxx, A)), ux);
+    pair(3, 4, r1, o1, fmaf(4.5f * uy, uy, fmaf(1.5f, syy, A)), uy);
+    pair(5, 6, r1, o1, fmaf(4.5f * uz, uz, fmaf(1.5f, szz, A)), uz);
+    const float Pxy = fmaf(1.5f, sxx + syy, A), Pxz = fmaf(1.5f, sxx + szz, A), Pyz = fmaf(1.5f, syy + szz, A);
+    float t = ux + uy;
+    pair(7, 8, r2, o2, fmaf(4.5f * t, t, fmaf(3.0f, sxy, Pxy)), t);
+    t = ux - uy;
+    pair(13, 14, r2, o2, fmaf(4.5f * t, t, fmaf(-3.0f, sxy, Pxy)), t);
+    t = ux + uz;
+    pair(9, 10, r2, o2, fmaf(4.5f * t, t, fmaf(3.0f, sxz, Pxz)), t);
+    t = ux - uz;
+    pair(15, 16, r2, o2, fmaf(4.5f * t, t, fmaf(-3.0f, sxz, Pxz)), t);
+    t = uy + uz;
+    pair(11, 12, r2, o2, fmaf(4.5f * t, t, fmaf(3.0f, syz, Pyz)), t);
+    t = uy - uz;
+    pair(17, 18, r2, o2, fmaf(4.5f * t, t, fmaf(-3.0f, syz, Pyz)), t);
+    const float R = fmaf(1.5f, trS, A);
+    t = ux + uy + uz;
+    pair(19, 20, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, sxy + sxz + syz, R)), t);
+    t = ux + uy - uz;
+    pair(21, 22, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, sxy - sxz - syz, R)), t);
+    t = ux - uy + uz;
+    pair(23, 24, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, sxz - sxy - syz, R)), t);
+    t = uy + uz - ux;
+    pair(25, 26, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, syz - sxy - sxz, R)), t);
 #pragma unroll
     for (int i = 0; i < 27; ++i) f[i] = fmaxf(f[i], 0.0f);
 }
 
-// Moment accumulators of the pulled populations, grouped by the sign of each
-// velocity component (the reference's sign groups, lbm_func_3d.py:528-562).
-struct Acc {
-    float xp = 0, xm = 0, x0 = 0, yp = 0, ym = 0, zp = 0, zm = 0;
-    float pxy = 0, pxz = 0, pyz = 0;
-};
-
-template <int I>
-__device__ __forceinline__ void accumulate(Acc& s, float g)
+// Direction index of velocity (cx, cy, cz) in the reference's order.
+__host__ __device__ constexpr int dir_index(int cx, int cy, int cz)
 {
-    constexpr int cx = D3CX[I], cy = D3CY[I], cz = D3CZ[I];
-    if (cx > 0) s.xp += g; else if (cx < 0) s.xm += g; else s.x0 += g;
-    if (cy > 0) s.yp += g; else if (cy < 0) s.ym += g;
-    if (cz > 0) s.zp += g; else if (cz < 0) s.zm += g;
-    if (cx * cy > 0) s.pxy += g; else if (cx * cy < 0) s.pxy -= g;
-    if (cx * cz > 0) s.pxz += g; else if (cx * cz < 0) s.pxz -= g;
-    if (cy * cz > 0) s.pyz += g; else if (cy * cz < 0) s.pyz -= g;
+    for (int i = 0; i < 27; ++i)
+        if (D3CX[i] == cx && D3CY[i] == cy && D3CZ[i] == cz) return i;
+    return -1;
 }
 
-// Regularized collision of lbm_func_3d.py:564-604 on the accumulated
-// moments, written as trace / deviator relaxation:
+// Zeroth, first and second moments of the 27 pulled populations g: rho,
+// j = sum c g, Pi = sum c c g (the reference's sign-group sums,
+// lbm_func_3d.py:528-562), reduced hierarchically: first along z for each
+// (cx, cy) column (sum, z-difference, z-pair sum), then along y for each
+// cx, then along x. 66 additions instead of one accumulator update per
+// population and component. out: rho, jx, jy, jz, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz.
+__device__ __forceinline__ void moments27(const float* g, float* out)
+{
+    float T[3][3], Dz[3][3], Qz[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float m = g[dir_index(a - 1, b - 1, -1)], o = g[dir_index(a - 1, b - 1, 0)];
+            const float p = g[dir_index(a - 1, b - 1, 1)];
+            Qz[a][b] = m + p;
+            T[a][b] = Qz[a][b] + o;
+            Dz[a][b] = p - m;
+        }
+    float Ty[3], QyT[3], DyT[3], DzS[3], DzDy[3], QzS[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        QyT[a] = T[a][0] + T[a][2];
+        Ty[a] = QyT[a] + T[a][1];
+        DyT[a] = T[a][2] - T[a][0];
+        DzS[a] = (Dz[a][0] + Dz[a][2]) + Dz[a][1];
+        DzDy[a] = Dz[a][2] - Dz[a][0];
+        QzS[a] = (Qz[a][0] + Qz[a][2]) + Qz[a][1];
+    }
+    const float pxx = Ty[0] + Ty[2];
+    out[0] = pxx + Ty[1];
+    out[1] = Ty[2] - Ty[0];
+    out[2] = (DyT[0] + DyT[2]) + DyT[1];
+    out[3] = (DzS[0] + DzS[2]) + DzS[1];
+    out[4] = pxx;
+    out[5] = (QyT[0] + QyT[2]) + QyT[1];
+    out[6] = (QzS[0] + QzS[2]) + QzS[1];
+    out[7] = DyT[2] - DyT[0];
+    out[8] = DzS[2] - DzS[0];
+    out[9] = (DzDy[0] + DzDy[2]) + DzDy[1];
+}
+
+// Regularized collision of lbm_func_3d.py:564-604 on the moments (rho, j, Pi)
+// of the pulled populations, written as trace / deviator relaxation:
 //   S'_aa = (1 - Om) (dev_aa / rho + u.u / 3) + Om u_a^2 + F_a u_a / rho
 //   S'_ab = (1 - Om) Pi_ab / rho + Om u_a u_b + (F_a u_b + F_b u_a) / (2 rho)
 // which equals the reference's Pi'/rho - 1/3 without the cancellation
 // against 1/3. out: rho, u + F/(2 rho), S (storage order).
-__device__ __forceinline__ void collide27(const Acc& s, const float* F, float omega, float* out)
+__device__ __forceinline__ void collide27(const float* mo, const float* F, float omega, float* out)
 {
-    const float pxx = s.xp + s.xm, pyy = s.yp + s.ym, pzz = s.zp + s.zm;
-    const float rho = pxx + s.x0;
+    const float rho = mo[0], pxx = mo[4], pyy = mo[5], pzz = mo[6];
     const float inv = 1.0f / rho;
-    const float ux = (s.xp - s.xm + 0.5f * F[0]) * inv;
-    const float uy = (s.yp - s.ym + 0.5f * F[1]) * inv;
-    const float uz = (s.zp - s.zm + 0.5f * F[2]) * inv;
+    const float hx = 0.5f * F[0] * inv, hy = 0.5f * F[1] * inv, hz = 0.5f * F[2] * inv;  // F / (2 rho)
+    const float ux = fmaf(mo[1], inv, hx), uy = fmaf(mo[2], inv, hy), uz = fmaf(mo[3], inv, hz);
     const float t3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
-    const float usq3 = (ux * ux + uy * uy + uz * uz) * (1.0f / 3.0f);
+    const float usq3 = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (1.0f / 3.0f);
     const float om1 = 1.0f - omega;
     out[0] = rho;
-    out[1] = ux + 0.5f * F[0] * inv;
-    out[2] = uy + 0.5f * F[1] * inv;
-    out[3] = uz + 0.5f * F[2] * inv;
-    out[4] = om1 * ((pxx - t3) * inv + usq3) + omega * ux * ux + F[0] * ux * inv;
-    out[5] = om1 * ((pyy - t3) * inv + usq3) + omega * uy * uy + F[1] * uy * inv;
-    out[6] = om1 * ((pzz - t3) * inv + usq3) + omega * uz * uz + F[2] * uz * inv;
-    out[7] = om1 * s.pxy * inv + omega * ux * uy + 0.5f * (F[0] * uy + F[1] * ux) * inv;
-    out[8] = om1 * s.pxz * inv + omega * ux * uz + 0.5f * (F[0] * uz + F[2] * ux) * inv;
-    out[9] = om1 * s.pyz * inv + omega * uy * uz + 0.5f * (F[1] * uz + F[2] * uy) * inv;
+    out[1] = ux + hx;
+    out[2] = uy + hy;
+    out[3] = uz + hz;
+    // forcing terms F_a u_b / rho written as 2 h_a u_b
+    out[4] = fmaf(om1, fmaf(pxx - t3, inv, usq3), fmaf(omega * ux, ux, 2.0f * hx * ux));
+    out[5] = fmaf(om1, fmaf(pyy - t3, inv, usq3), fmaf(omega * uy, uy, 2.0f * hy * uy));
+    out[6] = fmaf(om1, fmaf(pzz - t3, inv, usq3), fmaf(omega * uz, uz, 2.0f * hz * uz));
+    out[7] = fmaf(om1 * mo[7], inv, fmaf(omega * ux, uy, fmaf(hx, uy, hy * ux)));
+    out[8] = fmaf(om1 * mo[8], inv, fmaf(omega * ux, uz, fmaf(hx, uz, hz * ux)));
+    out[9] = fmaf(om1 * mo[9], inv, fmaf(omega * uy, uz, fmaf(hy, uz, hz * uy)));
 }
 
 // One face condition with the equilibrium S in closed form: f_eq_home_3d
@@ -172,13 +205,20 @@ __device__ __forceinline__ void load_moments(const EnvArgs& a, int64_t c, float*
     for (int f = 0; f < 10; ++f) m[f] = Storage<DT>::template load<float>(src, f * a.stride + c, f, a.inv_scale[f]);
 }
 
+// FP32 results are written with non-temporal stores: the step streams the
+// whole lattice, so the lines are not worth keeping in L2 for this launch.
 template <int DT>
 __device__ __forceinline__ void store_moments(const EnvArgs& a, int64_t c, const float* m)
 {
     using ST = typename Storage<DT>::type;
     ST* dst = static_cast<ST*>(a.dst);
+    if constexpr (DT == HL_F32) {
 #pragma unroll
-    for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+        for (int f = 0; f < 10; ++f) __builtin_nontemporal_store(m[f], reinterpret_cast<float*>(dst) + f * a.stride + c);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+    }
 }
 
 }  // namespace
@@ -223,25 +263,27 @@ k_tiled_d3q27(EnvArgs a, int zc)
     const bool near = a.cut_bits != nullptr && x0 + TX > a.cut_origin[0] && x0 < a.cut_origin[0] + a.cut_extent[0] &&
                       y0 + TY > a.cut_origin[1] && y0 < a.cut_origin[1] + a.cut_extent[1];
 
-    float hold_mid[9], hold_up0[9], hold_up1[9];
+    // up pops of the last two source planes, alternating by plane parity so
+    // that no hold has to be copied into another
+    float hold_mid[9], up_a[9], up_b[9];
     float pfa[10], pfb[10];  // prefetched source moments, alternating by plane
 
     // Source work for plane k: rebuild its 27 populations, write to buf the
     // down pops of plane k, the held mid pops of plane k-1 and the held up
     // pops of plane k-2, then shift the holds.
-    auto source_plane = [&](float(*buf)[NCOL], const float* m) {
+    // U holds the up pops of plane k-2 on entry and those of plane k on exit.
+    auto source_plane = [&](float(*buf)[NCOL], const float* m, float* U) {
         float f[27];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             buf[kMid[k]][tid] = hold_mid[k];
-            buf[kUp[k]][tid] = hold_up0[k];
-            hold_up0[k] = hold_up1[k];
+            buf[kUp[k]][tid] = U[k];
         }
         recon27(m, f);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
             buf[kDown[k]][tid] = f[kDown[k]];
-            hold_up1[k] = f[kUp[k]];
+            U[k] = f[kUp[k]];
             hold_mid[k] = f[kMid[k]];
         }
     };
@@ -249,17 +291,14 @@ k_tiled_d3q27(EnvArgs a, int zc)
     // Pull, collide and store the cells of plane z from buf.
     auto dest_plane = [&](float(*buf)[NCOL], int z) {
         const int64_t pc = (int64_t)z * plane + pcol;
-        Acc acc;
-#define HL_PULL(I) accumulate<I>(acc, buf[I][cp - D3CX[I] - D3CY[I] * NX]);
-        HL_PULL(0) HL_PULL(1) HL_PULL(2) HL_PULL(3) HL_PULL(4) HL_PULL(5) HL_PULL(6) HL_PULL(7) HL_PULL(8)
-        HL_PULL(9) HL_PULL(10) HL_PULL(11) HL_PULL(12) HL_PULL(13) HL_PULL(14) HL_PULL(15) HL_PULL(16)
-        HL_PULL(17) HL_PULL(18) HL_PULL(19) HL_PULL(20) HL_PULL(21) HL_PULL(22) HL_PULL(23) HL_PULL(24)
-        HL_PULL(25) HL_PULL(26)
-#undef HL_PULL
+        float g[27], mo[10];
+#pragma unroll
+        for (int i = 0; i < 27; ++i) g[i] = buf[i][cp - D3CX[i] - D3CY[i] * NX];
+        moments27(g, mo);
         float F[3] = {0.0f, 0.0f, 0.0f};
         if (FORCE) { F[0] = a.force[pc]; F[1] = a.force[a.stride + pc]; F[2] = a.force[2 * a.stride + pc]; }
         float out[10];
-        collide27(acc, F, omega, out);
+        collide27(mo, F, omega, out);
         // faces: x, then y (this plane), then z (planes 0 and nz-1 from planes 1 and nz-2)
         if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
         else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
@@ -286,14 +325,14 @@ k_tiled_d3q27(EnvArgs a, int zc)
         load_moments<DT>(a, (int64_t)(zb - 1) * plane + scol, m);
         recon27(m, f);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) hold_up0[k] = f[kUp[k]];
+        for (int k = 0; k < 9; ++k) up_a[k] = f[kUp[k]];
         load_moments<DT>(a, (int64_t)zb * plane + scol, m);
         recon27(m, f);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; hold_up1[k] = f[kUp[k]]; }
+        for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; up_b[k] = f[kUp[k]]; }
         load_moments<DT>(a, (int64_t)(zb + 1) * plane + scol, m);
         if (zb + 2 <= ze) load_moments<DT>(a, (int64_t)(zb + 2) * plane + scol, pfa);
-        source_plane(sp[0], m);
+        source_plane(sp[0], m, up_a);  // plane zb+1: up(zb-1) out, up(zb+1) in
     }
     __syncthreads();
 
@@ -302,10 +341,12 @@ k_tiled_d3q27(EnvArgs a, int zc)
     // the other buffer, then prefetches plane z+3; one barrier per plane. The
     // loop is unrolled by two so each prefetch lands in the registers it is
     // consumed from (no loop-carried copies that would wait on the loads).
-    auto step = [&](int z, const float* use, float* into) {
+    // step z rebuilds source plane z+2, whose up-pop hold is up_b when
+    // z - zb is even and up_a when odd (static in the unrolled loop)
+    auto step = [&](int z, const float* use, float* into, float* U) {
         const int cur = (z - zb) & 1;
         if (src_ok && z + 1 < ze) {
-            source_plane(sp[cur ^ 1], use);
+            source_plane(sp[cur ^ 1], use, U);
             // issued after `use` is consumed, so waits on `use` never cover these
             if (z + 3 <= ze) load_moments<DT>(a, (int64_t)(z + 3) * plane + scol, into);
         }
@@ -313,8 +354,8 @@ k_tiled_d3q27(EnvArgs a, int zc)
         __syncthreads();
     };
     for (int z = zb; z < ze; z += 2) {
-        step(z, pfa, pfb);
-        if (z + 1 < ze) step(z + 1, pfb, pfa);
+        step(z, pfa, pfb, up_b);
+        if (z + 1 < ze) step(z + 1, pfb, pfa, up_a);
     }
 }
 
