@@ -8,7 +8,10 @@
 // lbm_func_3d.py:510-511).
 #pragma clang fp contract(off)
 
+#include <cstring>
+
 #include "gather_post.h"
+
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -18,23 +21,6 @@ constexpr int kReduceBlock = 256;
 constexpr int kWaves = kReduceBlock / 64;
 constexpr int kMaxReduceBlocks = 1024;
 
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-__device__ __forceinline__ uint64_t wave_or(uint64_t v)
-{
-    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
-        hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
-    }
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // ---------------------------------------------------------------- init
 
@@ -157,11 +143,6 @@ hipError_t voxelize_polygons(const EnvArgs& a, uint8_t* mask, const float* verts
 }
 
 // -------------------------------------------------- near-body fix-up
-
-struct Region {
-    int x0, y0, z0, w, h, d;
-    __host__ __device__ int64_t n() const { return (int64_t)w * h * d; }
-};
 
 // Cells whose update can involve a cut link: interior cells within one cell
 // of the mask box, plus the face cells that project onto them.
@@ -486,6 +467,30 @@ hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* o
     }
     if (e != hipSuccess || !want_forces) return e;
     hipLaunchKernelGGL(k_sum_partials, width, 64, 0, s, part, nb, width, out);
+    return hipGetLastError();
+}
+
+FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces)
+{
+    FixArgs f;
+    std::memset(&f, 0, sizeof(f));
+    const Region rg = fixup_region(a, lattice);
+    if (a.mask == nullptr || rg.n() == 0 || ws == nullptr) return f;
+    const FixupLayout L = fixup_layout(rg);
+    char* base = static_cast<char*>(ws);
+    f.rg = rg;
+    f.list = reinterpret_cast<const int*>(base + L.list);
+    f.count = reinterpret_cast<const int*>(base + L.total);
+    f.width = a.nsolid * 6;
+    if (forces && f.width > 0) f.partial = reinterpret_cast<double*>(base + L.partial);
+    f.max_waves = kFixupWaves;  // partial rows available
+    return f;
+}
+
+hipError_t sum_partials(const double* partial, int rows, int width, double* out, hipStream_t s)
+{
+    if (width <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_partials, width, 64, 0, s, partial, rows, width, out);
     return hipGetLastError();
 }
 

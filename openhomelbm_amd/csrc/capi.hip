@@ -96,20 +96,16 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
 }
 
 // Library-owned scratch per (device, caller stream): a grow-only reduction
-// workspace (cut-cell list, force partials) and, for the near-body pass that
-// runs concurrently with the tiled kernel, a side stream (default priority)
-// with fork/join events. Calls on different caller streams never share
-// scratch, so their work may overlap; calls on one stream are ordered by that
-// stream (concurrent calls from several host threads on one stream are not
-// supported). Created on first use, kept until the process ends. Call a step or
+// workspace (cut-cell list, force partials).
+// Calls on different caller streams never share scratch, so their work may
+// overlap; calls on one stream are ordered by that stream (concurrent calls
+// from several host threads on one stream are not supported). Created on first use, kept until the process ends. Call a step or
 // reduction once outside any stream capture so the allocation does not happen
 // while capturing.
 struct StreamCtx {
     std::mutex mutex;  // held for the whole call that uses this context
     void* ws = nullptr;
     size_t bytes = 0;
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
 };
 std::mutex g_ctx_mutex;
 std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
@@ -125,15 +121,14 @@ int32_t stream_ctx(hipStream_t s, StreamCtx** out)
 }
 
 // Grows c's workspace to at least `bytes`. Work already enqueued on the
-// caller's stream (and the side stream) may still read the old buffer, so
-// both are drained before it is freed.
+// caller's stream may still read the old buffer, so it is drained before the
+// buffer is freed.
 int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
 {
     hipError_t e;
     if (c->bytes < bytes) {
         if (c->ws) {
             if ((e = hipStreamSynchronize(s)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-            if (c->side && (e = hipStreamSynchronize(c->side)) != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
             if ((e = hipFree(c->ws)) != hipSuccess) return hip_fail(e, "hipFree(workspace)");
             c->ws = nullptr;
             c->bytes = 0;
@@ -143,17 +138,6 @@ int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
         c->bytes = bytes;
     }
     *out = c->ws;
-    return HL_OK;
-}
-
-int32_t side_stream(StreamCtx* c)
-{
-    hipError_t e;
-    if (!c->side) {
-        if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess) return hip_fail(e, "hipStreamCreate");
-        if ((e = hipEventCreateWithFlags(&c->fork, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-        if ((e = hipEventCreateWithFlags(&c->join, hipEventDisableTiming)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-    }
     return HL_OK;
 }
 
@@ -243,43 +227,31 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
         hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
         if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
     }
-    StreamCtx* ss = nullptr;
-    if (tiled && ws) {
-        rc = side_stream(ctx);
-        if (rc) return rc;
-        ss = ctx;
-    }
     for (int k = 0; k < nsteps; ++k) {
         hl::EnvArgs a = make_args(env, cur);
         double* fout = want_forces ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
         hipError_t e = hipErrorNotSupported;
         bool fixed = false;
         if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) {
-            if (ss) {
-                // near-body fix-up (and forces) on the side stream, concurrent
-                // with the tiled kernel; they write disjoint cells
+            // with a body mask, the same launch also updates the near-body
+            // cells (wall rule, reference order) and reduces the forces
+            hl::FixArgs fx;
+            std::memset(&fx, 0, sizeof(fx));
+            hl::FixArgs* pfx = nullptr;
+            if (ws) {
                 hl::cut_bits_view(env->lattice, a, ws);
-                if ((e = hipEventRecord(ss->fork, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-                if ((e = hipStreamWaitEvent(ss->side, ss->fork, 0)) != hipSuccess) return hip_fail(e, "hipStreamWaitEvent");
-                e = hl::fixup(env->lattice, env->dtype, a, true, fout, ws, ss->side);
-                if (e != hipSuccess) return hip_fail(e, "fix-up kernel launch");
-                if ((e = hipEventRecord(ss->join, ss->side)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+                fx = hl::fixup_args(env->lattice, a, ws, fout != nullptr);
+                if (fx.list) pfx = &fx;
             }
-            e = hl::step_tiled(env->lattice, env->dtype, a, s);
+            e = hl::step_tiled(env->lattice, env->dtype, a, pfx, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
                 return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
-            if (ss) {
-                hipError_t e2 = hipStreamWaitEvent(s, ss->join, 0);
-                if (e2 != hipSuccess) return hip_fail(e2, "hipStreamWaitEvent");
-                if (e == hipErrorNotSupported) {
-                    // the tiled kernel declined this shape: redo the step with the gather kernel
-                    a.cut_bits = nullptr;
-                    e = hl::step_gather(env->lattice, env->dtype, a, s);
-                }
-                fixed = true;
-            } else if (e == hipSuccess) {
-                fixed = true;
-                if (fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
+            if (e == hipSuccess) {
+                fixed = pfx != nullptr || fout == nullptr;
+                // the launch's near-body workgroups left one force row each
+                if (pfx && fout) e = hl::sum_partials(fx.partial, fx.nfix, fx.width, fout, s);
+            } else if (e == hipErrorNotSupported) {
+                a.cut_bits = nullptr;  // the gather kernel does every cell
             }
         }
         if (e == hipErrorNotSupported) e = hl::step_gather(env->lattice, env->dtype, a, s);

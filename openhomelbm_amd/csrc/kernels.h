@@ -12,13 +12,42 @@ namespace hl {
 struct TileCfg {
     int tx, ty, zc;
 };
+
+// Half-open block of cells [x0, x0+w) x [y0, y0+h) x [z0, z0+d).
+struct Region {
+    int x0, y0, z0, w, h, d;
+    __host__ __device__ int64_t n() const { return (int64_t)w * h * d; }
+};
+
+// Near-body work that the tiled D3Q27 kernel's workgroups share ahead of
+// their tiled pass: the cells of `list` (count in *count, region order),
+// updated with the wall rule in the reference's operation order; with forces,
+// each of the nfix participating workgroups writes one FP64 partial row
+// (sum_partials adds them).
+struct FixArgs {
+    Region rg;
+    const int* list;
+    const int* count;
+    double* partial;   // [nfix][width], null without forces
+    int width;         // nsolid * 6
+    int max_waves;     // rows available in partial
+    int nfix;          // workgroups sharing the near-body work (set by the launcher)
+};
 bool tiling_supported(int lattice, int tx, int ty, int zc);
 void set_tiling(int lattice, TileCfg c);
 TileCfg get_tiling(int lattice);
 
 hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
 // Returns hipErrorNotSupported when no tiled kernel exists for this case.
-hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
+// With fx (near-body list built by cut_list, a.cut_bits set by
+// cut_bits_view), the same launch also updates the listed cells and, with
+// fx->partial, writes fx->nfix force rows (set here) for sum_partials.
+hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, FixArgs* fx, hipStream_t s);
+// Near-body arguments for one step over ws (null list when there is no mask;
+// partial rows only when forces are wanted).
+FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces);
+// out[k] = sum over rows r of partial[r * width + k], in a fixed order.
+hipError_t sum_partials(const double* partial, int rows, int width, double* out, hipStream_t s);
 
 
 hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, const double m[10], hipStream_t s);

@@ -294,6 +294,25 @@ __device__ __forceinline__ void wall_velocity_3d(const float* solids, int nsolid
     r[0] = d0; r[1] = d1; r[2] = d2;
 }
 
+// Wavefront-wide FP64 sum and 64-bit OR (butterfly over the 64 lanes).
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t wave_or(uint64_t v)
+{
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo |= (uint32_t)__shfl_xor((int)lo, off, 64);
+        hi |= (uint32_t)__shfl_xor((int)hi, off, 64);
+    }
+    return ((uint64_t)hi << 32) | lo;
+}
+
 // Global-memory view of a generic pointer. Pointers read from memory (not
 // kernel arguments) are generic to the compiler, so their accesses become
 // flat instructions, which count against the LDS wait counter as well and

@@ -35,6 +35,9 @@
 // arithmetic of a cell is the same in every tile-shape instantiation.
 #pragma clang fp contract(off)
 
+#include <cstring>
+
+#include "gather_post.h"
 #include "kernels.h"
 #include "lbm_device.h"
 
@@ -223,12 +226,121 @@ __device__ __forceinline__ void store_moments(const EnvArgs& a, int64_t c, const
 
 }  // namespace
 
+// Orders a wavefront's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Near-body share of workgroup blk of the tiled launch (the cells of fx.list
+// dealt two per wavefront and pass over the first fx.nfix workgroups, one
+// lane per direction): the wall rule and the collision in the reference's
+// operation order, as the stand-alone fix-up kernel does (aux_kernels.hip),
+// with per-wave FP64 force sums reduced in wave order to one partial row per
+// workgroup (summed by the next launch). lds: the launch's LDS, used before
+// the workgroup's tiled pass.
+template <int DT, int NT>
+__device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int blk)
+{
+    constexpr int W = NT / 64;
+    float(*pops)[2][32] = reinterpret_cast<float(*)[2][32]>(lds);
+    float(*sums)[2][16] = reinterpret_cast<float(*)[2][16]>(lds + W * 64);
+    double* accs = reinterpret_cast<double*>(lds + W * 96);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, i = lane & 31;
+    const int nw = fx.nfix * W, gw = blk * W + wave;
+    const int width = fx.partial ? fx.width : 0;
+    double* acc = accs + wave * width;
+    for (int k = lane; k < width; k += 64) acc[k] = 0.0;
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const Region rg = fx.rg;
+    const int ncut = *fx.count;
+    for (int base = gw * 2; base < ncut; base += nw * 2) {
+        const int e = base + h;
+        const bool cell = e < ncut;
+        int x = 0, y = 0, z = 0, px = 0, py = 0, pz = 0, mval_c = 0, sid = -1;
+        int64_t c = 0;
+        float F[6] = {0, 0, 0, 0, 0, 0};
+        if (cell) {
+            const int tt = fx.list[e];
+            x = rg.x0 + tt % rg.w;
+            y = rg.y0 + (tt / rg.w) % rg.h;
+            z = rg.z0 + tt / (rg.w * rg.h);
+            px = min(max(x, 1), nx - 2);
+            py = min(max(y, 1), ny - 2);
+            pz = min(max(z, 1), nz - 2);
+            c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
+            mval_c = a.mask[c];
+        }
+        if (cell && i < 27) {
+            float mc[10];
+            load_cell<float, DT>(a, c, 10, mc);
+            pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c);
+            // force of this lane's link (interior cells only: a face cell repeats its projection)
+            if (width && i > 0 && px == x && py == y && pz == z) {
+                const int mv = a.mask[c - D3CX[i] - (int64_t)nx * (D3CY[i] + (int64_t)ny * D3CZ[i])];
+                if ((mv != 0) != (mval_c != 0)) {
+                    const int s2 = mval_c ? mval_c - 1 : mv - 1;
+                    if (s2 < a.nsolid) {
+                        sid = s2;
+                        link_force<float, DT, 27>(a, mc, x, y, z, i, sid, F);
+                    }
+                }
+            }
+        }
+        wave_sync();
+        if (cell && i < 16) sums[wave][h][i] = d3q27_group_sum<float>(pops[wave][h], i);
+        wave_sync();
+        if (cell && i == 0) {
+            float Fx = 0, Fy = 0, Fz = 0, m[10];
+            if (a.force) { Fx = a.force[c]; Fy = a.force[a.stride + c]; Fz = a.force[2 * a.stride + c]; }
+            d3q27_collide_sums<float>(a, sums[wave][h], Fx, Fy, Fz, m);
+            if (px != x || py != y || pz != z) bc_chain<float, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
+            using ST = typename Storage<DT>::type;
+            ST* dst = static_cast<ST*>(a.dst);
+            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+#pragma unroll
+            for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+        }
+        if (width) {
+            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
+            while (uni) {
+                const int sv = __builtin_ctzll(uni);
+                uni &= uni - 1;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) {
+                    const double v = wave_sum(sid == sv ? (double)F[q] : 0.0);
+                    if (lane == 0) acc[sv * 6 + q] += v;
+                }
+            }
+        }
+        wave_sync();  // pops and sums are rewritten by the next pass
+    }
+    if (!width) return;
+    // the workgroup's waves in order, one partial row per workgroup
+    __syncthreads();
+    for (int k = tid; k < width; k += NT) {
+        double v = 0.0;
+        for (int w = 0; w < W; ++w) v += accs[w * width + k];
+        fx.partial[(int64_t)blk * width + k] = v;
+    }
+}
+
 template <int DT, int TX, int TY, bool FORCE>
 __global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
-k_tiled_d3q27(EnvArgs a, int zc)
+k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
 {
     constexpr int NX = TX + 2, NY = TY + 2, NCOL = NX * NY;
+    constexpr int NT = (NCOL + 63) / 64 * 64;
     __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
+    static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) <= sizeof(sp), "near-body work fits the LDS");
+    const int L = (int)blockIdx.x;
+    // the near-body cells of this step, spread over the first fx.nfix
+    // workgroups ahead of their tiled pass (tiled cells skip them)
+    if (fx.list) {
+        if (L < fx.nfix) fixup_block<DT, NT>(a, fx, &sp[0][0][0], L);
+        __syncthreads();  // the LDS now holds populations
+    }
 
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -237,9 +349,6 @@ k_tiled_d3q27(EnvArgs a, int zc)
     // dispatch order, so dispatch slot L runs on XCD L % 8. Give XCD k the
     // contiguous run of tiles [k * per, (k + 1) * per) (x fastest, then y,
     // then z) so that tiles sharing halo cache lines share one L2.
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int ntile = gx * gy * gridDim.z;
-    const int L = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
     const int per = ntile / 8;
     const int T = L < per * 8 ? (L % 8) * per + L / 8 : L;
     const int x0 = (T % gx) * TX, y0 = ((T / gx) % gy) * TY;
@@ -381,41 +490,53 @@ namespace {
 TileCfg tile_config(const EnvArgs&) { return g_tile3d; }
 
 template <int DT, int TX, int TY>
-hipError_t launch_tiled(const EnvArgs& a, int zc, hipStream_t s)
+hipError_t launch_tiled(const EnvArgs& a, int zc, FixArgs* fxp, hipStream_t s)
 {
     // every tile must hold at least 3 columns per axis for the face projection
     if ((a.nx % TX) == 1 || (a.nx % TX) == 2 || (a.ny % TY) == 1 || (a.ny % TY) == 2) return hipErrorNotSupported;
     if (zc < 1) return hipErrorInvalidValue;
     constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
-    dim3 grid((a.nx + TX - 1) / TX, (a.ny + TY - 1) / TY, (a.nz - 2 + zc - 1) / zc);
+    const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY, gz = (a.nz - 2 + zc - 1) / zc;
+    const int ntile = gx * gy * gz;
+    FixArgs fx;
+    std::memset(&fx, 0, sizeof(fx));
+    if (fxp && fxp->list) {
+        fx = *fxp;
+        // every workgroup (up to the partial buffer's rows) takes a share of the
+        // near-body cells before its tiled pass
+        fx.nfix = ntile < fx.max_waves ? ntile : fx.max_waves;
+        if (fx.nfix < 1) return hipErrorInvalidValue;
+        fxp->nfix = fx.nfix;
+    }
+    const dim3 grid(ntile);
     // the body-force field is a template switch: a run-time branch around its
     // loads would make the compiler wait on the in-flight prefetch at the merge
-    if (a.force) hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, true>), grid, dim3(threads), 0, s, a, zc);
-    else hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, false>), grid, dim3(threads), 0, s, a, zc);
+    if (a.force) hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, true>), grid, dim3(threads), 0, s, a, zc, gx, gy, ntile, fx);
+    else hipLaunchKernelGGL((k_tiled_d3q27<DT, TX, TY, false>), grid, dim3(threads), 0, s, a, zc, gx, gy, ntile, fx);
     return hipGetLastError();
 }
 
 template <int DT>
-hipError_t dispatch_tile(const EnvArgs& a, hipStream_t s)
+hipError_t dispatch_tile(const EnvArgs& a, FixArgs* fx, hipStream_t s)
 {
     const TileCfg c = tile_config(a);
-    if (c.tx == 32 && c.ty == 16) return launch_tiled<DT, 32, 16>(a, c.zc, s);
-    if (c.tx == 32 && c.ty == 8) return launch_tiled<DT, 32, 8>(a, c.zc, s);
-    if (c.tx == 64 && c.ty == 8) return launch_tiled<DT, 64, 8>(a, c.zc, s);
-    if (c.tx == 64 && c.ty == 4) return launch_tiled<DT, 64, 4>(a, c.zc, s);
-    if (c.tx == 16 && c.ty == 16) return launch_tiled<DT, 16, 16>(a, c.zc, s);
+    if (c.tx == 32 && c.ty == 16) return launch_tiled<DT, 32, 16>(a, c.zc, fx, s);
+    if (c.tx == 32 && c.ty == 8) return launch_tiled<DT, 32, 8>(a, c.zc, fx, s);
+    if (c.tx == 64 && c.ty == 8) return launch_tiled<DT, 64, 8>(a, c.zc, fx, s);
+    if (c.tx == 64 && c.ty == 4) return launch_tiled<DT, 64, 4>(a, c.zc, fx, s);
+    if (c.tx == 16 && c.ty == 16) return launch_tiled<DT, 16, 16>(a, c.zc, fx, s);
     return hipErrorNotSupported;
 }
 
 }  // namespace
 
-hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, hipStream_t s)
+hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, FixArgs* fx, hipStream_t s)
 {
     if (lattice != HL_D3Q27) return hipErrorNotSupported;
     switch (dtype) {
-    case HL_F32: return dispatch_tile<HL_F32>(a, s);
-    case HL_F16: return dispatch_tile<HL_F16>(a, s);
-    case HL_BF16: return dispatch_tile<HL_BF16>(a, s);
+    case HL_F32: return dispatch_tile<HL_F32>(a, fx, s);
+    case HL_F16: return dispatch_tile<HL_F16>(a, fx, s);
+    case HL_BF16: return dispatch_tile<HL_BF16>(a, fx, s);
     default: return hipErrorNotSupported;  // FP64 runs the gather kernel (the oracle path)
     }
 }
