@@ -35,9 +35,9 @@
 namespace hl {
 
 constexpr int kBatchMaxSolids = 16;
-// step tile: kTX x kTY cells, 256 threads; lane = column, each wave kRows rows
-constexpr int kTX = 64, kTY = 4;
-constexpr int kThreads = 256, kWaves = kThreads / 64, kRows = kTY / kWaves;
+// step tile: kTX x kTY cells, 512 threads; lane = column, each wave kRows rows
+constexpr int kTX = 64, kTY = 8;
+constexpr int kThreads = 512, kWaves = kThreads / 64, kRows = kTY / kWaves;
 constexpr int kLX = kTX + 3, kLY = kTY + 3;  // staged region: 2 cells below, 1 above
 constexpr int kStage = (kLX * kLY + kThreads - 1) / kThreads;
 constexpr int kVoxW = 16;         // voxelizer item: 16 x 16 cells
@@ -131,7 +131,8 @@ __device__ __forceinline__ double bwave_sum(double v)
 __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
 {
     __shared__ EnvArgs ea;
-    __shared__ float tile[6][kLY][kLX];
+    __shared__ float tile[6][kLY][kLX];   // moments (near the body only)
+    __shared__ float ptile[9][kLY][kLX];  // outgoing populations of every staged cell
     __shared__ uint8_t mtile[kLY][kLX + 1];
     __shared__ float pose[kBatchMaxSolids * 8];
     __shared__ double acc[kWaves][kPartialWidth];
@@ -157,16 +158,17 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
     // loads are issued before the first wait
     float v[kStage][6];
     int mv[kStage];
+    bool in[kStage];
 #pragma unroll
     for (int r = 0; r < kStage; ++r) {
         const int k = tid + r * kThreads;
         const int lx = k % kLX, ly = k / kLX;
         const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
-        const bool in = k < kLX * kLY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const int64_t c = (int64_t)(in ? gx : 0) + (int64_t)nx * (in ? gy : 0);
+        in[r] = k < kLX * kLY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const int64_t c = (int64_t)(in[r] ? gx : 0) + (int64_t)nx * (in[r] ? gy : 0);
 #pragma unroll
-        for (int f = 0; f < 6; ++f) v[r][f] = in ? src[f * st + c] : 0.0f;
-        mv[r] = (masked && in) ? mask[c] : 0;
+        for (int f = 0; f < 6; ++f) v[r][f] = in[r] ? src[f * st + c] : 0.0f;
+        mv[r] = (masked && in[r]) ? mask[c] : 0;
     }
     float pv = 0.0f;
     if (masked && poses && tid < nsolid * 8) pv = poses[tid];
@@ -175,13 +177,25 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
         reinterpret_cast<uint32_t*>(&ea)[tid] = reinterpret_cast<const uint32_t*>(&be.base)[tid];
     if (near)
         for (int k = tid; k < kWaves * kPartialWidth; k += kThreads) (&acc[0][0])[k] = 0.0;
+    // each staged cell rebuilds its 9 outgoing populations once (the pull of
+    // a neighbour reads them instead of rebuilding from the moments); the
+    // values are those the reference's per-link reconstruction gives
 #pragma unroll
     for (int r = 0; r < kStage; ++r) {
         const int k = tid + r * kThreads;
         if (k < kLX * kLY) {
+            const int lx = k % kLX, ly = k / kLX;
+            if (masked) {
 #pragma unroll
-            for (int f = 0; f < 6; ++f) tile[f][k / kLX][k % kLX] = v[r][f];
-            mtile[k / kLX][k % kLX] = (uint8_t)mv[r];
+                for (int f = 0; f < 6; ++f) tile[f][ly][lx] = v[r][f];
+                mtile[ly][lx] = (uint8_t)mv[r];
+            }
+            if (in[r]) {
+                float f9[9];
+                d2q9_pops<float>(v[r], f9);
+#pragma unroll
+                for (int i = 0; i < 9; ++i) ptile[i][ly][lx] = f9[i];
+            }
         }
     }
     if (tid < kBatchMaxSolids * 8) pose[tid] = pv;
@@ -204,28 +218,25 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
         // a face cell takes its interior projection's update, then the face chain
         const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
         const int tx = px - it.x0 + 2, ty = py - it.y0 + 2;
-        const int mval_c = masked ? mtile[ty][tx] : 0;
-        float mc[6];
-#pragma unroll
-        for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
-        // stream_and_collide (lbm_func.py:322-467) from the staged moments
+        // stream_and_collide (lbm_func.py:322-467): the staged populations,
+        // or the wall rule from the cell's own moments on a cut link
         float pop[9];
         uint32_t cb = 0;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            int cut = 0, sid = -1;
-            if (masked) {
+        for (int i = 0; i < 9; ++i) pop[i] = ptile[i][ty - D2CY[i]][tx - D2CX[i]];
+        if (masked) {
+            const int mval_c = mtile[ty][tx];
+            float mc[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
                 const int mval_s = mtile[ty - D2CY[i]][tx - D2CX[i]];
                 if ((mval_s != 0) != (mval_c != 0)) {
-                    cut = 1;
-                    sid = mval_c ? mval_c - 1 : mval_s - 1;
                     cb |= 1u << i;
+                    pop[i] = d2q9_link_pop<float>(ea, px, py, i, mc, 1, mval_c ? mval_c - 1 : mval_s - 1, mc);
                 }
             }
-            float ms[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) ms[f] = tile[f][ty - D2CY[i]][tx - D2CX[i]];
-            pop[i] = d2q9_link_pop<float>(ea, px, py, i, mc, cut, sid, ms);
         }
         float m[6];
         d2q9_collide<float>(ea, pop, 0.0f, 0.0f, m);

@@ -96,14 +96,12 @@ template <> struct Storage<HL_BF16> {
 
 // ------------------------------------------------------- reconstruction
 
-// D2Q9 population i from (rho, u, S): mlCalDistribution,
-// envs/lbm/lbm_func.py:685-788, same operation order.
+// D2Q9 population i from (rho, u, S) and the third-order terms A_xxy, A_xyy:
+// mlCalDistribution, envs/lbm/lbm_func.py:685-788, same operation order.
 template <typename T>
-__device__ __forceinline__ T d2q9_recon(T rho, T ux, T uy, T Sxx, T Syy, T Sxy, int i)
+__device__ __forceinline__ T d2q9_recon_a(T rho, T ux, T uy, T Sxx, T Syy, T Sxy, T Axxy, T Axyy, int i)
 {
-    const T one = 1, two = 2, three = 3, half = T(0.5), c45 = T(4.5), nine = 9;
-    T Axxy = Sxx * uy + two * Sxy * ux - two * ux * ux * uy;
-    T Axyy = Syy * ux + two * Sxy * uy - two * ux * uy * uy;
+    const T one = 1, three = 3, half = T(0.5), c45 = T(4.5), nine = 9;
     const T w0 = T(4) / T(9), w1 = T(1) / T(9), w5 = T(1) / T(36);
     switch (i) {
     case 0: return w0 * rho * (one - half * three * Sxx - half * three * Syy);
@@ -115,6 +113,38 @@ __device__ __forceinline__ T d2q9_recon(T rho, T ux, T uy, T Sxx, T Syy, T Sxy, 
     case 6: return w5 * rho * (one - three * ux + three * uy + three * (Sxx + Syy - three * Sxy) + nine * (Axxy - Axyy));
     case 7: return w5 * rho * (one - three * ux - three * uy + three * (Sxx + Syy + three * Sxy) - nine * (Axxy + Axyy));
     default: return w5 * rho * (one + three * ux - three * uy + three * (Sxx + Syy - three * Sxy) - nine * (Axxy - Axyy));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void d2q9_third(T ux, T uy, T Sxx, T Syy, T Sxy, T& Axxy, T& Axyy)
+{
+    const T two = 2;
+    Axxy = Sxx * uy + two * Sxy * ux - two * ux * ux * uy;
+    Axyy = Syy * ux + two * Sxy * uy - two * ux * uy * uy;
+}
+
+template <typename T>
+__device__ __forceinline__ T d2q9_recon(T rho, T ux, T uy, T Sxx, T Syy, T Sxy, int i)
+{
+    T Axxy, Axyy;
+    d2q9_third<T>(ux, uy, Sxx, Syy, Sxy, Axxy, Axyy);
+    return d2q9_recon_a<T>(rho, ux, uy, Sxx, Syy, Sxy, Axxy, Axyy, i);
+}
+
+// All 9 populations of one cell's moments m (rho, ux, uy, Sxx, Syy, Sxy),
+// each exactly as d2q9_recon computes it, clamped at 0 as on the
+// reference's streaming branch (lbm_func.py:409-412).
+template <typename T>
+__device__ __forceinline__ void d2q9_pops(const T* m, T* f)
+{
+    T Axxy, Axyy;
+    d2q9_third<T>(m[1], m[2], m[3], m[4], m[5], Axxy, Axyy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        T p = d2q9_recon_a<T>(m[0], m[1], m[2], m[3], m[4], m[5], Axxy, Axyy, i);
+        if (p < T(0)) p = T(0);
+        f[i] = p;
     }
 }
 
