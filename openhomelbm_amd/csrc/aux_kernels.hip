@@ -258,6 +258,24 @@ __global__ void __launch_bounds__(64) k_cut_write(const uint64_t* ballots, const
     if ((b >> lane) & 1ull) list[offsets[blk] + __popcll(b & ((1ull << lane) - 1ull))] = blk * 64 + lane;
 }
 
+// Per-column form of the same bitmap, for kernels that march along z: for
+// region column (rx, ry), word j of cols[(ry * w + rx) * nw ...] holds bit k
+// for region plane 64 j + k.
+__global__ void __launch_bounds__(64) k_cut_columns(const uint64_t* ballots, Region rg, int nw, uint64_t* cols)
+{
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (c >= rg.w * rg.h) return;
+    const int64_t layer = (int64_t)rg.w * rg.h;
+    for (int j = 0; j < nw; ++j) {
+        uint64_t word = 0;
+        for (int k = 0; k < 64 && 64 * j + k < rg.d; ++k) {
+            const int64_t t = c + layer * (64 * j + k);
+            word |= ((ballots[t >> 6] >> (t & 63)) & 1ull) << k;
+        }
+        cols[(int64_t)c * nw + j] = word;
+    }
+}
+
 template <typename T, int DT, int Q>
 __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* list, const int* count, int store,
                                                double* partial)
@@ -370,8 +388,8 @@ __global__ void k_zero(double* out, int n)
 constexpr int kFixupWaves = 1024;
 
 struct FixupLayout {
-    size_t partial, ballots, counts, offsets, total, list, bytes;
-    int nblk;
+    size_t partial, ballots, counts, offsets, total, list, cols, bytes;
+    int nblk, colwords;
 };
 
 static FixupLayout fixup_layout(const Region& rg)
@@ -387,6 +405,8 @@ static FixupLayout fixup_layout(const Region& rg)
     L.offsets = take((size_t)(L.nblk > 0 ? L.nblk : 1) * sizeof(int));
     L.total = take(sizeof(int));
     L.list = take((size_t)(n > 0 ? n : 1) * sizeof(int));
+    L.colwords = (rg.d + 63) / 64;
+    L.cols = take((size_t)(n > 0 ? (int64_t)rg.w * rg.h * L.colwords : 1) * sizeof(uint64_t));
     L.bytes = off;
     return L;
 }
@@ -401,10 +421,13 @@ void cut_bits_view(int lattice, EnvArgs& a, void* ws)
     const Region rg = fixup_region(a, lattice);
     if (a.mask == nullptr || rg.n() == 0 || ws == nullptr) {
         a.cut_bits = nullptr;
+        a.cut_cols = nullptr;
         return;
     }
     const FixupLayout L = fixup_layout(rg);
     a.cut_bits = reinterpret_cast<const uint64_t*>(static_cast<char*>(ws) + L.ballots);
+    a.cut_cols = reinterpret_cast<const uint64_t*>(static_cast<char*>(ws) + L.cols);
+    a.cut_colwords = L.colwords;
     a.cut_origin[0] = rg.x0; a.cut_origin[1] = rg.y0; a.cut_origin[2] = rg.z0;
     a.cut_extent[0] = rg.w; a.cut_extent[1] = rg.h; a.cut_extent[2] = rg.d;
 }
@@ -425,6 +448,8 @@ hipError_t cut_list(int lattice, const EnvArgs& a, void* ws, hipStream_t s)
     hipLaunchKernelGGL(k_cut_count, L.nblk, 64, 0, s, a, rg, Q, ballots, counts);
     hipLaunchKernelGGL(k_cut_scan, 1, 1024, 0, s, counts, L.nblk, offsets, total);
     hipLaunchKernelGGL(k_cut_write, L.nblk, 64, 0, s, ballots, offsets, L.nblk, list);
+    uint64_t* cols = reinterpret_cast<uint64_t*>(base + L.cols);
+    hipLaunchKernelGGL(k_cut_columns, (rg.w * rg.h + 63) / 64, 64, 0, s, ballots, rg, L.colwords, cols);
     return hipGetLastError();
 }
 

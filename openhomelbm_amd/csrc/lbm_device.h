@@ -376,6 +376,10 @@ struct EnvArgs {
     const uint64_t* cut_bits;
     int32_t cut_origin[3];
     int32_t cut_extent[3];
+    // the same cells per column: word j of cut_cols[(ry * cw + rx) * cut_colwords ...]
+    // holds bit k for z = cz0 + 64 j + k
+    const uint64_t* cut_cols;
+    int32_t cut_colwords;
 };
 
 __device__ __forceinline__ bool fixup_owns(const EnvArgs& a, int x, int y, int z)

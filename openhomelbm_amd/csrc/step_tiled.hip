@@ -22,11 +22,18 @@
 //   * face cells gather at their projection onto the interior and apply the
 //     face conditions in the order x, y, z (plane 0 and nz-1 are written by
 //     the chunks that own planes 1 and nz-2);
-//   * cells with cut links (next to a body mask) are not written here: the
-//     near-body fix-up kernel, running concurrently on a second stream,
-//     updates them with the mask wall rule in the reference's operation
-//     order and accumulates the body forces; the tiled kernel itself carries
-//     no wall-rule code and so stays small in registers;
+//   * cells with cut links (next to a body mask) are not written by the
+//     tiled pass: the first workgroups of the same launch update them with
+//     the mask wall rule in the reference's operation order and reduce the
+//     body forces before their own tiled pass (fixup_block), so the tiled
+//     pass carries no wall-rule code and stays small in registers;
+//   * global memory goes through raw buffer loads and stores whose
+//     out-of-range offset replaces every per-lane branch (halo columns
+//     outside the lattice, lanes without a destination cell, cells owned by
+//     the near-body pass), so every wave issues the same memory operations
+//     on every path and the vector-memory counter (in order, loads and
+//     stores alike) waits for the prefetched plane only, never for the
+//     previous plane's stores;
 //   * LDS is double-buffered and the loop is software-pipelined: while the
 //     cells of plane z pull from one buffer, the sources of plane z+2 fill
 //     the other, so each plane costs one block barrier and the two halves
@@ -199,29 +206,72 @@ __device__ __forceinline__ void bc_face_closed(float* m, int type, const float* 
     m[1] = ub[0]; m[2] = ub[1]; m[3] = ub[2];
 }
 
-template <int DT>
-__device__ __forceinline__ void load_moments(const EnvArgs& a, int64_t c, float* m)
+// Raw buffer access to a state array: one resource spans all its fields, the
+// per-lane offset selects the column (or kOOB: loads read 0, stores are
+// dropped by the hardware range check), the scalar offset the field and
+// plane. Every lane issues the same loads and stores whatever its role, so
+// the step's memory operations are the same on every control-flow path and
+// the in-order vector-memory counter waits for exactly the loads in use.
+constexpr uint32_t kOOB = 0x80000000u;
+#ifndef WAVES_ATTR
+#define WAVES_ATTR
+#endif
+constexpr int kNT = 2;  // non-temporal: the step streams the whole lattice once
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes)
 {
-    using ST = typename Storage<DT>::type;
-    const ST* src = static_cast<const ST*>(a.src);
-#pragma unroll
-    for (int f = 0; f < 10; ++f) m[f] = Storage<DT>::template load<float>(src, f * a.stride + c, f, a.inv_scale[f]);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
 
-// FP32 results are written with non-temporal stores: the step streams the
-// whole lattice, so the lines are not worth keeping in L2 for this launch.
-template <int DT>
-__device__ __forceinline__ void store_moments(const EnvArgs& a, int64_t c, const float* m)
-{
-    using ST = typename Storage<DT>::type;
-    ST* dst = static_cast<ST*>(a.dst);
-    if constexpr (DT == HL_F32) {
-#pragma unroll
-        for (int f = 0; f < 10; ++f) __builtin_nontemporal_store(m[f], reinterpret_cast<float*>(dst) + f * a.stride + c);
-    } else {
-#pragma unroll
-        for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+template <int DT> struct BufIO;
+template <> struct BufIO<HL_F32> {
+    static constexpr uint32_t size = 4;
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int, float)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0));
     }
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int, float)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, kNT);
+    }
+};
+// 16-bit storage: the encodings of Storage<HL_F16> / Storage<HL_BF16>
+template <> struct BufIO<HL_F16> {
+    static constexpr uint32_t size = 2;
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int f, float inv_scale)
+    {
+        const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)s, 0);
+        const float x = (float)__builtin_bit_cast(_Float16, h) * inv_scale;
+        return f == 0 ? x + 1.0f : x;
+    }
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int f, float scale)
+    {
+        const float w = f == 0 ? x - 1.0f : x;
+        const _Float16 h = (_Float16)(w * scale);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, h), r, (int)v, (int)s, kNT);
+    }
+};
+template <> struct BufIO<HL_BF16> {
+    static constexpr uint32_t size = 2;
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int f, float inv_scale)
+    {
+        const float x = bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)s, 0)) * inv_scale;
+        return f == 0 ? x + 1.0f : x;
+    }
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int f, float scale)
+    {
+        const float w = f == 0 ? x - 1.0f : x;
+        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16(w * scale), r, (int)v, (int)s, kNT);
+    }
+};
+
+// Bits [o, o + 64) of a column's cut bitstring (nw words; bits outside it are 0).
+__device__ __forceinline__ uint64_t column_window(const uint64_t* col, int nw, int o)
+{
+    const int j = o >> 6, sh = o & 63;  // floor division, also for o < 0
+    const uint64_t lo = (j >= 0 && j < nw) ? col[j] : 0ull;
+    const uint64_t hi = (j + 1 >= 0 && j + 1 < nw) ? col[j + 1] : 0ull;
+    return sh == 0 ? lo : (lo >> sh) | (hi << (64 - sh));
 }
 
 }  // namespace
@@ -327,11 +377,13 @@ __device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int
 }
 
 template <int DT, int TX, int TY, bool FORCE>
-__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64)
+__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) WAVES_ATTR
 k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
 {
     constexpr int NX = TX + 2, NY = TY + 2, NCOL = NX * NY;
     constexpr int NT = (NCOL + 63) / 64 * 64;
+    using IO = BufIO<DT>;
+    constexpr uint32_t E = IO::size;
     __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
     static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) <= sizeof(sp), "near-body work fits the LDS");
     const int L = (int)blockIdx.x;
@@ -344,7 +396,6 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
 
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int64_t plane = (int64_t)nx * ny;
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin in
     // dispatch order, so dispatch slot L runs on XCD L % 8. Give XCD k the
     // contiguous run of tiles [k * per, (k + 1) * per) (x fastest, then y,
@@ -353,29 +404,62 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     const int T = L < per * 8 ? (L % 8) * per + L / 8 : L;
     const int x0 = (T % gx) * TX, y0 = ((T / gx) % gy) * TY;
     const int zb = 1 + (T / (gx * gy)) * zc;
-    const int ze = min(zb + zc, nz - 1);  // destination planes [zb, ze)
+    const int ze = min(zb + zc, nz - 1);  // destination planes [zb, ze), at most 64
 
-    // source column of this thread (tile + halo)
+    // byte offsets: per field and per plane (scalar), per column (lane)
+    const uint32_t fbytes = (uint32_t)a.stride * E, pbytes = (uint32_t)(nx * ny) * E;
+    const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(a.src, 10 * fbytes);
+    const __amdgpu_buffer_rsrc_t rdst = buffer_rsrc(a.dst, 10 * fbytes);
+    const __amdgpu_buffer_rsrc_t rfor = buffer_rsrc(a.force, FORCE ? 3 * (uint32_t)a.stride * 4 : 0);
+
+    // source column of this thread (tile + halo); lanes past the halo keep
+    // an LDS slot of their own out of the way
     const int sgx = x0 - 1 + tid % NX, sgy = y0 - 1 + tid / NX;
-    const bool src_ok = tid < NCOL && sgx >= 0 && sgx < nx && sgy >= 0 && sgy < ny;
-    const int64_t scol = (int64_t)sgy * nx + sgx;
+    const bool src_lds = tid < NCOL;
+    const bool src_ok = src_lds && sgx >= 0 && sgx < nx && sgy >= 0 && sgy < ny;
+    const uint32_t vsrc = src_ok ? (uint32_t)(sgy * nx + sgx) * E : kOOB;
 
     // destination cell of this thread
     const int dx = x0 + tid % TX, dy = y0 + tid / TX;
     const bool dst_ok = tid < TX * TY && dx < nx && dy < ny;
     const int px = min(max(dx, 1), nx - 2), py = min(max(dy, 1), ny - 2);
-    const int cp = (py - y0 + 1) * NX + (px - x0 + 1);
-    const int64_t pcol = (int64_t)py * nx + px;
+    const int cp = dst_ok ? (py - y0 + 1) * NX + (px - x0 + 1) : 0;
+    const uint32_t vdst = dst_ok ? (uint32_t)(dy * nx + dx) * E : kOOB;
+    const uint32_t vfor = dst_ok ? (uint32_t)(py * nx + px) * 4 : kOOB;
 
     const float omega = 1.0f / (a.nu * 3.0f + 0.5f);
-    // does this tile touch the fix-up region at all (block-uniform)?
-    const bool near = a.cut_bits != nullptr && x0 + TX > a.cut_origin[0] && x0 < a.cut_origin[0] + a.cut_extent[0] &&
-                      y0 + TY > a.cut_origin[1] && y0 < a.cut_origin[1] + a.cut_extent[1];
+    // cells of this column the near-body pass owns, bit k for plane zb + k
+    // (from the per-column cut bitmap; only tiles that meet its region)
+    uint64_t own = 0;
+    if (a.cut_cols != nullptr && x0 + TX > a.cut_origin[0] && x0 < a.cut_origin[0] + a.cut_extent[0] &&
+        y0 + TY > a.cut_origin[1] && y0 < a.cut_origin[1] + a.cut_extent[1]) {
+        const int rx = dx - a.cut_origin[0], ry = dy - a.cut_origin[1];
+        if (dst_ok && rx >= 0 && rx < a.cut_extent[0] && ry >= 0 && ry < a.cut_extent[1])
+            own = column_window(a.cut_cols + (int64_t)(ry * a.cut_extent[0] + rx) * a.cut_colwords, a.cut_colwords,
+                                zb - a.cut_origin[2]);
+    }
+
+    auto load_plane = [&](int z, float* m) {
+        const uint32_t sz = (uint32_t)z * pbytes;
+#pragma unroll
+        for (int f = 0; f < 10; ++f) m[f] = IO::load(rsrc, vsrc, sz + f * fbytes, f, a.inv_scale[f]);
+    };
+    auto load_force = [&](int z, float* F) {
+        const uint32_t sz = (uint32_t)z * (uint32_t)(nx * ny) * 4;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) F[q] = BufIO<HL_F32>::load(rfor, vfor, sz + q * (uint32_t)a.stride * 4, 0, 1.0f);
+    };
+    auto store_plane = [&](int z, uint32_t v, const float* m) {
+        const uint32_t sz = (uint32_t)z * pbytes;
+#pragma unroll
+        for (int f = 0; f < 10; ++f) IO::store(rdst, v, sz + f * fbytes, m[f], f, a.scale[f]);
+    };
 
     // up pops of the last two source planes, alternating by plane parity so
     // that no hold has to be copied into another
     float hold_mid[9], up_a[9], up_b[9];
     float pfa[10], pfb[10];  // prefetched source moments, alternating by plane
+    float fa[3] = {0.0f, 0.0f, 0.0f}, fb[3] = {0.0f, 0.0f, 0.0f};  // body force of the next dest plane
 
     // Source work for plane k: rebuild its 27 populations, write to buf the
     // down pops of plane k, the held mid pops of plane k-1 and the held up
@@ -397,74 +481,95 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         }
     };
 
-    // Pull, collide and store the cells of plane z from buf.
-    auto dest_plane = [&](float(*buf)[NCOL], int z) {
-        const int64_t pc = (int64_t)z * plane + pcol;
-        float g[27], mo[10];
+    // Pull, collide and apply the x and y faces for the cell of plane z
+    // (lanes without a destination cell leave out untouched).
+    auto dest_plane = [&](float(*buf)[NCOL], const float* F, float* out) {
+        if (dst_ok) {
+            float g[27], mo[10];
 #pragma unroll
-        for (int i = 0; i < 27; ++i) g[i] = buf[i][cp - D3CX[i] - D3CY[i] * NX];
-        moments27(g, mo);
-        float F[3] = {0.0f, 0.0f, 0.0f};
-        if (FORCE) { F[0] = a.force[pc]; F[1] = a.force[a.stride + pc]; F[2] = a.force[2 * a.stride + pc]; }
-        float out[10];
-        collide27(mo, F, omega, out);
-        // faces: x, then y (this plane), then z (planes 0 and nz-1 from planes 1 and nz-2)
-        if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
-        else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
-        if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
-        else if (dy == 0) bc_face_closed(out, a.bc_type[3], a.bc_value[3], 1);
-        // cells the near-body fix-up owns are left to it (it runs concurrently)
-        if (!(near && fixup_owns(a, dx, dy, z))) store_moments<DT>(a, (int64_t)z * plane + (int64_t)dy * nx + dx, out);
-        if (z == 1 && !(near && fixup_owns(a, dx, dy, 0))) {
+            for (int i = 0; i < 27; ++i) g[i] = buf[i][cp - D3CX[i] - D3CY[i] * NX];
+            moments27(g, mo);
+            collide27(mo, F, omega, out);
+            if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
+            else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
+            if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
+            else if (dy == 0) bc_face_closed(out, a.bc_type[3], a.bc_value[3], 1);
+        }
+    };
+
+    // One plane step: source plane z+2 from `use` into the other buffer,
+    // prefetch plane z+3 into `into` and the next plane's body force into
+    // fin, pull/collide/store plane z; one barrier. EDGE: the first or last
+    // steps of the chunk (source/prefetch limits, the z faces); the steady
+    // steps issue the same memory operations on every path.
+    auto step = [&](int z, const float* use, float* into, float* U, const float* fuse, float* fin, bool edge) {
+        const int cur = (z - zb) & 1;
+        if (!edge || z + 1 < ze) {
+            // the set of plane z+3 was consumed a step ago: refill it before
+            // the source work, so the loads have a whole step to land
+            if (!edge || z + 3 <= ze) load_plane(z + 3, into);
+            if (FORCE && (!edge || z + 1 < ze)) load_force(z + 1, fin);
+            if (src_lds) source_plane(sp[cur ^ 1], use, U);
+        }
+        float out[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        dest_plane(sp[cur], fuse, out);
+        // cells the near-body pass owns are left to it
+        store_plane(z, (own >> (z - zb)) & 1ull ? kOOB : vdst, out);
+        if (edge && z == 1) {  // face z = 0 from the cell above it
             float face[10];
 #pragma unroll
             for (int k = 0; k < 10; ++k) face[k] = out[k];
             bc_face_closed(face, a.bc_type[5], a.bc_value[5], 2);
-            store_moments<DT>(a, (int64_t)dy * nx + dx, face);
+            store_plane(0, dst_ok && !fixup_owns(a, dx, dy, 0) ? vdst : kOOB, face);
         }
-        if (z == nz - 2 && !(near && fixup_owns(a, dx, dy, nz - 1))) {
+        if (edge && z == nz - 2) {  // face z = nz-1 from the cell below it
             bc_face_closed(out, a.bc_type[4], a.bc_value[4], 2);
-            store_moments<DT>(a, (int64_t)(nz - 1) * plane + (int64_t)dy * nx + dx, out);
+            store_plane(nz - 1, dst_ok && !fixup_owns(a, dx, dy, nz - 1) ? vdst : kOOB, out);
         }
+        __syncthreads();
     };
 
     // prologue: planes zb-1 (up pops), zb (mid/up pops), zb+1 (into buffer 0)
-    if (src_ok) {
-        float m[10], f[27];
-        load_moments<DT>(a, (int64_t)(zb - 1) * plane + scol, m);
-        recon27(m, f);
+    {
+        float m0[10], m1[10], m2[10];
+        load_plane(zb - 1, m0);
+        load_plane(zb, m1);
+        load_plane(zb + 1, m2);
+        if (zb + 2 <= ze) load_plane(zb + 2, pfa);
+        if (FORCE) load_force(zb, fa);
+        if (src_lds) {
+            float f[27];
+            recon27(m0, f);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) up_a[k] = f[kUp[k]];
-        load_moments<DT>(a, (int64_t)zb * plane + scol, m);
-        recon27(m, f);
+            for (int k = 0; k < 9; ++k) up_a[k] = f[kUp[k]];
+            recon27(m1, f);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; up_b[k] = f[kUp[k]]; }
-        load_moments<DT>(a, (int64_t)(zb + 1) * plane + scol, m);
-        if (zb + 2 <= ze) load_moments<DT>(a, (int64_t)(zb + 2) * plane + scol, pfa);
-        source_plane(sp[0], m, up_a);  // plane zb+1: up(zb-1) out, up(zb+1) in
+            for (int k = 0; k < 9; ++k) { hold_mid[k] = f[kMid[k]]; up_b[k] = f[kUp[k]]; }
+            source_plane(sp[0], m2, up_a);  // plane zb+1: up(zb-1) out, up(zb+1) in
+        }
     }
     __syncthreads();
 
     // Software pipeline: step z pulls plane z from buffer (z-zb)&1 while
     // rebuilding the sources of plane z+2 (prefetched one step earlier) into
     // the other buffer, then prefetches plane z+3; one barrier per plane. The
-    // loop is unrolled by two so each prefetch lands in the registers it is
-    // consumed from (no loop-carried copies that would wait on the loads).
-    // step z rebuilds source plane z+2, whose up-pop hold is up_b when
-    // z - zb is even and up_a when odd (static in the unrolled loop)
-    auto step = [&](int z, const float* use, float* into, float* U) {
-        const int cur = (z - zb) & 1;
-        if (src_ok && z + 1 < ze) {
-            source_plane(sp[cur ^ 1], use, U);
-            // issued after `use` is consumed, so waits on `use` never cover these
-            if (z + 3 <= ze) load_moments<DT>(a, (int64_t)(z + 3) * plane + scol, into);
+    // steady loop is unrolled by two so each prefetch lands in the registers
+    // it is consumed from (no loop-carried copies that would wait on the
+    // loads); step z rebuilds source plane z+2, whose up-pop hold is up_b
+    // when z - zb is even and up_a when odd. The first step (face z = 0) and
+    // the last three (fewer sources, face z = nz-1) are peeled.
+    step(zb, pfa, pfb, up_b, fa, fb, true);
+    int z = zb + 1;
+    for (; z + 4 <= ze; z += 2) {
+        step(z, pfb, pfa, up_a, fb, fa, false);
+        step(z + 1, pfa, pfb, up_b, fa, fb, false);
+    }
+    if (z < ze) {
+        step(z, pfb, pfa, up_a, fb, fa, true);
+        if (z + 1 < ze) {
+            step(z + 1, pfa, pfb, up_b, fa, fb, true);
+            if (z + 2 < ze) step(z + 2, pfb, pfa, up_a, fb, fa, true);
         }
-        if (dst_ok) dest_plane(sp[cur], z);
-        __syncthreads();
-    };
-    for (int z = zb; z < ze; z += 2) {
-        step(z, pfa, pfb, up_b);
-        if (z + 1 < ze) step(z + 1, pfb, pfa, up_a);
     }
 }
 
@@ -473,7 +578,7 @@ static TileCfg g_tile3d{64, 8, 16};
 
 bool tiling_supported(int lattice, int tx, int ty, int zc)
 {
-    if (lattice != HL_D3Q27 || zc < 1 || zc > 4096) return false;
+    if (lattice != HL_D3Q27 || zc < 1 || zc > 64) return false;
     return (tx == 32 && ty == 16) || (tx == 32 && ty == 8) || (tx == 64 && ty == 8) || (tx == 64 && ty == 4) ||
            (tx == 16 && ty == 16);
 }
@@ -494,7 +599,9 @@ hipError_t launch_tiled(const EnvArgs& a, int zc, FixArgs* fxp, hipStream_t s)
 {
     // every tile must hold at least 3 columns per axis for the face projection
     if ((a.nx % TX) == 1 || (a.nx % TX) == 2 || (a.ny % TY) == 1 || (a.ny % TY) == 2) return hipErrorNotSupported;
-    if (zc < 1) return hipErrorInvalidValue;
+    if (zc < 1 || zc > 64) return hipErrorInvalidValue;
+    // buffer offsets are 32-bit with the out-of-range marker at 2^31
+    if ((uint64_t)a.stride * 10 * BufIO<DT>::size >= (1ull << 31)) return hipErrorNotSupported;
     constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
     const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY, gz = (a.nz - 2 + zc - 1) / zc;
     const int ntile = gx * gy * gz;
