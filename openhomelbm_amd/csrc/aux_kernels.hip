@@ -495,7 +495,7 @@ hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* o
     return hipGetLastError();
 }
 
-FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces)
+FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces, int parity)
 {
     FixArgs f;
     std::memset(&f, 0, sizeof(f));
@@ -507,8 +507,11 @@ FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces)
     f.list = reinterpret_cast<const int*>(base + L.list);
     f.count = reinterpret_cast<const int*>(base + L.total);
     f.width = a.nsolid * 6;
-    if (forces && f.width > 0) f.partial = reinterpret_cast<double*>(base + L.partial);
-    f.max_waves = kFixupWaves;  // partial rows available
+    // two banks of rows, by step parity, so that a launch can sum the
+    // previous step's rows while it writes its own
+    f.max_waves = kFixupWaves / 2;
+    if (forces && f.width > 0)
+        f.partial = reinterpret_cast<double*>(base + L.partial) + (size_t)(parity & 1) * f.max_waves * f.width;
     return f;
 }
 

@@ -227,6 +227,18 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
         hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
         if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
     }
+    // force rows a tiled launch left for the next launch to sum
+    struct Pending {
+        const double* rows = nullptr;
+        int nrows = 0, width = 0;
+        double* out = nullptr;
+    } pend;
+    auto flush = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (pend.rows) e = hl::sum_partials(pend.rows, pend.nrows, pend.width, pend.out, s);
+        pend = Pending();
+        return e;
+    };
     for (int k = 0; k < nsteps; ++k) {
         hl::EnvArgs a = make_args(env, cur);
         double* fout = want_forces ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
@@ -234,31 +246,49 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
         bool fixed = false;
         if (kernel == HL_KERNEL_AUTO || kernel == HL_KERNEL_TILED) {
             // with a body mask, the same launch also updates the near-body
-            // cells (wall rule, reference order) and reduces the forces
+            // cells (wall rule, reference order), writes this step's force
+            // rows and sums the previous step's
             hl::FixArgs fx;
             std::memset(&fx, 0, sizeof(fx));
             hl::FixArgs* pfx = nullptr;
             if (ws) {
                 hl::cut_bits_view(env->lattice, a, ws);
-                fx = hl::fixup_args(env->lattice, a, ws, fout != nullptr);
-                if (fx.list) pfx = &fx;
+                fx = hl::fixup_args(env->lattice, a, ws, fout != nullptr, k);
+                if (fx.list) {
+                    pfx = &fx;
+                    fx.prev = pend.rows;
+                    fx.prev_rows = pend.nrows;
+                    fx.prev_out = pend.out;
+                }
             }
             e = hl::step_tiled(env->lattice, env->dtype, a, pfx, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
                 return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
             if (e == hipSuccess) {
                 fixed = pfx != nullptr || fout == nullptr;
+                if (pfx) pend = Pending();  // summed by this launch
                 // the launch's near-body workgroups left one force row each
-                if (pfx && fout) e = hl::sum_partials(fx.partial, fx.nfix, fx.width, fout, s);
+                if (pfx && fout) {
+                    pend.rows = fx.partial;
+                    pend.nrows = fx.nfix;
+                    pend.width = fx.width;
+                    pend.out = fout;
+                }
             } else if (e == hipErrorNotSupported) {
                 a.cut_bits = nullptr;  // the gather kernel does every cell
+                a.cut_cols = nullptr;
             }
         }
-        if (e == hipErrorNotSupported) e = hl::step_gather(env->lattice, env->dtype, a, s);
+        if (e == hipErrorNotSupported) {
+            e = flush();
+            if (e == hipSuccess) e = hl::step_gather(env->lattice, env->dtype, a, s);
+        }
         if (e == hipSuccess && !fixed && fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
         if (e != hipSuccess) return hip_fail(e, "step kernel launch");
         cur ^= 1;
     }
+    hipError_t e = flush();
+    if (e != hipSuccess) return hip_fail(e, "force sum launch");
     return HL_OK;
 }
 

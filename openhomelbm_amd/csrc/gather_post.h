@@ -124,6 +124,11 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
     const int64_t st = a.stride;
     const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]), wi = d3w<T>(i);
     const int64_t s = (int64_t)(x - D3CX[i]) + (int64_t)a.nx * ((int64_t)(y - D3CY[i]) + (int64_t)a.ny * (z - D3CZ[i]));
+    // the source moments are loaded in the same round trip as the mask test
+    // (the source cell is inside the lattice either way)
+    T m[10];
+#pragma unroll
+    for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
     int cut = 0, sid = -1;
     if (masked) {
         const int mval_s = a.mask[s];
@@ -141,9 +146,6 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
         T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
         pop_i = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, wi);
     } else {
-        T m[10];
-#pragma unroll
-        for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
         pop_i = d3q27_recon<T>(m[0], m[1], m[2], m[3], m[4], m[7], m[8], m[5], m[9], m[6], dx, dy, dz, wi);
         if (pop_i < T(0)) pop_i = T(0);
     }

@@ -32,6 +32,11 @@ struct FixArgs {
     int width;         // nsolid * 6
     int max_waves;     // rows available in partial
     int nfix;          // workgroups sharing the near-body work (set by the launcher)
+    // the previous step's force rows, summed by this launch (one wave per
+    // output, the order of sum_partials): out[k] = sum_r prev[r * width + k]
+    const double* prev;
+    int prev_rows;
+    double* prev_out;
 };
 bool tiling_supported(int lattice, int tx, int ty, int zc);
 void set_tiling(int lattice, TileCfg c);
@@ -45,7 +50,7 @@ hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
 hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, FixArgs* fx, hipStream_t s);
 // Near-body arguments for one step over ws (null list when there is no mask;
 // partial rows only when forces are wanted).
-FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces);
+FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces, int parity = 0);
 // out[k] = sum over rows r of partial[r * width + k], in a fixed order.
 hipError_t sum_partials(const double* partial, int rows, int width, double* out, hipStream_t s);
 

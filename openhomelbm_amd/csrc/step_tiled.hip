@@ -376,6 +376,23 @@ __device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int
     }
 }
 
+// The previous step's force rows: output k is summed by wave k % W of
+// workgroup (k / W) % grid, its lanes taking rows lane, lane + 64, ... in order and a
+// butterfly combining them (the order of sum_partials, so the sums agree
+// bitwise).
+template <int NT>
+__device__ void sum_previous(const FixArgs& fx, int blk)
+{
+    constexpr int W = NT / 64;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int k = blk * W + wave; k < fx.width; k += (int)gridDim.x * W) {
+        double v = 0.0;
+        for (int r = lane; r < fx.prev_rows; r += 64) v += fx.prev[(int64_t)r * fx.width + k];
+        v = wave_sum(v);
+        if (lane == 0) fx.prev_out[k] = v;
+    }
+}
+
 template <int DT, int TX, int TY, bool FORCE>
 __global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) WAVES_ATTR
 k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
@@ -387,13 +404,6 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
     static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) <= sizeof(sp), "near-body work fits the LDS");
     const int L = (int)blockIdx.x;
-    // the near-body cells of this step, spread over the first fx.nfix
-    // workgroups ahead of their tiled pass (tiled cells skip them)
-    if (fx.list) {
-        if (L < fx.nfix) fixup_block<DT, NT>(a, fx, &sp[0][0][0], L);
-        __syncthreads();  // the LDS now holds populations
-    }
-
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin in
@@ -537,6 +547,14 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         load_plane(zb + 1, m2);
         if (zb + 2 <= ze) load_plane(zb + 2, pfa);
         if (FORCE) load_force(zb, fa);
+        // the near-body cells of this step, spread over the first fx.nfix
+        // workgroups ahead of their tiled pass (tiled cells skip them); the
+        // loads above land meanwhile
+        if (fx.list) {
+            if (L < fx.nfix) fixup_block<DT, NT>(a, fx, &sp[0][0][0], L);
+            if (fx.prev) sum_previous<NT>(fx, L);
+            __syncthreads();  // the LDS is free again
+        }
         if (src_lds) {
             float f[27];
             recon27(m0, f);

@@ -173,3 +173,35 @@ def test_full_size_sphere_tiled_vs_gather():
     a.step(20, kernel=N.HL_KERNEL_TILED)
     m1 = a.mass()
     assert np.isfinite(m1) and abs(m1 - m0) / m0 < 1e-3
+
+
+def test_force_history_one_call_equals_step_by_step():
+    """Within one multi-step call each launch sums the previous step's force
+    rows (the last step's in a closing launch); one step per call sums its
+    own rows in a separate launch. Same rows, same order: the histories and
+    states agree bitwise, and both match the near-body pass run alone on the
+    same states."""
+    env, caps, poses, force = _tiled_scene(11)
+    m0 = random_state(27, SHAPE, seed=45).astype(np.float32)
+    n = 5
+    hists, states = [], []
+    for per_call in (False, True):
+        lat = _gpu_lattice(env, caps, poses, force)
+        lat.set_state(m0)
+        h = torch.zeros((n, lat.nsolid, 6), dtype=torch.float64, device=lat.device)
+        if per_call:
+            for k in range(n):
+                lat.step(1, kernel=N.HL_KERNEL_TILED, force_history=h[k:k + 1])
+        else:
+            lat.step(n, kernel=N.HL_KERNEL_TILED, force_history=h)
+        hists.append(h.cpu().numpy())
+        states.append(lat.get_state(np.float32))
+    assert np.array_equal(hists[0], hists[1])
+    assert np.array_equal(states[0], states[1])
+    assert np.isfinite(hists[0]).all() and np.abs(hists[0]).max() > 0
+    # the last step's forces, recomputed from the state it read
+    lat = _gpu_lattice(env, caps, poses, force)
+    lat.set_state(m0)
+    lat.step(n - 1, kernel=N.HL_KERNEL_TILED)
+    f = lat.forces().cpu().numpy()
+    np.testing.assert_allclose(f, hists[0][n - 1], rtol=1e-10, atol=1e-13)
