@@ -123,6 +123,35 @@ int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* vert
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Render and diagnostic images of state `cur` into device FP32 out (the
+ * reference's u_img / u_img_rgb / u_img_xy / u_img_xz / u_img_xz_front
+ * buffers, here with x (or the first in-plane axis) fastest). Vorticity uses
+ * central differences inside and one-sided ones at the faces over the grid
+ * length 1/nx; solid cells read 1000 in vorticity images. Bodies are the
+ * voxel mask: solid where masked or (thickness t, 0 <= t <= 4) within
+ * distance t of a masked cell; a projected ray is solid if it meets the mask.
+ */
+#define HL_RENDER_SPEED 0            /* get_u_img (lbm_func.py:163-166), get_u_img_xy/yz_3d (lbm_func_3d.py:753-776) */
+#define HL_RENDER_VORTICITY 1        /* get_vorticity_img (lbm_func.py:196-233), get_vorticity_xy/yz_3d (lbm_func_3d.py:779-840) */
+#define HL_RENDER_VORTICITY_SOLID 2  /* get_vorticity_with_solid_img (lbm_func.py:236-319), ..._with_solid_3d (lbm_func_3d.py:843-988) */
+#define HL_RENDER_RGB 3              /* get_velocity_rgb (lbm_func.py:169-193): out[3 * pixel + channel] */
+#define HL_RENDER_SOLID 4            /* get_solid_boundary_img (lbm_func.py:804-864): 1 solid, 0 fluid */
+#define HL_PROJECT_FIRST_SPEED 0     /* get_u_projection_topdown_3d (lbm_func_3d.py:990-1030) */
+#define HL_PROJECT_MAX_SPEED 1       /* get_u_projection_{max_topdown,side,front}_3d (lbm_func_3d.py:1033-1201) */
+#define HL_PROJECT_MAX_VORTICITY 2   /* get_vorticity_projection_{topdown,side,front}_3d (lbm_func_3d.py:1072-1313) */
+
+/* One plane: normal 2 = the xy plane at z = index (a D2Q9 lattice: index 0),
+ * out[y * nx + x]; normal 0 (D3Q27) = the yz plane at x = index,
+ * out[z * ny + y]. max_velocity scales HL_RENDER_RGB (> 0). */
+int32_t hl_render_plane(const hl_env* env, int32_t cur, int32_t kind, int32_t normal, int32_t index,
+                        float thickness, float max_velocity, float* out, void* stream);
+/* One D3Q27 projection along axis (0: x, out[z * ny + y]; 1: y,
+ * out[z * nx + x]; 2: z, out[y * nx + x]), face cells skipped; the first
+ * speed is taken from the high-index end, vorticity is the component about
+ * the axis (signed value of the largest magnitude over the interior). */
+int32_t hl_render_projection(const hl_env* env, int32_t cur, int32_t kind, int32_t axis, float* out, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batches: many resident D2Q9 FP32 lattices, of uneven sizes, advanced
  * together with one launch per kernel kind per step (work-lists of
  * (environment, tile) items). Replaces the nworld dimension of LBM_Solver

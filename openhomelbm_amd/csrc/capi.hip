@@ -394,4 +394,44 @@ int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
     return HL_OK;
 }
 
+int32_t hl_render_plane(const hl_env* env, int32_t cur, int32_t kind, int32_t normal, int32_t index,
+                        float thickness, float max_velocity, float* out, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (!out) return fail(HL_E_INVALID, "out is NULL");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (kind < HL_RENDER_SPEED || kind > HL_RENDER_SOLID) return fail(HL_E_INVALID, "unknown render kind %d", kind);
+    const bool three = env->lattice == HL_D3Q27;
+    if (normal != 2 && !(three && normal == 0)) return fail(HL_E_INVALID, "plane normal %d not supported", normal);
+    const int n = normal == 2 ? env->nz : env->nx;
+    if (index < 0 || index >= (three ? n : 1)) return fail(HL_E_INVALID, "plane index %d out of range", index);
+    if (!(thickness >= 0.0f) || thickness > (float)hl::max_render_thickness())
+        return fail(HL_E_INVALID, "thickness must be in [0, %d]", hl::max_render_thickness());
+    if (kind == HL_RENDER_RGB && !(max_velocity > 0.0f)) return fail(HL_E_INVALID, "max_velocity must be > 0");
+    const hl::EnvArgs a = make_args(env, cur);
+    const float gl = (float)(1.0 / env->nx);
+    hipError_t e = hl::render_plane(env->dtype, a, kind, normal, index, gl, thickness, max_velocity, out,
+                                    static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "render kernel launch");
+    return HL_OK;
+}
+
+int32_t hl_render_projection(const hl_env* env, int32_t cur, int32_t kind, int32_t axis, float* out, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (!out) return fail(HL_E_INVALID, "out is NULL");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (env->lattice != HL_D3Q27) return fail(HL_E_UNSUPPORTED, "projections are for D3Q27 lattices");
+    if (kind < HL_PROJECT_FIRST_SPEED || kind > HL_PROJECT_MAX_VORTICITY)
+        return fail(HL_E_INVALID, "unknown projection kind %d", kind);
+    if (axis < 0 || axis > 2) return fail(HL_E_INVALID, "axis must be 0, 1 or 2");
+    const hl::EnvArgs a = make_args(env, cur);
+    const float gl = (float)(1.0 / env->nx);
+    hipError_t e = hl::render_projection(env->dtype, a, kind, axis, gl, out, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "projection kernel launch");
+    return HL_OK;
+}
+
 }  // extern "C"

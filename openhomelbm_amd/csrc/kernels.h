@@ -82,4 +82,11 @@ int32_t check_env(const hl_env* e);
 EnvArgs env_args(const hl_env* e, int cur);
 hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws, hipStream_t s);
 
+// Render images (render.hip): one plane (normal 2: xy at z = index, normal 0:
+// yz at x = index) or one projection along an axis; gl is the grid length.
+hipError_t render_plane(int dtype, const EnvArgs& a, int kind, int normal, int index, float gl, float thickness,
+                        float vmax, float* out, hipStream_t s);
+hipError_t render_projection(int dtype, const EnvArgs& a, int kind, int axis, float gl, float* out, hipStream_t s);
+int max_render_thickness();
+
 }  // namespace hl

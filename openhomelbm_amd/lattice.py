@@ -323,6 +323,30 @@ class Lattice:
         N.check(self.lib.hl_forces(C.byref(e), self.cur, self._forces_out.data_ptr(), _stream_ptr(stream)), "hl_forces")
         return self._forces_out[: self.nsolid]
 
+    # ------------------------------------------------------------ images
+    def render_plane(self, kind: int, normal: int = 2, index: int = 0, thickness: float = 0.0,
+                     max_velocity: float = 1.0, stream=None) -> torch.Tensor:
+        """Device FP32 image of one plane of the present state (N.HL_RENDER_*):
+        [ny, nx] for the xy plane (normal 2; a D2Q9 lattice), [nz, ny] for the
+        yz plane at x = index (normal 0); RGB adds a trailing axis of 3."""
+        h, w = (self.ny, self.nx) if normal == 2 else (self.nz, self.ny)
+        shape = (h, w, 3) if kind == N.HL_RENDER_RGB else (h, w)
+        out = torch.empty(shape, dtype=torch.float32, device=self.device)
+        e = self.env()
+        N.check(self.lib.hl_render_plane(C.byref(e), self.cur, int(kind), int(normal), int(index), float(thickness),
+                                         float(max_velocity), out.data_ptr(), _stream_ptr(stream)), "hl_render_plane")
+        return out
+
+    def render_projection(self, kind: int, axis: int, stream=None) -> torch.Tensor:
+        """Device FP32 projection of a D3Q27 state along axis (N.HL_PROJECT_*):
+        [nz, ny] along x, [nz, nx] along y, [ny, nx] along z."""
+        shape = {0: (self.nz, self.ny), 1: (self.nz, self.nx), 2: (self.ny, self.nx)}.get(int(axis), (1, 1))
+        out = torch.empty(shape, dtype=torch.float32, device=self.device)
+        e = self.env()
+        N.check(self.lib.hl_render_projection(C.byref(e), self.cur, int(kind), int(axis), out.data_ptr(),
+                                              _stream_ptr(stream)), "hl_render_projection")
+        return out
+
     def mass(self, stream=None) -> float:
         e = self.env()
         N.check(self.lib.hl_mass(C.byref(e), self.cur, self._mass_out.data_ptr(), _stream_ptr(stream)), "hl_mass")

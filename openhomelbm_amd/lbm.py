@@ -48,6 +48,7 @@ class HomeFlow:
         self._dtype = dtype
         self._device = device
         self._lat: Optional[Lattice] = None
+        self._images: dict = {}  # render buffers written by openhomelbm_amd.render
 
     # ------------------------------------------------------------ setup
     def Initialize(self, nx: int, ny: int, n_objects: int = 1) -> None:
@@ -122,9 +123,22 @@ class HomeFlow:
 
     @property
     def u_img(self) -> HostArray:
-        """Velocity magnitude of the present state (the render buffer's default content)."""
-        u = self.u.torch()
-        return HostArray(torch.sqrt((u * u).sum(-1)).cpu().numpy())
+        """Render buffer [nx, ny]: the last image a render kernel wrote
+        (openhomelbm_amd.render), else the velocity magnitude of the present state."""
+        img = self._images.get("u_img")
+        if img is None:
+            from .render import get_u_img
+            get_u_img(self)
+            img = self._images["u_img"]
+        return HostArray(img.cpu().numpy())
+
+    @property
+    def u_img_rgb(self) -> HostArray:
+        """RGB render buffer [nx, ny, 3] written by get_velocity_rgb (zeros before)."""
+        img = self._images.get("u_img_rgb")
+        if img is None:
+            return HostArray(np.zeros((self.nx, self.ny, 3), np.float32))
+        return HostArray(img.cpu().numpy())
 
     @property
     def vis_shear(self) -> float:

@@ -54,6 +54,7 @@ class HomeFlow3D:
         self._dtype = dtype
         self._device = device
         self._lat: Optional[Lattice] = None
+        self._images: dict = {}  # render buffers written by openhomelbm_amd.render
 
     def Initialize(self, nx: int, ny: int, nz: int, n_objects: int = 1) -> None:
         """Allocate the world: rho = 1, u = 0, S = 0 (InitFlow3D,
@@ -121,6 +122,18 @@ class HomeFlow3D:
     forcex = property(lambda s: ForceField(s.lattice, 0))
     forcey = property(lambda s: ForceField(s.lattice, 1))
     forcez = property(lambda s: ForceField(s.lattice, 2))
+
+    def _image(self, name: str, shape) -> HostArray:
+        img = self._images.get(name)
+        if img is None:
+            return HostArray(np.zeros(shape, np.float32))
+        return HostArray(img.cpu().numpy())
+
+    # render buffers written by openhomelbm_amd.render (zeros before), in the
+    # reference's index order (lbm_core_3d.py:149-222)
+    u_img_xy = property(lambda s: s._image("u_img_xy", (s.nx, s.ny)))
+    u_img_xz = property(lambda s: s._image("u_img_xz", (s.ny, s.nz)))
+    u_img_xz_front = property(lambda s: s._image("u_img_xz_front", (s.nx, s.nz)))
 
     @property
     def flag(self) -> HostArray:
