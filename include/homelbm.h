@@ -152,6 +152,60 @@ int32_t hl_render_plane(const hl_env* env, int32_t cur, int32_t kind, int32_t no
 int32_t hl_render_projection(const hl_env* env, int32_t cur, int32_t kind, int32_t axis, float* out, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Coupling with a rigid-body engine, device buffers throughout (no host
+ * round trip between the engine's step and the lattice's). Entries whose
+ * solid or body index is out of range are skipped.
+ */
+#define HL_COUPLE_SCALED 0    /* F / scale, torque / scale^2 (2D torque: / scale) */
+#define HL_COUPLE_PHYSICAL 1  /* F * force_conv, torque * torque_conv */
+
+/* Engine body poses of nworld worlds -> each world's pose table (tables: a
+ * device array of nworld device pose-table pointers, nsolid rows each). Body
+ * b of world w is at pos + pos_stride * (w * nbody + b) and quat (w, x, y,
+ * z) likewise; its table row solid_ids[b] gets (pos - engine_origin[b]) *
+ * scale[b] + lattice_origin[b] and the rotation (2D: the planar angle's cos
+ * and sin), after the row's current pose moved to its "last" half (a row not
+ * yet initialized gets last = current; initialized [nworld][nsolid] is set).
+ * Replaces convert_and_update_solid_batch_2d (envs/lbm/lbm_fluid_env_func.py:72-105)
+ * and convert_and_update_solid_batch_3d (envs/lbm3d/lbm_fluid_env_3d_func.py:42-99). */
+int32_t hl_couple_poses(int32_t lattice, int32_t nworld, int32_t nbody, int32_t nsolid, const float* pos,
+                        int32_t pos_stride, const float* quat, int32_t quat_stride, const int32_t* solid_ids,
+                        const float* engine_origin, const float* lattice_origin, const float* scale,
+                        float* const* tables, int32_t* initialized, void* stream);
+
+/* FP64 force rows (world w, solid k at rows + w * world_stride + 6 k: F(3),
+ * torque(3)) -> engine loads forces / torques [nworld][nbody][3] in `mode`
+ * (2D: planar, Fz = 0 and torque about z only), and when xfrc is not NULL
+ * also the spatial forces xfrc [nworld][xfrc_bodies][6] at body_ids[b].
+ * Replaces extract_forces_torques_batch (envs/lbm/lbm_fluid_env_func.py:51-69),
+ * extract_forces_torques_batch_3d / _physical_3d
+ * (envs/lbm3d/lbm_fluid_env_3d_func.py:102-185) and fill_xfrc_kernel /
+ * fill_xfrc_3d_kernel (lbm_fluid_env_func.py:28-48; lbm_fluid_env_3d_func.py:188-214). */
+int32_t hl_couple_forces(int32_t lattice, int32_t mode, int32_t nworld, int32_t nbody, int32_t nsolid,
+                         const double* rows, int64_t world_stride, const int32_t* solid_ids, const float* scale,
+                         float force_conv, float torque_conv, float* forces, float* torques, float* xfrc,
+                         int32_t xfrc_bodies, const int32_t* body_ids, void* stream);
+
+/* One semi-implicit Euler step of free rigid bodies (state [nworld][nbody][13]:
+ * position, quaternion (w, x, y, z), velocity, angular velocity in the world
+ * frame; params [nbody][4]: mass, principal inertia) under forces / torques
+ * [nworld][nbody][3] and gravity. No reference counterpart: it stands in for
+ * the MuJoCo-Warp step the reference's coupled environments take, which does
+ * not run here. */
+int32_t hl_free_bodies_step(int32_t nworld, int32_t nbody, float* state, const float* params, const float* forces,
+                            const float* torques, float gx, float gy, float gz, float dt, void* stream);
+
+/* Posed capsules for hl_voxelize: body-frame segments shapes [nbody][7]
+ * (a, b, radius in engine units) of every world's bodies -> caps
+ * [nworld][nbody][8] in lattice coordinates (the mapping of hl_couple_poses),
+ * solid id solid_ids[b]. No reference counterpart (the reference ray-casts
+ * the posed mesh, lbm_func_3d.py:254-316). */
+int32_t hl_body_capsules(int32_t nworld, int32_t nbody, const float* pos, int32_t pos_stride, const float* quat,
+                         int32_t quat_stride, const float* shapes, const int32_t* solid_ids,
+                         const float* engine_origin, const float* lattice_origin, const float* scale, float* caps,
+                         void* stream);
+
+/* ------------------------------------------------------------------------
  * Batches: many resident D2Q9 FP32 lattices, of uneven sizes, advanced
  * together with one launch per kernel kind per step (work-lists of
  * (environment, tile) items). Replaces the nworld dimension of LBM_Solver

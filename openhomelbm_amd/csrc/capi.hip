@@ -434,4 +434,70 @@ int32_t hl_render_projection(const hl_env* env, int32_t cur, int32_t kind, int32
     return HL_OK;
 }
 
+int32_t hl_couple_poses(int32_t lattice, int32_t nworld, int32_t nbody, int32_t nsolid, const float* pos,
+                        int32_t pos_stride, const float* quat, int32_t quat_stride, const int32_t* solid_ids,
+                        const float* engine_origin, const float* lattice_origin, const float* scale,
+                        float* const* tables, int32_t* initialized, void* stream)
+{
+    if (lattice != HL_D2Q9 && lattice != HL_D3Q27) return fail(HL_E_INVALID, "unknown lattice %d", lattice);
+    if (nworld < 1 || nbody < 1 || nbody > HL_MAX_SOLIDS || nsolid < 1 || nsolid > HL_MAX_SOLIDS)
+        return fail(HL_E_INVALID, "need nworld >= 1 and 1 <= nbody, nsolid <= %d", HL_MAX_SOLIDS);
+    if ((int64_t)nworld * nbody > (1 << 24)) return fail(HL_E_INVALID, "too many bodies");
+    if (pos_stride < 3 || quat_stride < 4) return fail(HL_E_INVALID, "pos_stride >= 3 and quat_stride >= 4");
+    if (!pos || !quat || !solid_ids || !engine_origin || !lattice_origin || !scale || !tables || !initialized)
+        return fail(HL_E_INVALID, "a buffer is NULL");
+    hipError_t e = hl::couple_poses(lattice, nworld, nbody, nsolid, pos, pos_stride, quat, quat_stride, solid_ids,
+                                    engine_origin, lattice_origin, scale, tables, initialized,
+                                    static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "pose kernel launch");
+}
+
+int32_t hl_couple_forces(int32_t lattice, int32_t mode, int32_t nworld, int32_t nbody, int32_t nsolid,
+                         const double* rows, int64_t world_stride, const int32_t* solid_ids, const float* scale,
+                         float force_conv, float torque_conv, float* forces, float* torques, float* xfrc,
+                         int32_t xfrc_bodies, const int32_t* body_ids, void* stream)
+{
+    if (lattice != HL_D2Q9 && lattice != HL_D3Q27) return fail(HL_E_INVALID, "unknown lattice %d", lattice);
+    if (mode != HL_COUPLE_SCALED && mode != HL_COUPLE_PHYSICAL) return fail(HL_E_INVALID, "unknown mode %d", mode);
+    if (nworld < 1 || nbody < 1 || nbody > HL_MAX_SOLIDS || nsolid < 1 || nsolid > HL_MAX_SOLIDS)
+        return fail(HL_E_INVALID, "need nworld >= 1 and 1 <= nbody, nsolid <= %d", HL_MAX_SOLIDS);
+    if ((int64_t)nworld * nbody > (1 << 24)) return fail(HL_E_INVALID, "too many bodies");
+    if (world_stride < 6 * (int64_t)nsolid) return fail(HL_E_INVALID, "world_stride < 6 * nsolid");
+    if (!rows || !solid_ids || !forces || !torques || (mode == HL_COUPLE_SCALED && !scale))
+        return fail(HL_E_INVALID, "a buffer is NULL");
+    if (xfrc && (!body_ids || xfrc_bodies < 1)) return fail(HL_E_INVALID, "xfrc needs body_ids and xfrc_bodies >= 1");
+    hipError_t e = hl::couple_forces(lattice, mode, nworld, nbody, nsolid, rows, world_stride, solid_ids, scale,
+                                     force_conv, torque_conv, forces, torques, xfrc, xfrc_bodies, body_ids,
+                                     static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "force kernel launch");
+}
+
+int32_t hl_free_bodies_step(int32_t nworld, int32_t nbody, float* state, const float* params, const float* forces,
+                            const float* torques, float gx, float gy, float gz, float dt, void* stream)
+{
+    if (nworld < 1 || nbody < 1 || (int64_t)nworld * nbody > (1 << 24))
+        return fail(HL_E_INVALID, "need nworld, nbody >= 1");
+    if (!state || !params || !forces || !torques) return fail(HL_E_INVALID, "a buffer is NULL");
+    if (!(dt > 0.0f)) return fail(HL_E_INVALID, "dt must be > 0");
+    const float g[3] = {gx, gy, gz};
+    hipError_t e = hl::free_bodies_step(nworld, nbody, state, params, forces, torques, g, dt,
+                                        static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "body kernel launch");
+}
+
+int32_t hl_body_capsules(int32_t nworld, int32_t nbody, const float* pos, int32_t pos_stride, const float* quat,
+                         int32_t quat_stride, const float* shapes, const int32_t* solid_ids,
+                         const float* engine_origin, const float* lattice_origin, const float* scale, float* caps,
+                         void* stream)
+{
+    if (nworld < 1 || nbody < 1 || (int64_t)nworld * nbody > (1 << 24))
+        return fail(HL_E_INVALID, "need nworld, nbody >= 1");
+    if (pos_stride < 3 || quat_stride < 4) return fail(HL_E_INVALID, "pos_stride >= 3 and quat_stride >= 4");
+    if (!pos || !quat || !shapes || !solid_ids || !engine_origin || !lattice_origin || !scale || !caps)
+        return fail(HL_E_INVALID, "a buffer is NULL");
+    hipError_t e = hl::body_capsules(nworld, nbody, pos, pos_stride, quat, quat_stride, shapes, solid_ids,
+                                     engine_origin, lattice_origin, scale, caps, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "capsule kernel launch");
+}
+
 }  // extern "C"

@@ -89,4 +89,17 @@ hipError_t render_plane(int dtype, const EnvArgs& a, int kind, int normal, int i
 hipError_t render_projection(int dtype, const EnvArgs& a, int kind, int axis, float gl, float* out, hipStream_t s);
 int max_render_thickness();
 
+// Coupling with a rigid-body engine (coupling.hip); see include/homelbm.h.
+hipError_t couple_poses(int lattice, int nworld, int nbody, int nsolid, const float* pos, int pstride,
+                        const float* quat, int qstride, const int32_t* solid, const float* eorigin,
+                        const float* lorigin, const float* scale, float* const* tables, int32_t* init, hipStream_t s);
+hipError_t couple_forces(int lattice, int mode, int nworld, int nbody, int nsolid, const double* rows,
+                         int64_t wstride, const int32_t* solid, const float* scale, float fconv, float tconv,
+                         float* fout, float* tout, float* xfrc, int xbodies, const int32_t* body, hipStream_t s);
+hipError_t free_bodies_step(int nworld, int nbody, float* state, const float* params, const float* force,
+                            const float* torque, const float g[3], float dt, hipStream_t s);
+hipError_t body_capsules(int nworld, int nbody, const float* pos, int pstride, const float* quat, int qstride,
+                         const float* shapes, const int32_t* solid, const float* eorigin, const float* lorigin,
+                         const float* scale, float* caps, hipStream_t s);
+
 }  // namespace hl
