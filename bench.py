@@ -39,6 +39,8 @@ def parse():
                    help="seconds per arrival rate of the serving sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--karman-steps", type=int, default=40000,
                    help="steps of the FP32 vs FP64 Karman drag/Strouhal check reported beside the D3Q27 line (0 = skip)")
+    p.add_argument("--coupled-steps", type=int, default=200,
+                   help="steps of the two-way coupled sphere reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--precision-steps", type=int, default=5000,
                    help="steps of the FP64/FP32/FP16/BF16 eel sweep reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
@@ -108,6 +110,40 @@ def cpu_baseline(w, seconds: float):
             break
     return {"value": round(w.ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"{steps} FP32 steps of the full {w.name} lattice, OpenMP oracle, {el:.1f} s"}
+
+
+def bench_coupled(K: int, W: int = 20):
+    """Two-way coupled step on the device: a sphere (radius 4, twice the
+    fluid density) released in a 64x32x32 D3Q27 channel (inlet 0.05); per step
+    posed capsule -> mask, pose -> pose table, lattice step with its force
+    rows, rows -> body loads, free-body step. Synthetic, seeded geometry."""
+    import math
+
+    from openhomelbm_amd import coupling as CP
+    from openhomelbm_amd.lattice import Lattice
+
+    r = 4.0
+    mass = 2.0 * 4.0 / 3.0 * math.pi * r ** 3
+    lat = Lattice(27, 64, 32, 32, nsolid=1)
+    lat.nu = 0.02
+    lat.bc_type = [0, 1, 1, 1, 1, 1]
+    lat.bc_value = [[0.05, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+    lat.init_uniform((0.05, 0.0, 0.0))
+    fb = CP.FreeBodies(1, [[[20.0, 16.0, 16.0]]], [mass], [[0.4 * mass * r * r] * 3])
+    world = CP.CoupledWorlds([lat], fb, CP.BodyMap([0]), [[0, 0, 0, 0, 0, 0, r]], box=(8, 56, 8, 24, 8, 24))
+    world.step(W)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    world.step(K)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    s = fb.state.cpu().numpy()[0, 0]
+    return {"workload": "coupled_sphere_64x32x32", "steps": K, "ms_per_step": round(ms, 4),
+            "mlups": round(64 * 32 * 32 / (ms * 1e-3) / 1e6, 1), "body_x": round(float(s[0]), 4),
+            "body_vx": round(float(s[7]), 6),
+            "step": "capsule mask + pose table + lattice step with forces + load conversion + free-body step"}
 
 
 def bench_batched(K: int, W: int, per_env_steps: int, graph_steps: int = 50):
@@ -287,6 +323,11 @@ def main():
         prec["seconds"] = round(time.perf_counter() - t0p, 1)
         log(f"[bench] precision sweep {prec}")
 
+    coupled = None
+    if args.coupled_steps > 0 and args.workload == "d3q27_sphere":
+        coupled = bench_coupled(args.coupled_steps)
+        log(f"[bench] coupled {coupled}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(w, args.cpu_seconds)
@@ -309,7 +350,7 @@ def main():
                    "grid": [w.nx, w.ny, w.nz], "nu": w.nu, "u_in": w.u_in[0], "kernel": args.kernel,
                    "step": "stream/collide/boundary kernel + FP64 per-solid force reduction",
                    "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve, "karman": karm,
-                   "precision": prec},
+                   "precision": prec, "coupled": coupled},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel_us": round(kernel_ms * 1e3, 2), "copy_gbs": round(copy_gbs, 1),
