@@ -119,6 +119,8 @@ def bench_coupled(K: int, W: int = 20):
     rows, rows -> body loads, free-body step. Synthetic, seeded geometry."""
     import math
 
+    import torch
+
     from openhomelbm_amd import coupling as CP
     from openhomelbm_amd.lattice import Lattice
 
