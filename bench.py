@@ -12,6 +12,7 @@ CPU-baseline of the oracle on the host cores).
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -110,6 +111,76 @@ def cpu_baseline(w, seconds: float):
             break
     return {"value": round(w.ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"{steps} FP32 steps of the full {w.name} lattice, OpenMP oracle, {el:.1f} s"}
+
+
+def bench_karman_speed(K: int = 2000, W: int = 50):
+    """The 512x128 D2Q9 cylinder (FP32, forces every step): one hl_step call
+    per step (the reference calls solver.step() per substep), one call for K
+    steps, and replays of a recorded one-step graph pair."""
+    import torch
+
+    from openhomelbm_amd import inputs
+
+    w = inputs.d2q9_karman(1001)
+    lat = inputs.build_lattice(w, "f32")
+    h = torch.zeros((K, lat.nsolid, 6), dtype=torch.float64, device=lat.device)
+    lat.step(W, force_history=h[:W])
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / K
+
+    per_call = timed(lambda: [lat.step(1, force_history=h[k:k + 1]) for k in range(K)])
+    one_call = timed(lambda: lat.step(K, force_history=h))
+    g1 = lat.capture(1, force_history=h[:1])
+    for _ in range(W):
+        g1.replay()
+    graph1 = timed(lambda: [g1.replay() for _ in range(K)])
+    n = 50
+    gn = lat.capture(n, force_history=h[:n])
+    gn.replay()
+    graphn = timed(lambda: [gn.replay() for _ in range(K // n)]) * K / (K // n * n)
+    mlups = lambda ms: round(w.ncell / (ms * 1e-3) / 1e6, 1)
+    return {"workload": "d2q9_karman_512x128", "steps": K, "us_per_step_per_call": round(per_call * 1e3, 2),
+            "us_per_step_one_call": round(one_call * 1e3, 2), "us_per_step_graph1": round(graph1 * 1e3, 2),
+            f"us_per_step_graph{n}": round(graphn * 1e3, 2), "mlups_per_call": mlups(per_call),
+            "mlups_one_call": mlups(one_call), "mlups_graph1": mlups(graph1), f"mlups_graph{n}": mlups(graphn)}
+
+
+def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
+    """The reference's API pattern: LBM_Solver(512, 128, nworld) with a square
+    body per world (configure_solid), solver.step() per step; the batched
+    route (every world in one launch sequence) against world by world."""
+    import torch
+
+    from openhomelbm_amd.lbm import LBM_Solver
+
+    sq = np.array([[-0.02, -0.02], [0.02, -0.02], [0.02, 0.02], [-0.02, 0.02]], np.float32)
+    out = {"worlds": nworld, "grid": [512, 128], "steps": K}
+    for name, batched in (("batched", True), ("per_world", False)):
+        s = LBM_Solver(512, 128, solid_num=1, nworld=nworld, batched=batched)
+        for f in s.flows:
+            f.vis_shear = 0.008
+            f.bc_type = [0, 1, 1, 1]
+            f.bc_value = [[0.05, 0.0], [0, 0], [0, 0], [0, 0]]
+            f.configure_solid(0, sq, position=(96.0, 65.0), angle=0.0, scale=1.0)
+        for _ in range(W):
+            s.step()
+        torch.cuda.synchronize()
+        gc.collect()  # no collection of earlier solvers inside the timed steps
+        t0 = time.perf_counter()
+        for _ in range(K):
+            s.step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / K
+        out[f"ms_per_step_{name}"] = round(ms, 4)
+        out[f"mlups_{name}"] = round(nworld * 512 * 128 / (ms * 1e-3) / 1e6, 1)
+    return out
 
 
 def bench_coupled(K: int, W: int = 20):
@@ -313,6 +384,8 @@ def main():
 
         t0k = time.perf_counter()
         karm = karman.compare(steps=args.karman_steps, window=min(25000, args.karman_steps // 2), log=log)
+        karm["speed"] = bench_karman_speed()
+        karm["lbm_solver_worlds"] = bench_solver_worlds()
         karm["seconds"] = round(time.perf_counter() - t0k, 1)
         log(f"[bench] karman {karm}")
 

@@ -212,6 +212,16 @@ class Lattice:
         N.check(self.lib.hl_step(C.byref(e), self.cur, int(nsteps), int(kernel), fh, _stream_ptr(stream)), "hl_step")
         self.cur ^= nsteps & 1
 
+    def capture(self, nsteps: int, kernel: int = N.HL_KERNEL_AUTO,
+                force_history: Optional[torch.Tensor] = None) -> "StepGraph":
+        """Record `nsteps` steps into hipGraphs (LBM_Solver.step's graph,
+        envs/lbm/lbm_solver.py:97-141): one graph per starting buffer parity
+        (one for an even nsteps), replayed by StepGraph.replay(). The graph keeps
+        this configuration's launch arguments: device buffers (state, mask,
+        pose table, force_history) are read at replay time, but the mask box,
+        body count and parameters are those of the capture."""
+        return StepGraph(self, nsteps, kernel, force_history)
+
     # ------------------------------------------------------------ bodies
     def set_mask(self, mask: Optional[np.ndarray], box=None) -> None:
         """Install a host mask ((z,) y, x uint8); box defaults to its nonzero bounding box."""
@@ -358,3 +368,32 @@ class Lattice:
         """Algorithmic HBM bytes per cell update: every field read once, written once."""
         size = {"f32": 4, "f64": 8, "f16": 2, "bf16": 2}[self.dtype]
         return 2 * self.nfields * size
+
+
+class StepGraph:
+    """Multi-step hipGraphs of one Lattice (see Lattice.capture)."""
+
+    def __init__(self, lat: "Lattice", nsteps: int, kernel: int, force_history: Optional[torch.Tensor]):
+        if nsteps < 1:
+            raise ValueError("nsteps must be >= 1")
+        self.lat, self.nsteps = lat, int(nsteps)
+        self.stream = torch.cuda.Stream(device=lat.device)
+        if lat.mask is not None and lat.nsolid > 0:
+            # size this stream's workspace outside the capture (no allocation may be recorded)
+            lat.forces(stream=self.stream)
+        self.stream.synchronize()
+        cur0 = lat.cur
+        self.graphs = {}
+        for p in ([cur0] if nsteps % 2 == 0 else [cur0, cur0 ^ 1]):
+            lat.cur = p
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g, stream=self.stream):
+                lat.step(self.nsteps, kernel, force_history=force_history)
+            self.graphs[p] = g
+        lat.cur = cur0
+
+    def replay(self) -> None:
+        """Run the recorded steps from the lattice's present parity (on the
+        current stream, asynchronously)."""
+        self.graphs[self.lat.cur].replay()
+        self.lat.cur ^= self.nsteps & 1

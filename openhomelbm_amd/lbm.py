@@ -88,6 +88,8 @@ class HomeFlow:
         }
         self._capsules: dict = {}
         self._last_pose: Optional[np.ndarray] = None
+        self._uploaded: Optional[np.ndarray] = None  # pose table last copied to the device
+        self._at_rest = False  # uploaded table has last == current and the solids are unchanged since
         self._forces = torch.zeros((1, max(n_objects, 1), 6), dtype=torch.float64, device=L.device)
         self._box = (0, 0, 0, 0, 0, 0)
 
@@ -291,9 +293,15 @@ class HomeFlow:
         L = self.lattice
         if self.n_objects == 0:
             return
+        if not self._solid_dirty and self._at_rest:
+            return  # the uploaded table (pose | same pose) and the mask still hold
         now = self._poses()
         last = now if self._last_pose is None else self._last_pose
-        L.set_poses(np.concatenate([now, last], axis=1)[: self.n_objects])
+        table = np.concatenate([now, last], axis=1)[: self.n_objects].astype(np.float32)
+        if self._uploaded is None or not np.array_equal(table, self._uploaded):  # static bodies: upload once
+            L.set_poses(table)
+            self._uploaded = table
+        self._at_rest = bool(np.array_equal(now, last))
         self._last_pose = now
         if not self._solid_dirty:
             return
@@ -330,9 +338,15 @@ class LBM_Solver:
     """Worlds of one size stepped together (envs/lbm/lbm_solver.py:21-141)."""
 
     def __init__(self, nx: int, ny: int, solid_num: int = 1, nworld: int = 1, device: Optional[Any] = None,
-                 dtype: str = "f32") -> None:
+                 dtype: str = "f32", batched: bool = True) -> None:
         if nworld < 1:
             raise ValueError("nworld must be >= 1")
+        self._use_batch = bool(batched)
+        self._batch = None
+        self._batch_keys = None
+        self._batch_boxes = None
+        self._batch_cur = None
+        self._hist = None
         self.nx, self.ny = nx, ny
         self.solid_num = solid_num
         self.nworld = nworld
@@ -352,12 +366,55 @@ class LBM_Solver:
         self.solid_ids_wp = None
         self.solid_id_to_index = {}
 
+    def _batchable(self) -> bool:
+        from .batch import MAX_SOLIDS
+        return self._use_batch and len(self.flows) > 1 and all(
+            f.lattice.dtype == "f32" and f.lattice.force is None and f.lattice.nsolid <= MAX_SOLIDS
+            for f in self.flows)
+
+    @staticmethod
+    def _descriptor_key(L):
+        """What a batch keeps of a world's descriptor, except the mask box
+        (updated in place): buffers, viscosity, faces."""
+        return (L.mask.data_ptr() if L.mask is not None else 0, L.force is None, L.nsolid, float(L.nu),
+                tuple(L.bc_type), tuple(tuple(v) for v in L.bc_value), L.buf.data_ptr(), L.solids.data_ptr())
+
     def step(self) -> None:
-        """Advance every world one step (asynchronous). The reference captures a
-        graph on the first call; here every call launches the fused kernels, so
-        resetting `captured` is harmless."""
+        """Advance every world one step (asynchronous): with several FP32 worlds
+        and no body-force field, one batched launch sequence for all of them
+        (bitwise the same as stepping each world alone), else world by world.
+        The reference captures a graph on the first call; resetting `captured`
+        is harmless here."""
+        if not self._batchable():
+            for flow in self.flows:
+                flow.step()
+            self.captured = True
+            return
+        from .batch import Batch
         for flow in self.flows:
-            flow.step()
+            flow._prepare()
+        lats = [f.lattice for f in self.flows]
+        keys = [self._descriptor_key(L) for L in lats]
+        if self._batch is None or keys != self._batch_keys:
+            if self._batch is not None:
+                self._batch.close()
+            self._batch = Batch(lats)
+            self._batch_keys = keys
+            self._batch_boxes = [tuple(L.mask_box) for L in lats]
+            self._hist = self._batch.history_buffer(1)
+            # each world's force rows are read from the batch's history in place
+            for i, f in enumerate(self.flows):
+                if f.n_objects > 0:
+                    f._forces = self._hist[:, i, : f._forces.shape[1]]
+        else:
+            if [L.cur for L in lats] != self._batch_cur:  # a world was stepped or reset on its own
+                self._batch._push_state()
+            for i, L in enumerate(lats):
+                if tuple(L.mask_box) != self._batch_boxes[i]:  # bodies moved: new mask box
+                    self._batch.set_kinematics(i, None, None, box=L.mask_box)
+                    self._batch_boxes[i] = tuple(L.mask_box)
+        self._batch.step(1, force_history=self._hist)
+        self._batch_cur = [L.cur for L in lats]
         self.captured = True
 
     def create_solid_from_mujoco(self, solid_id, model, data, body_or_geom_name, lbm_position=None, lbm_scale=0.3,

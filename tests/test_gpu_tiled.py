@@ -205,3 +205,34 @@ def test_force_history_one_call_equals_step_by_step():
     lat.step(n - 1, kernel=N.HL_KERNEL_TILED)
     f = lat.forces().cpu().numpy()
     np.testing.assert_allclose(f, hists[0][n - 1], rtol=1e-10, atol=1e-13)
+
+
+@pytest.mark.parametrize("lattice,nsteps", [(27, 3), (9, 2)])
+def test_graph_replays_equal_eager_steps(lattice, nsteps):
+    """Recorded multi-step graphs (both parities for an odd count) replayed
+    four times, with eager work and synchronisations in between: every
+    replay's state and force history equal the same steps run eagerly."""
+    if lattice == 27:
+        env, caps, poses, force = _tiled_scene(12)
+        shape = SHAPE
+    else:
+        from tests.test_gpu_parity import _scene
+        shape = (40, 56)
+        env, caps, poses, force = _scene(9, shape, 3)
+    m0 = random_state(lattice, shape, seed=46).astype(np.float32)
+    a = _gpu_lattice(env, caps, poses, force)
+    b = _gpu_lattice(env, caps, poses, force)
+    a.set_state(m0)
+    b.set_state(m0)
+    ha = torch.zeros((nsteps, a.nsolid, 6), dtype=torch.float64, device=a.device)
+    hb = torch.zeros_like(ha)
+    g = b.capture(nsteps, force_history=hb)
+    for r in range(4):
+        a.step(nsteps, force_history=ha)
+        g.replay()
+        torch.cuda.synchronize()
+        assert a.cur == b.cur
+        assert np.array_equal(a.get_state(np.float32), b.get_state(np.float32)), r
+        assert torch.equal(ha, hb), r
+        _ = (torch.randn(1000, device=a.device) * 2).sum().item()  # eager work between replays
+        a.mass()
