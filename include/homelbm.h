@@ -119,6 +119,18 @@ int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc);
 int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* verts, const int32_t* counts,
                              int32_t npoly, int32_t maxv, const int32_t box[6], void* stream);
 
+/* Triangle-mesh voxelizer (D3Q27): device tris [ntri][9] (the posed vertices
+ * a, b, c of each triangle in lattice coordinates) of closed meshes, solid
+ * ids [ntri] (< 64); a cell centre is inside when a +x ray from it crosses
+ * its solid's triangles an odd number of times; mask = lowest such solid + 1,
+ * 0 elsewhere in the box (merge != 0: only cells still 0 are set). A column
+ * meeting more than 64 crossings sets device *overflow (if not NULL).
+ * Replaces the BVH link queries of get_cutcell_multi_3d
+ * (envs/lbm3d/lbm_func_3d.py:254-316) behind create_solid_from_mesh
+ * (envs/lbm3d/lbm_solver_3d.py:129-242): the posed mesh becomes a mask. */
+int32_t hl_voxelize_mesh(const hl_env* env, uint8_t* mask, const float* tris, const int32_t* tri_solid,
+                         int32_t ntri, const int32_t box[6], int32_t merge, int32_t* overflow, void* stream);
+
 /* FP64 sum of rho over all cells of state `cur` into device out[0]. */
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 

@@ -394,6 +394,24 @@ int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
     return HL_OK;
 }
 
+int32_t hl_voxelize_mesh(const hl_env* env, uint8_t* mask, const float* tris, const int32_t* tri_solid,
+                         int32_t ntri, const int32_t box[6], int32_t merge, int32_t* overflow, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (env->lattice != HL_D3Q27) return fail(HL_E_UNSUPPORTED, "mesh bodies are for D3Q27 lattices");
+    if (!mask || !box || (ntri > 0 && (!tris || !tri_solid))) return fail(HL_E_INVALID, "a buffer is NULL");
+    if (ntri < 0) return fail(HL_E_INVALID, "ntri must be >= 0");
+    const int n3[3] = {env->nx, env->ny, env->nz};
+    for (int k = 0; k < 3; ++k)
+        if (box[2 * k] < 0 || box[2 * k + 1] > n3[k] || box[2 * k] > box[2 * k + 1])
+            return fail(HL_E_INVALID, "box [%d, %d) outside axis %d of extent %d", box[2 * k], box[2 * k + 1], k, n3[k]);
+    const hl::EnvArgs a = make_args(env, 0);
+    hipError_t e = hl::voxelize_mesh(a, mask, tris, tri_solid, ntri, box, merge, overflow,
+                                     static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "mesh voxelizer launch");
+}
+
 int32_t hl_render_plane(const hl_env* env, int32_t cur, int32_t kind, int32_t normal, int32_t index,
                         float thickness, float max_velocity, float* out, void* stream)
 {

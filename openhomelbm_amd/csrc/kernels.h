@@ -89,6 +89,11 @@ hipError_t render_plane(int dtype, const EnvArgs& a, int kind, int normal, int i
 hipError_t render_projection(int dtype, const EnvArgs& a, int kind, int axis, float gl, float* out, hipStream_t s);
 int max_render_thickness();
 
+// Triangle-mesh voxelizer (mesh.hip): tris [ntri][9] posed vertices, solid
+// ids [ntri]; rewrites the box (merge: only sets cells still 0).
+hipError_t voxelize_mesh(const EnvArgs& a, uint8_t* mask, const float* tris, const int32_t* tsolid, int ntri,
+                         const int32_t box[6], int merge, int32_t* overflow, hipStream_t s);
+
 // Coupling with a rigid-body engine (coupling.hip); see include/homelbm.h.
 hipError_t couple_poses(int lattice, int nworld, int nbody, int nsolid, const float* pos, int pstride,
                         const float* quat, int qstride, const int32_t* solid, const float* eorigin,

@@ -100,6 +100,7 @@ class Lattice:
         self.solids = torch.zeros((max(nsolid, 1), self.pose_width), dtype=torch.float32, device=self.device)
         self._init_identity_poses()
         self.mask: Optional[torch.Tensor] = None
+        self._mesh_overflow: Optional[torch.Tensor] = None
         self.mask_box = (0, 0, 0, 0, 0, 0)
         self._mask_dirty = None
         self.force: Optional[torch.Tensor] = None
@@ -280,6 +281,49 @@ class Lattice:
                                      _stream_ptr(stream)), "hl_voxelize")
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
+
+    def voxelize_mesh(self, tris: np.ndarray, solids: Sequence[int], box=None, merge: bool = False,
+                      stream=None) -> None:
+        """Mask from closed triangle meshes (D3Q27): tris [ntri, 9] posed
+        vertices in lattice coordinates, solids [ntri] solid ids; the box
+        defaults to the triangles' bounding box (one cell of padding). Without
+        merge the box (and the previous mask box) is rewritten; with merge only
+        cells still 0 are set."""
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+        sid = np.ascontiguousarray(solids, dtype=np.int32).reshape(-1)
+        if sid.shape[0] != t.shape[0]:
+            raise ValueError("one solid id per triangle")
+        if t.shape[0] and (sid.min() < 0 or sid.max() >= min(self.nsolid, 64)):
+            raise ValueError("triangle solid ids must be in [0, nsolid)")
+        if self.mask is None:
+            self.mask = torch.zeros(self.ncell, dtype=torch.uint8, device=self.device)
+        if box is None:
+            if t.shape[0] == 0:
+                box = (0, 0, 0, 0, 0, 0)
+            else:
+                v = t.reshape(-1, 3).astype(np.float64)
+                lo, hi = v.min(0), v.max(0)
+                box = []
+                for a, n in enumerate((self.nx, self.ny, self.nz)):
+                    b0 = max(0, int(math.floor(lo[a])) - 1)
+                    box += [b0, max(b0, min(n, int(math.ceil(hi[a])) + 2))]
+        box = tuple(int(v) for v in box)
+        write = box if merge else union_box(self._mask_dirty, box)
+        self._tris_dev = torch.from_numpy(t).to(self.device)
+        self._tsid_dev = torch.from_numpy(sid).to(self.device)
+        if self._mesh_overflow is None:
+            self._mesh_overflow = torch.zeros(1, dtype=torch.int32, device=self.device)
+        b = (C.c_int32 * 6)(*write)
+        e = self.env()
+        N.check(self.lib.hl_voxelize_mesh(C.byref(e), self.mask.data_ptr(), self._tris_dev.data_ptr(),
+                                          self._tsid_dev.data_ptr(), t.shape[0], b, 1 if merge else 0,
+                                          self._mesh_overflow.data_ptr(), _stream_ptr(stream)), "hl_voxelize_mesh")
+        self.mask_box = union_box(self.mask_box, box) if merge else tuple(box)
+        self._mask_dirty = tuple(self.mask_box)
+
+    def mesh_overflowed(self) -> bool:
+        """True when a mesh voxelization met a ray with more than 64 crossings."""
+        return bool(self._mesh_overflow is not None and int(self._mesh_overflow.item()))
 
     def voxelize_polygons(self, verts: np.ndarray, counts: Sequence[int], box=None, stream=None) -> None:
         """Rebuild the mask from closed 2D polygons (even-odd rule) on the GPU.

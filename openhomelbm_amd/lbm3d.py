@@ -5,13 +5,13 @@ Same class names, constructor signatures, methods and field names; fields
 are LatticeField / HostArray handles with .numpy() / .assign(). Differences,
 all deliberate:
 
-* Bodies are analytic capsules and spheres (create_solid_capsule,
-  create_solid_sphere) in the body frame, scaled like the reference's meshes
-  (lbm_scale * nx) and posed by position + quaternion (w, x, y, z). Each step
-  they are voxelized into an integer mask and the wall rule runs on mask links
+* Bodies are closed triangle meshes (create_solid_from_mesh) or analytic
+  capsules and spheres (create_solid_capsule, create_solid_sphere) in the body
+  frame, scaled by lbm_scale * nx and posed by position + quaternion
+  (w, x, y, z). When a pose or shape changes they are voxelized into an
+  integer mask (meshes by +x ray parity) and the wall rule runs on mask links
   (mask(x) != mask(x-c_i)) instead of BVH ray queries (get_cutcell_multi_3d,
-  lbm_func_3d.py:254-316). create_solid_from_mesh needs triangle-mesh ray
-  queries, which this library does not provide; it raises.
+  lbm_func_3d.py:254-316).
 * solid_force / solid_torque come from a deterministic FP64 reduction.
 * step() is asynchronous; reading a field synchronizes.
 """
@@ -25,7 +25,18 @@ import torch
 
 from . import _native as N
 from .arrays import ForceField, HostArray, LatticeField, as_numpy
-from .lattice import Lattice
+from .lattice import Lattice, capsule_box, union_box
+
+
+def _tri_box(tris: np.ndarray, L: Lattice):
+    """Half-open cell box around posed triangles (one cell of padding)."""
+    v = np.asarray(tris, np.float64).reshape(-1, 3)
+    lo, hi = v.min(0), v.max(0)
+    box = []
+    for a, n in enumerate((L.nx, L.ny, L.nz)):
+        b0 = max(0, int(math.floor(lo[a])) - 1)
+        box += [b0, max(b0, min(n, int(math.ceil(hi[a])) + 2))]
+    return tuple(box)
 
 ML_FLUID, ML_SOLID, ML_WALL = 0, 1, 2
 
@@ -89,6 +100,7 @@ class HomeFlow3D:
             "mesh_transforms_initialized": HostArray(np.zeros(n, np.int32), dtype=np.int32),
         }
         self._shapes: Dict[int, Tuple[np.ndarray, np.ndarray, float]] = {}  # body-frame capsules (lattice units)
+        self._meshes: Dict[int, Tuple[np.ndarray, np.ndarray]] = {}  # body-frame meshes (lattice units)
         self._last: Optional[np.ndarray] = None  # pose table of the previous step
         self.capsules = np.zeros((0, 8), np.float32)  # world-frame capsules of the last voxelization
         self._dirty = True
@@ -213,6 +225,19 @@ class HomeFlow3D:
         self._solid["solid_bound_radius"][solid_id] = max(np.linalg.norm(a), np.linalg.norm(b)) + radius
         self._dirty = True
 
+    def set_mesh(self, solid_id: int, vertices, faces) -> None:
+        """Body-frame closed triangle mesh (lattice units) of solid solid_id."""
+        if solid_id < 0 or solid_id >= self.n_objects:
+            raise ValueError(f"solid_id {solid_id} out of range [0, {self.n_objects})")
+        v = np.asarray(vertices, np.float64).reshape(-1, 3)
+        f = np.asarray(faces, np.int64).reshape(-1, 3)
+        if f.size and (f.min() < 0 or f.max() >= len(v)):
+            raise ValueError("face indices out of range")
+        self._meshes[solid_id] = (v, f)
+        self._shapes.pop(solid_id, None)
+        self._solid["solid_bound_radius"][solid_id] = float(np.linalg.norm(v, axis=1).max()) if len(v) else 0.0
+        self._dirty = True
+
     def _pose_table(self) -> np.ndarray:
         n = max(self.n_objects, 1)
         P = np.zeros((n, 12))
@@ -245,8 +270,21 @@ class HomeFlow3D:
             R, p = now[k, 3:12].reshape(3, 3), now[k, 0:3]
             caps.append(list(R @ a + p) + list(R @ b + p) + [r, k])
         self.capsules = np.array(caps, np.float32).reshape(-1, 8)
-        if caps:
+        tris, tsid = [], []
+        for k, (v, f) in sorted(self._meshes.items()):
+            R, p = now[k, 3:12].reshape(3, 3), now[k, 0:3]
+            w = v @ R.T + p
+            tris.append(w[f].reshape(-1, 9))
+            tsid.append(np.full(len(f), k, np.int32))
+        if caps and tris:
+            T = np.concatenate(tris).astype(np.float32)
+            box = union_box(capsule_box(self.capsules, (self.nx, self.ny, self.nz)), _tri_box(T, L))
+            L.voxelize(self.capsules, box=box, stream=stream)
+            L.voxelize_mesh(T, np.concatenate(tsid), merge=True, stream=stream)
+        elif caps:
             L.voxelize(self.capsules, stream=stream)
+        elif tris:
+            L.voxelize_mesh(np.concatenate(tris), np.concatenate(tsid), stream=stream)
         self._dirty = False
 
     def step(self, stream=None) -> None:
@@ -300,9 +338,34 @@ class LBM_Solver3D:
             raise ValueError(f"solid_id {solid_id} out of range [0, {self.solid_num})")
 
     def create_solid_from_mesh(self, solid_id, mesh, lbm_position=None, lbm_scale=0.15, init_quaternion=(1, 0, 0, 0),
-                               mujoco_origin=None):
-        raise NotImplementedError("triangle-mesh bodies need BVH ray queries, which this library does not provide; "
-                                  "describe the body with create_solid_capsule / create_solid_sphere")
+                               mujoco_origin=None) -> Dict[str, Any]:
+        """The same closed triangle mesh in every world (lbm_solver_3d.py:129-242):
+        mesh has .vertices [n, 3] and .faces [m, 3] (a trimesh.Trimesh or any
+        object with those arrays, or a (vertices, faces) pair) in its body frame,
+        not recentred; scaled by lbm_scale * nx, its origin placed at
+        lbm_position (the domain centre by default) with init_quaternion
+        (w, x, y, z). The posed mesh is voxelized into each world's mask (the
+        reference ray-casts a BVH per link instead)."""
+        self._check_solid(solid_id)
+        if isinstance(mesh, (tuple, list)) and len(mesh) == 2:
+            verts, faces = mesh
+        else:
+            verts, faces = mesh.vertices, mesh.faces
+        s = lbm_scale * self.nx
+        scaled = np.asarray(verts, np.float64).reshape(-1, 3) * s
+        if lbm_position is None:
+            lbm_position = (self.nx * 0.5, self.ny * 0.5, self.nz * 0.5)
+        for flow in self.flows:
+            flow.set_mesh(solid_id, scaled, faces)
+            flow.solid_position[solid_id] = lbm_position
+            flow.solid_quaternion[solid_id] = init_quaternion
+        if mujoco_origin is None:
+            mujoco_origin = np.zeros(3, np.float32)
+        self.mujoco_mappings[solid_id] = {"mujoco_origin": np.array(mujoco_origin, np.float32),
+                                          "lbm_origin": np.array(lbm_position, np.float32), "scale": s}
+        self.meshes[solid_id] = mesh
+        return {"mesh": mesh, "vertices": scaled, "scale": s, "lbm_position": lbm_position,
+                "quaternion": init_quaternion}
 
     def create_solid_capsule(self, solid_id: int, p0: Sequence[float], p1: Sequence[float], radius: float,
                              lbm_position: Optional[Sequence[float]] = None, lbm_scale: float = 0.15,

@@ -170,8 +170,8 @@ def test_lbm_solver3d_pose_update_and_reset():
     assert np.all(f.rho.numpy() == 1.0) and np.all(f.u.numpy() == 0.0)
     s.finalize_mappings([0, 1])
     assert s.scales_wp.shape == (2,)
-    with pytest.raises(NotImplementedError):
-        s.create_solid_from_mesh(0, None)
+    with pytest.raises(ValueError):
+        s.create_solid_from_mesh(0, (np.zeros((3, 3)), [[0, 1, 5]]))  # face index out of range
 
 
 def _moving_worlds(batched, nworld=3, steps=6):
