@@ -131,6 +131,24 @@ int32_t hl_voxelize_polygons(const hl_env* env, uint8_t* mask, const float* vert
 int32_t hl_voxelize_mesh(const hl_env* env, uint8_t* mask, const float* tris, const int32_t* tri_solid,
                          int32_t ntri, const int32_t box[6], int32_t merge, int32_t* overflow, void* stream);
 
+/* D2Q9 steps with the reference's exact polygon cut cells instead of the
+ * mask: each link of an interior cell is tested against the segments of the
+ * polygons whose bounding circle holds the cell (get_cutcell,
+ * envs/lbm/lbm_func.py:545-607), and a cut link takes the wall rule at the
+ * crossing's material point, its velocity from this step's and the previous
+ * step's vertices, its lever arm from the pose table's position
+ * (stream_and_collide, lbm_func.py:346-401); the per-step graph of
+ * LBM_Solver.step (lbm_solver.py:97-141) with precompute_transformed_segments
+ * done by the caller. Device inputs: verts / verts_last [nsolid][maxv][2]
+ * transformed vertices (lbm_func.py:509-542), counts [nsolid], radius2
+ * [nsolid] = (solid_max_radius * solid_scale * nx) * (solid_scale * nx) in
+ * FP32. verts_last applies to the first step; later steps of the call see
+ * verts twice (bodies at rest). env->mask is ignored; force_history as in
+ * hl_step (Fx, Fy at 0, 1, the torque scaled by 1/nx at 5). */
+int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const float* verts,
+                         const float* verts_last, const int32_t* counts, int32_t maxv, const float* radius2,
+                         double* force_history, void* stream);
+
 /* FP64 sum of rho over all cells of state `cur` into device out[0]. */
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 

@@ -292,6 +292,34 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
     return HL_OK;
 }
 
+int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const float* verts,
+                         const float* verts_last, const int32_t* counts, int32_t maxv, const float* radius2,
+                         double* force_history, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (env->lattice != HL_D2Q9) return fail(HL_E_UNSUPPORTED, "polygon cut cells are D2Q9 only");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (env->nsolid < 1) return fail(HL_E_INVALID, "polygon cut cells need nsolid >= 1");
+    if (!verts || !counts || !radius2) return fail(HL_E_INVALID, "verts, counts and radius2 must be non-NULL");
+    if (maxv < 1) return fail(HL_E_INVALID, "maxv must be >= 1, got %d", maxv);
+    if (nsteps == 0) return HL_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    void* ws = nullptr;
+    rc = workspace(ctx, s, hl::polygon_workspace_bytes(), &ws);
+    if (rc) return rc;
+    hl::EnvArgs a = make_args(env, cur);
+    hipError_t e = hl::step_polygons(env->dtype, a, nsteps, verts, verts_last ? verts_last : verts, counts, maxv,
+                                     radius2, force_history && env->nsolid > 0 ? force_history : nullptr, ws, s);
+    if (e != hipSuccess) return hip_fail(e, "polygon step launch");
+    return HL_OK;
+}
+
 int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
 {
     int32_t rc = validate(env);

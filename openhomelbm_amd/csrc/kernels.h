@@ -94,6 +94,14 @@ int max_render_thickness();
 hipError_t voxelize_mesh(const EnvArgs& a, uint8_t* mask, const float* tris, const int32_t* tsolid, int ntri,
                          const int32_t box[6], int merge, int32_t* overflow, hipStream_t s);
 
+// Exact polygon cut cells for D2Q9 (polygon.hip): nsteps steps from a0 (its
+// src/dst swap every step) with the segment-link predicate; ws holds at
+// least polygon_workspace_bytes(); force_history [nsteps][nsolid * 6] or null.
+size_t polygon_workspace_bytes();
+hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* verts, const float* verts_last,
+                         const int32_t* counts, int maxv, const float* radius2, double* force_history, void* ws,
+                         hipStream_t s);
+
 // Coupling with a rigid-body engine (coupling.hip); see include/homelbm.h.
 hipError_t couple_poses(int lattice, int nworld, int nbody, int nsolid, const float* pos, int pstride,
                         const float* quat, int qstride, const int32_t* solid, const float* eorigin,

@@ -1,0 +1,311 @@
+// Exact polygon cut cells for D2Q9 (the reference's boundary, no mask).
+//
+// The reference tests every link of every fluid cell against the segments of
+// the polygon bodies whose bounding circle holds the cell (get_cutcell,
+// envs/lbm/lbm_func.py:545-607) and, on a crossing, applies the wall rule at
+// the crossing's material point (stream_and_collide, lbm_func.py:346-401).
+// Far from bodies that is the plain pull step, so a step here is:
+//   1. the gather kernel over the whole lattice with no mask (plain pull);
+//   2. k_poly_band over the bounding box of the bounding circles (found once
+//      per call by k_poly_bbox): cells whose interior projection lies in a
+//      circle are recomputed with the segment predicate and rewritten, face
+//      cells through the same face chain as the gather kernel; interior
+//      cells also emit the momentum exchange of their cut links;
+//   3. k_sum_partials over the band's fixed number of per-wave FP64 rows.
+// The band kernel is a fixed grid of waves striding over the box, so the
+// FP64 force order does not depend on scheduling. Arithmetic follows the
+// reference's operation order with contraction off.
+#pragma clang fp contract(off)
+
+#include <math.h>
+
+#include "gather_post.h"
+#include "kernels.h"
+#include "lbm_device.h"
+
+namespace hl {
+
+namespace {
+
+constexpr int kPolyWaves = 256;  // band grid: one row of force partials per wave
+
+struct PolyArgs {
+    const float* verts;       // [nsolid][maxv][2], this step
+    const float* verts_last;  // same layout, previous step
+    const int32_t* counts;    // [nsolid]
+    int32_t maxv;
+    const float* radius2;     // [nsolid]
+};
+
+__device__ __forceinline__ int poly_count(const PolyArgs& p, int s)
+{
+    const int n = p.counts[s];
+    return n < 0 ? 0 : (n > p.maxv ? p.maxv : n);
+}
+
+// Solids whose bounding circle holds (x, y): the early exit of get_cutcell
+// (a NaN distance is not skipped, as there).
+template <typename T>
+__device__ __forceinline__ uint64_t poly_near(const EnvArgs& a, const PolyArgs& p, int x, int y)
+{
+    uint64_t near = 0;
+    for (int s = 0; s < a.nsolid; ++s) {
+        if (poly_count(p, s) == 0) continue;
+        const T ox = T(x) - T(a.solids[8 * s]), oy = T(y) - T(a.solids[8 * s + 1]);
+        const T d2 = ox * ox + oy * oy;
+        if (!(d2 > T(p.radius2[s]))) near |= 1ull << s;
+    }
+    return near;
+}
+
+// Nearest crossing (to the cell) of link i of cell (x, y) with a segment of
+// the near solids; returns the solid or -1, with the crossing's position
+// along its segment and the segment's first vertex.
+template <typename T>
+__device__ __forceinline__ int poly_cut(const PolyArgs& p, uint64_t near, int x, int y, int i, T* ratio, int* seg)
+{
+    T cutcell = T(-1), sr = T(-1);
+    int sid = -1, sg = -1;
+    const T bx = T(x), by = T(y);
+    const T ax = bx - T(D2CX[i]), ay = by - T(D2CY[i]);
+    while (near) {
+        const int s = __builtin_ctzll(near);
+        near &= near - 1;
+        const int num = poly_count(p, s);
+        const float* v = p.verts + (int64_t)s * p.maxv * 2;
+        for (int n = 0; n < num; ++n) {
+            const int m = n + 1 == num ? 0 : n + 1;
+            const T cx = T(v[2 * n]), cy = T(v[2 * n + 1]), dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
+            const T abc = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
+            const T abd = (ax - dx) * (by - dy) - (ay - dy) * (bx - dx);
+            if (abc * abd >= T(0)) continue;
+            const T cda = (cx - ax) * (dy - ay) - (cy - ay) * (dx - ax);
+            const T cdb = cda + abc - abd;
+            if (cda * cdb > T(0)) continue;
+            const T t = T(1) - cda / (abd - abc);
+            if (cutcell < T(0) || t < cutcell) {
+                cutcell = t;
+                sid = s;
+                sr = abc / (abc - abd);
+                sg = n;
+            }
+        }
+    }
+    if (cutcell >= T(0) && sid >= 0) {
+        *ratio = sr;
+        *seg = sg;
+        return sid;
+    }
+    return -1;
+}
+
+// Velocity (time step 1) and lever arm of the crossing's material point.
+template <typename T>
+__device__ __forceinline__ void poly_point(const EnvArgs& a, const PolyArgs& p, int sid, int seg, T ratio, T up[2],
+                                           T r[2])
+{
+    const int num = poly_count(p, sid), m = seg + 1 == num ? 0 : seg + 1;
+    const float* v = p.verts + (int64_t)sid * p.maxv * 2;
+    const float* l = p.verts_last + (int64_t)sid * p.maxv * 2;
+    const T pcx = T(v[2 * seg]) + (T(v[2 * m]) - T(v[2 * seg])) * ratio;
+    const T pcy = T(v[2 * seg + 1]) + (T(v[2 * m + 1]) - T(v[2 * seg + 1])) * ratio;
+    const T plx = T(l[2 * seg]) + (T(l[2 * m]) - T(l[2 * seg])) * ratio;
+    const T ply = T(l[2 * seg + 1]) + (T(l[2 * m + 1]) - T(l[2 * seg + 1])) * ratio;
+    up[0] = pcx - plx;
+    up[1] = pcy - ply;
+    r[0] = pcx - T(a.solids[8 * sid]);
+    r[1] = pcy - T(a.solids[8 * sid + 1]);
+}
+
+// Bounding box (half-open [x0, x1) x [y0, y1)) of the cells whose interior
+// projection can lie in a bounding circle; a circle with a non-finite centre
+// or radius takes the whole lattice.
+__global__ void k_poly_bbox(EnvArgs a, PolyArgs p, int* bbox)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const float hx = (float)(a.nx - 2), hy = (float)(a.ny - 2);
+    float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
+    for (int s = 0; s < a.nsolid; ++s) {
+        if (poly_count(p, s) == 0) continue;
+        const float cx = a.solids[8 * s], cy = a.solids[8 * s + 1], r2 = p.radius2[s];
+        if (!isfinite(cx) || !isfinite(cy) || !isfinite(r2)) {
+            x0 = 1.0f; x1 = hx; y0 = 1.0f; y1 = hy;
+            continue;
+        }
+        if (r2 < 0.0f) continue;
+        const float r = sqrtf(r2);
+        x0 = fminf(x0, floorf(cx - r) - 2.0f);
+        x1 = fmaxf(x1, ceilf(cx + r) + 2.0f);
+        y0 = fminf(y0, floorf(cy - r) - 2.0f);
+        y1 = fmaxf(y1, ceilf(cy + r) + 2.0f);
+    }
+    // clip to the interior, then one more cell for the faces
+    x0 = fmaxf(x0, 1.0f); x1 = fminf(x1, hx);
+    y0 = fmaxf(y0, 1.0f); y1 = fminf(y1, hy);
+    if (x0 > x1 || y0 > y1) {
+        bbox[0] = bbox[1] = bbox[2] = bbox[3] = 0;
+        return;
+    }
+    bbox[0] = (int)x0 - 1;
+    bbox[1] = (int)x1 + 2;
+    bbox[2] = (int)y0 - 1;
+    bbox[3] = (int)y1 + 2;
+}
+
+template <typename T, int DT>
+__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial)
+{
+    __shared__ double acc[HL_MAX_SOLIDS * 6];
+    using S = Storage<DT>;
+    using ST = typename S::type;
+    const ST* src = static_cast<const ST*>(a.src);
+    ST* dst = static_cast<ST*>(a.dst);
+    const int lane = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+    const int width = a.nsolid * 6;
+    if (partial)
+        for (int k = lane; k < width; k += 64) acc[k] = 0.0;
+    __syncthreads();
+    const int bx0 = bbox[0], by0 = bbox[2];
+    const int bw = bbox[1] - bbox[0], bh = bbox[3] - bbox[2];
+    const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
+    const T grid_length = T(1) / T(nx);
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+        const int64_t t = base + lane;
+        int lsid[9];
+        T lf[9][3];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) lsid[i] = -1;
+        uint64_t mine = 0;
+        if (t < n) {
+            const int x = bx0 + (int)(t % bw), y = by0 + (int)(t / bw);
+            const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
+            const uint64_t near = poly_near<T>(a, p, px, py);
+            if (near) {
+                const bool interior = px == x && py == y;
+                const int64_t c = (int64_t)px + (int64_t)nx * py;
+                T mc[6];
+                load_cell<T, DT>(a, c, 6, mc);
+                T pop[9];
+#pragma unroll
+                for (int i = 0; i < 9; ++i) {
+                    T ratio = 0;
+                    int seg = -1;
+                    const int sid = i == 0 ? -1 : poly_cut<T>(p, near, px, py, i, &ratio, &seg);
+                    if (sid >= 0) {
+                        T up[2], r[2];
+                        poly_point<T>(a, p, sid, seg, ratio, up, r);
+                        const T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+                        const T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+                        const T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+                        T q = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+                        if (q < T(0)) q = T(0);
+                        pop[i] = q;
+                        if (partial && interior) {
+                            const int inv = D2INV[i];
+                            const T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+                            const T Fx = cur_f * (T(D2CX[inv]) - up[0]) - q * (T(D2CX[i]) - up[0]);
+                            const T Fy = cur_f * (T(D2CY[inv]) - up[1]) - q * (T(D2CY[i]) - up[1]);
+                            lsid[i] = sid;
+                            lf[i][0] = Fx;
+                            lf[i][1] = Fy;
+                            lf[i][2] = (r[0] * Fy - r[1] * Fx) * grid_length;
+                            mine |= 1ull << sid;
+                        }
+                    } else {
+                        const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
+                        T m[6];
+#pragma unroll
+                        for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                        T q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+                        if (q < T(0)) q = T(0);
+                        pop[i] = q;
+                    }
+                }
+                T Fx = 0, Fy = 0, out[6];
+                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+                d2q9_collide<T>(a, pop, Fx, Fy, out);
+                if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
+                const int64_t d = (int64_t)x + (int64_t)nx * y;
+#pragma unroll
+                for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+            }
+        }
+        if (partial) {
+            // per solid: this lane's links in ascending order, then the wave
+            uint64_t uni = wave_or(mine);
+            while (uni) {
+                const int s = __builtin_ctzll(uni);
+                uni &= uni - 1;
+                double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int i = 1; i < 9; ++i)
+                    if (lsid[i] == s) {
+                        v[0] += (double)lf[i][0];
+                        v[1] += (double)lf[i][1];
+                        v[2] += (double)lf[i][2];
+                    }
+                const double fx = wave_sum(v[0]), fy = wave_sum(v[1]), tz = wave_sum(v[2]);
+                if (lane == 0) {
+                    acc[s * 6 + 0] += fx;
+                    acc[s * 6 + 1] += fy;
+                    acc[s * 6 + 5] += tz;
+                }
+            }
+        }
+    }
+    if (!partial) return;
+    __syncthreads();
+    for (int k = lane; k < width; k += 64) partial[(int64_t)blockIdx.x * width + k] = acc[k];
+}
+
+template <typename T, int DT>
+hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, double* partial, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, 0, s, a, p, bbox, partial);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t polygon_workspace_bytes() { return 256 + (size_t)kPolyWaves * HL_MAX_SOLIDS * 6 * sizeof(double); }
+
+hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* verts, const float* verts_last,
+                         const int32_t* counts, int maxv, const float* radius2, double* force_history, void* ws,
+                         hipStream_t s)
+{
+    int* bbox = static_cast<int*>(ws);
+    double* partial = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
+    PolyArgs p{verts, verts_last, counts, maxv, radius2};
+    hipLaunchKernelGGL(k_poly_bbox, 1, 64, 0, s, a0, p, bbox);
+    hipError_t e = hipGetLastError();
+    const int width = a0.nsolid * 6;
+    for (int k = 0; k < nsteps && e == hipSuccess; ++k) {
+        EnvArgs a = a0;
+        if (k & 1) {
+            a.src = a0.dst;
+            a.dst = const_cast<void*>(a0.src);
+        }
+        a.mask = nullptr;
+        a.cut_bits = nullptr;
+        a.cut_cols = nullptr;
+        a.box[0] = a.box[2] = a.box[4] = 1 << 29;
+        a.box[1] = a.box[3] = a.box[5] = -(1 << 29);
+        // steps after the first see the same vertices twice (bodies at rest)
+        p.verts_last = k == 0 ? verts_last : verts;
+        double* fout = force_history && width > 0 ? force_history + (int64_t)k * width : nullptr;
+        e = step_gather(HL_D2Q9, dtype, a, s);
+        if (e != hipSuccess) break;
+        switch (dtype) {
+        case HL_F32: e = launch_band<float, HL_F32>(a, p, bbox, fout ? partial : nullptr, s); break;
+        case HL_F64: e = launch_band<double, HL_F64>(a, p, bbox, fout ? partial : nullptr, s); break;
+        case HL_F16: e = launch_band<float, HL_F16>(a, p, bbox, fout ? partial : nullptr, s); break;
+        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, bbox, fout ? partial : nullptr, s); break;
+        default: e = hipErrorInvalidValue;
+        }
+        if (e == hipSuccess && fout) e = sum_partials(partial, kPolyWaves, width, fout, s);
+    }
+    return e;
+}
+
+}  // namespace hl

@@ -213,6 +213,41 @@ class Lattice:
         N.check(self.lib.hl_step(C.byref(e), self.cur, int(nsteps), int(kernel), fh, _stream_ptr(stream)), "hl_step")
         self.cur ^= nsteps & 1
 
+    def step_polygons(self, nsteps: int, verts: torch.Tensor, verts_last: Optional[torch.Tensor],
+                      counts: torch.Tensor, radius2: torch.Tensor, force_history: Optional[torch.Tensor] = None,
+                      stream=None) -> None:
+        """Advance nsteps (D2Q9) with the exact polygon cut cells
+        (get_cutcell, envs/lbm/lbm_func.py:545-607) instead of the mask:
+        verts / verts_last device FP32 [nsolid, maxv, 2] transformed vertices of
+        this and the previous step, counts device int32 [nsolid], radius2
+        device FP32 [nsolid] squared bounding radii (lattice units)."""
+        if self.lattice != 9:
+            raise ValueError("polygon cut cells are D2Q9 only")
+        if self.nsolid < 1:
+            raise ValueError("polygon cut cells need at least one solid")
+        vs = [verts, counts, radius2] + ([verts_last] if verts_last is not None else [])
+        for t, dt in zip(vs, (torch.float32, torch.int32, torch.float32, torch.float32)):
+            if t.dtype != dt or not t.is_contiguous() or t.device != self.device:
+                raise ValueError("verts / verts_last / radius2 FP32 and counts int32, contiguous, on the lattice's device")
+        if verts.dim() != 3 or verts.shape[0] != self.nsolid or verts.shape[2] != 2:
+            raise ValueError(f"verts must be [nsolid={self.nsolid}, maxv, 2], got {tuple(verts.shape)}")
+        if verts_last is not None and verts_last.shape != verts.shape:
+            raise ValueError("verts_last must have the shape of verts")
+        if counts.numel() != self.nsolid or radius2.numel() != self.nsolid:
+            raise ValueError("counts and radius2 need one entry per solid")
+        fh = None
+        if force_history is not None:
+            if (force_history.dtype != torch.float64 or not force_history.is_contiguous()
+                    or force_history.numel() < nsteps * self.nsolid * 6 or force_history.device != self.device):
+                raise ValueError("force_history must be a contiguous FP64 device tensor [nsteps, nsolid, 6]")
+            fh = force_history.data_ptr()
+        e = self.env()
+        N.check(self.lib.hl_step_polygons(C.byref(e), self.cur, int(nsteps), verts.data_ptr(),
+                                          verts_last.data_ptr() if verts_last is not None else None,
+                                          counts.data_ptr(), int(verts.shape[1]), radius2.data_ptr(), fh,
+                                          _stream_ptr(stream)), "hl_step_polygons")
+        self.cur ^= nsteps & 1
+
     def capture(self, nsteps: int, kernel: int = N.HL_KERNEL_AUTO,
                 force_history: Optional[torch.Tensor] = None) -> "StepGraph":
         """Record `nsteps` steps into hipGraphs (LBM_Solver.step's graph,

@@ -10,7 +10,9 @@ Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
   step voxelizes them into an integer mask with the reference's even-odd
   inside test, and the wall rule runs on mask links (mask(x) != mask(x-c_i))
   instead of segment-link intersections (get_cutcell, lbm_func.py:545-607).
-  Capsule bodies (add_capsule) are an extension.
+  LBM_Solver(..., cut_cell=True) (or HomeFlow.cut_cell = True) runs the
+  reference's own segment-link predicate and material-point wall rule
+  instead (hl_step_polygons). Capsule bodies (add_capsule) are an extension.
 * Forces and torques are reduced in FP64 in a fixed order and are
   deterministic; solid_forcex / solid_forcey / torque hold them as FP32 like
   the reference's atomics.
@@ -92,6 +94,11 @@ class HomeFlow:
         self._at_rest = False  # uploaded table has last == current and the solids are unchanged since
         self._forces = torch.zeros((1, max(n_objects, 1), 6), dtype=torch.float64, device=L.device)
         self._box = (0, 0, 0, 0, 0, 0)
+        # cut_cell: the reference's segment-link cut cells (get_cutcell) instead of mask links
+        self.cut_cell = False
+        self._seg_prev: Optional[np.ndarray] = None  # last step's transformed vertices (zeros before the first)
+        self._seg_host = None  # (verts, last, counts, radius2) last uploaded
+        self._seg_dev = None
 
     def _mark_dirty(self):
         self._solid_dirty = True
@@ -303,6 +310,8 @@ class HomeFlow:
             self._uploaded = table
         self._at_rest = bool(np.array_equal(now, last))
         self._last_pose = now
+        if self.cut_cell:
+            self._prepare_segments()
         if not self._solid_dirty:
             return
         verts, num = self._transformed()
@@ -327,18 +336,61 @@ class HomeFlow:
             L.voxelize(np.array(caps, np.float32), stream=stream)
         self._solid_dirty = False
 
+    def _prepare_segments(self) -> None:
+        """Device tables of the exact cut-cell step: this step's transformed
+        vertices and the previous step's, as precompute_transformed_segments
+        keeps them (lbm_func.py:509-542: a vertex whose stored position is
+        exactly (0, 0) counts as new, so its last position is the new one),
+        and the squared bounding radii (solid_max_radius * scale) * scale in
+        FP32 with scale = solid_scale * nx (lbm_func.py:569-574)."""
+        if self._capsules:
+            raise ValueError("cut_cell needs polygon bodies (configure_solid), not capsules")
+        f32 = np.float32
+        verts, num = self._transformed()
+        prev = self._seg_prev
+        if prev is None or prev.shape != verts.shape:
+            grown = np.zeros_like(verts)
+            if prev is not None:
+                k = min(prev.shape[1], verts.shape[1])
+                grown[:, :k] = prev[:, :k]
+            prev = grown
+        last = prev.copy()
+        for k in range(verts.shape[0]):
+            n = int(num[k])
+            fresh = (prev[k, :n, 0] == 0.0) & (prev[k, :n, 1] == 0.0)
+            last[k, :n] = np.where(fresh[:, None], verts[k, :n], prev[k, :n])
+            prev[k, :n] = verts[k, :n]
+        self._seg_prev = prev
+        S = self._solid
+        scale = S["solid_scale"].numpy().astype(f32) * f32(self.nx)
+        r2 = (S["solid_max_radius"].numpy().astype(f32) * scale) * scale
+        n = self.n_objects
+        host = (np.ascontiguousarray(verts[:n], f32), np.ascontiguousarray(last[:n], f32),
+                np.ascontiguousarray(num[:n], np.int32), np.ascontiguousarray(r2[:n], f32))
+        old = self._seg_host
+        if old is not None and all(a.shape == b.shape and np.array_equal(a, b) for a, b in zip(host, old)):
+            return
+        dev = self.lattice.device
+        self._seg_dev = tuple(torch.from_numpy(h.copy()).to(dev) for h in host)
+        self._seg_host = host
+
     def step(self, stream=None) -> None:
         """One time step with the per-solid force of the state it reads."""
         self._prepare(stream)
         L = self.lattice
-        L.step(1, N.HL_KERNEL_AUTO, force_history=self._forces if self.n_objects > 0 else None, stream=stream)
+        fh = self._forces if self.n_objects > 0 else None
+        if self.cut_cell and self.n_objects > 0:
+            v, vl, cnt, r2 = self._seg_dev
+            L.step_polygons(1, v, vl, cnt, r2, force_history=fh, stream=stream)
+            return
+        L.step(1, N.HL_KERNEL_AUTO, force_history=fh, stream=stream)
 
 
 class LBM_Solver:
     """Worlds of one size stepped together (envs/lbm/lbm_solver.py:21-141)."""
 
     def __init__(self, nx: int, ny: int, solid_num: int = 1, nworld: int = 1, device: Optional[Any] = None,
-                 dtype: str = "f32", batched: bool = True) -> None:
+                 dtype: str = "f32", batched: bool = True, cut_cell: bool = False) -> None:
         if nworld < 1:
             raise ValueError("nworld must be >= 1")
         self._use_batch = bool(batched)
@@ -358,6 +410,7 @@ class LBM_Solver:
         self.flows = [HomeFlow(dtype=dtype, device=self.device) for _ in range(nworld)]
         for flow in self.flows:
             flow.Initialize(nx, ny, n_objects=solid_num)
+            flow.cut_cell = bool(cut_cell)
         self.flows_wp = self.flows
         self.mujoco_mappings = {}
         self.mujoco_origins_wp = None
@@ -370,6 +423,7 @@ class LBM_Solver:
         from .batch import MAX_SOLIDS
         return self._use_batch and len(self.flows) > 1 and all(
             f.lattice.dtype == "f32" and f.lattice.force is None and f.lattice.nsolid <= MAX_SOLIDS
+            and not f.cut_cell
             for f in self.flows)
 
     @staticmethod

@@ -77,6 +77,25 @@ void hlo_voxelize(const hlo_env* e, const float* caps, int32_t ncap,
 void hlo_voxelize_polygons(const hlo_env* e, const float* verts, const int32_t* counts, int32_t npoly,
                            int32_t maxv, const int32_t box[6], uint8_t* mask);
 
+/* Polygon bodies for the exact cut-cell step (2D). verts / verts_last:
+ * [nsolid][maxv][2] transformed vertices of this step and of the previous
+ * one (precompute_transformed_segments, envs/lbm/lbm_func.py:509-542),
+ * counts[nsolid] vertices each; radius2[nsolid] = the squared bounding
+ * radius in lattice units, (solid_max_radius * scale) * scale in FP32. */
+typedef struct hlo_polys {
+    const float* verts;
+    const float* verts_last;
+    const int32_t* counts;
+    int32_t maxv;
+    const float* radius2;
+} hlo_polys;
+
+/* One D2Q9 step with the reference's segment-link predicate (get_cutcell)
+ * and material-point wall rule instead of the mask rule; forces (nullable,
+ * [nsolid][6] FP64, Fx, Fy at 0, 1 and the torque at 5) of the state read. */
+void hlo_step_polygons_f32(const hlo_env* e, const hlo_polys* p, const float* src, float* dst, double* forces);
+void hlo_step_polygons_f64(const hlo_env* e, const hlo_polys* p, const double* src, double* dst, double* forces);
+
 /* FP64 sum of rho over all cells. */
 double hlo_mass_f32(const hlo_env* e, const float* src);
 

@@ -41,6 +41,16 @@ class _Env(C.Structure):
     ]
 
 
+class _Polys(C.Structure):
+    _fields_ = [
+        ("verts", C.c_void_p),
+        ("verts_last", C.c_void_p),
+        ("counts", C.c_void_p),
+        ("maxv", C.c_int32),
+        ("radius2", C.c_void_p),
+    ]
+
+
 _lib = None
 
 
@@ -63,6 +73,9 @@ def lib():
         L.hlo_recon_f64.restype = C.c_double
         L.hlo_voxelize.argtypes = [C.POINTER(_Env), P, C.c_int32, P, P]
         L.hlo_voxelize_polygons.argtypes = [C.POINTER(_Env), P, P, C.c_int32, C.c_int32, P, P]
+        for name in ("hlo_step_polygons_f32", "hlo_step_polygons_f64"):
+            getattr(L, name).argtypes = [C.POINTER(_Env), C.POINTER(_Polys), P, P, P]
+            getattr(L, name).restype = None
         L.hlo_mass_f32.argtypes = [C.POINTER(_Env), P]
         L.hlo_mass_f32.restype = C.c_double
         _lib = L
@@ -175,6 +188,30 @@ def compute_forces(env: Env, src: np.ndarray, precision: str = "f32") -> np.ndar
     fn = lib().hlo_forces_f32 if precision == "f32" else lib().hlo_forces_f64
     fn(C.byref(e), src.ctypes.data, out.ctypes.data)
     return out[: e.nsolid]
+
+
+def step_polygons(env: Env, src: np.ndarray, verts: np.ndarray, verts_last: np.ndarray, counts: Sequence[int],
+                  radius2: Sequence[float], precision: str = "f32"):
+    """One D2Q9 step with the reference's segment-link cut cells
+    (get_cutcell, lbm_func.py:545-607): verts / verts_last [nsolid, maxv, 2]
+    FP32 transformed vertices of this and the previous step, counts, radius2
+    the squared bounding radii. Returns (state, forces [nsolid, 6]) with the
+    forces of the state read. env.mask is ignored."""
+    dtype = np.float32 if precision == "f32" else np.float64
+    src = _check(env, src, dtype)
+    dst = np.empty_like(src)
+    e = env.c_env()
+    e.mask = None
+    v = np.ascontiguousarray(verts, np.float32)
+    vl = np.ascontiguousarray(verts_last, np.float32)
+    assert v.shape == vl.shape and v.ndim == 3 and v.shape[0] == e.nsolid and v.shape[2] == 2
+    cnt = np.ascontiguousarray(counts, np.int32)
+    r2 = np.ascontiguousarray(radius2, np.float32)
+    pol = _Polys(v.ctypes.data, vl.ctypes.data, cnt.ctypes.data, v.shape[1], r2.ctypes.data)
+    out = np.zeros((max(e.nsolid, 1), 6), dtype=np.float64)
+    fn = lib().hlo_step_polygons_f32 if precision == "f32" else lib().hlo_step_polygons_f64
+    fn(C.byref(e), C.byref(pol), src.ctypes.data, dst.ctypes.data, out.ctypes.data)
+    return dst, out[: e.nsolid]
 
 
 def init_uniform(env: Env, u: Sequence[float], precision: str = "f32") -> np.ndarray:

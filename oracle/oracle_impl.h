@@ -535,6 +535,147 @@ void FN(hlo_init_uniform)(const hlo_env* e, R ux, R uy, R uz, R* dst)
     }
 }
 
+/* ------------------------------------------------- polygon cut cells (2D) */
+
+static int FN(poly_count)(const hlo_polys* p, int s)
+{
+    int n = p->counts[s];
+    return n < 0 ? 0 : (n > p->maxv ? p->maxv : n);
+}
+
+/* get_cutcell, envs/lbm/lbm_func.py:545-607: the crossing nearest to a of
+ * the link a = (x - dx, y - dy) -> b = (x, y) with a segment c -> d of the
+ * solids whose bounding circle holds b (a solid is skipped when
+ * |b - position|^2 > radius2). Returns the solid (-1: no crossing), the
+ * position along c -> d and the index of c. */
+static int FN(poly_cut)(const hlo_env* e, const hlo_polys* p, R x, R y, R dx, R dy, R* ratio, int* seg)
+{
+    R cutcell = -1, sr = -1;
+    int sid = -1, sg = -1;
+    const R ax = x - dx, ay = y - dy, bx = x, by = y;
+    for (int s = 0; s < e->nsolid; ++s) {
+        const R ox = bx - (R)e->solids[8 * s], oy = by - (R)e->solids[8 * s + 1];
+        const R d2 = ox * ox + oy * oy;
+        if (d2 > (R)p->radius2[s]) continue;
+        const int num = FN(poly_count)(p, s);
+        const float* v = p->verts + (int64_t)s * p->maxv * 2;
+        for (int n = 0; n < num; ++n) {
+            const int m = (n + 1) % num;
+            const R cx = v[2 * n], cy = v[2 * n + 1], ddx = v[2 * m], ddy = v[2 * m + 1];
+            const R abc = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
+            const R abd = (ax - ddx) * (by - ddy) - (ay - ddy) * (bx - ddx);
+            if (abc * abd >= (R)0) continue;
+            const R cda = (cx - ax) * (ddy - ay) - (cy - ay) * (ddx - ax);
+            const R cdb = cda + abc - abd;
+            if (cda * cdb > (R)0) continue;
+            const R t = (R)1 - cda / (abd - abc);
+            if (cutcell < (R)0 || t < cutcell) {
+                cutcell = t;
+                sid = s;
+                sr = abc / (abc - abd);
+                sg = n;
+            }
+        }
+    }
+    if (cutcell >= (R)0 && sid >= 0) {
+        *ratio = sr;
+        *seg = sg;
+        return sid;
+    }
+    return -1;
+}
+
+/* The cut branch of stream_and_collide (lbm_func.py:346-401): the crossing
+ * as a material point of segment seg, its velocity from this step's and the
+ * previous step's vertices (time step 1), the lever arm from the solid's
+ * position. */
+static void FN(poly_point)(const hlo_env* e, const hlo_polys* p, int sid, int seg, R ratio, R up[2], R r[2])
+{
+    const int num = FN(poly_count)(p, sid), m = (seg + 1) % num;
+    const float* v = p->verts + (int64_t)sid * p->maxv * 2;
+    const float* l = p->verts_last + (int64_t)sid * p->maxv * 2;
+    const R pcx = (R)v[2 * seg] + ((R)v[2 * m] - (R)v[2 * seg]) * ratio;
+    const R pcy = (R)v[2 * seg + 1] + ((R)v[2 * m + 1] - (R)v[2 * seg + 1]) * ratio;
+    const R plx = (R)l[2 * seg] + ((R)l[2 * m] - (R)l[2 * seg]) * ratio;
+    const R ply = (R)l[2 * seg + 1] + ((R)l[2 * m + 1] - (R)l[2 * seg + 1]) * ratio;
+    up[0] = pcx - plx;
+    up[1] = pcy - ply;
+    r[0] = pcx - (R)e->solids[8 * sid];
+    r[1] = pcy - (R)e->solids[8 * sid + 1];
+}
+
+void FN(hlo_step_polygons)(const hlo_env* e, const hlo_polys* p, const R* src, R* dst, double* forces)
+{
+    const int64_t st = e->stride;
+    const int nx = e->nx, ny = e->ny;
+    const R grid_length = (R)1 / (R)nx;
+    if (forces)
+        for (int k = 0; k < 6 * e->nsolid; ++k) forces[k] = 0.0;
+    for (int y = 1; y < ny - 1; ++y)
+        for (int x = 1; x < nx - 1; ++x) {
+            const int64_t c = (int64_t)x + (int64_t)nx * y;
+            R pop[9];
+            const R rho_c = src[c], ux_c = src[st + c], uy_c = src[2 * st + c];
+            const R Sxx_c = src[3 * st + c], Syy_c = src[4 * st + c], Sxy_c = src[5 * st + c];
+            for (int i = 0; i < 9; ++i) {
+                R ratio = 0;
+                int seg = -1;
+                const int sid = FN(poly_cut)(e, p, (R)x, (R)y, (R)D2_CX[i], (R)D2_CY[i], &ratio, &seg);
+                if (sid >= 0) {
+                    R up[2], r[2];
+                    FN(poly_point)(e, p, sid, seg, ratio, up, r);
+                    R Sxx_s = up[0] * up[0] + Sxx_c - ux_c * ux_c;
+                    R Syy_s = up[1] * up[1] + Syy_c - uy_c * uy_c;
+                    R Sxy_s = up[0] * up[1] + Sxy_c - ux_c * uy_c;
+                    pop[i] = FN(d2q9_recon)(rho_c, up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+                    if (pop[i] < (R)0) pop[i] = 0;
+                    const int inv = D2_INV[i];
+                    R cur_f = FN(d2q9_recon)(rho_c, ux_c, uy_c, Sxx_c, Syy_c, Sxy_c, inv);
+                    R Fx = cur_f * ((R)D2_CX[inv] - up[0]) - pop[i] * ((R)D2_CX[i] - up[0]);
+                    R Fy = cur_f * ((R)D2_CY[inv] - up[1]) - pop[i] * ((R)D2_CY[i] - up[1]);
+                    R Tz = (r[0] * Fy - r[1] * Fx) * grid_length;
+                    if (forces) {
+                        forces[6 * sid] += (double)Fx;
+                        forces[6 * sid + 1] += (double)Fy;
+                        forces[6 * sid + 5] += (double)Tz;
+                    }
+                } else {
+                    const int64_t s = (int64_t)(x - D2_CX[i]) + (int64_t)nx * (y - D2_CY[i]);
+                    pop[i] = FN(d2q9_recon)(src[s], src[st + s], src[2 * st + s], src[3 * st + s],
+                                            src[4 * st + s], src[5 * st + s], i);
+                    if (pop[i] < (R)0) pop[i] = 0;
+                }
+            }
+            R Fx = 0, Fy = 0;
+            if (e->force) { Fx = (R)e->force[c]; Fy = (R)e->force[st + c]; }
+            R rhoVar = pop[0] + pop[1] + pop[2] + pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+            R invRho = (R)1 / rhoVar;
+            R ux = (pop[1] - pop[2] + pop[5] - pop[6] - pop[7] + pop[8] + (R)0.5 * Fx) * invRho;
+            R uy = (pop[3] - pop[4] + pop[5] + pop[6] - pop[7] - pop[8] + (R)0.5 * Fy) * invRho;
+            R pixx = pop[1] + pop[2] + pop[5] + pop[6] + pop[7] + pop[8];
+            R piyy = pop[3] + pop[4] + pop[5] + pop[6] + pop[7] + pop[8];
+            R pixy = pop[5] + pop[7] - pop[6] - pop[8];
+            R Omega = (R)1 / (((R)e->nu) * (R)3 + (R)0.5);
+            R Ru2 = rhoVar * ux * ux;
+            R Rv2 = rhoVar * uy * uy;
+            R pixx_part = (pixx - piyy) / (R)2;
+            R piyy_part = (piyy - pixx) / (R)2;
+            R one = 1;
+            pixy = (one - Omega) * pixy + ux * uy * rhoVar * Omega + (one - (R)0.5 * Omega) * (Fy * ux + Fx * uy);
+            R pixx_n = rhoVar / (R)3 + pixx_part * (one - Omega) + (one + Omega) / (R)2 * Ru2
+                     + (one - Omega) / (R)2 * Rv2 + Fx * ux + (one - Omega) / (R)2 * (Fx * ux - Fy * uy);
+            R piyy_n = rhoVar / (R)3 + piyy_part * (one - Omega) + (one + Omega) / (R)2 * Rv2
+                     + (one - Omega) / (R)2 * Ru2 + Fy * uy + (one - Omega) / (R)2 * (Fy * uy - Fx * ux);
+            dst[c] = rhoVar;
+            dst[st + c] = ux + Fx * invRho / (R)2;
+            dst[2 * st + c] = uy + Fy * invRho / (R)2;
+            dst[3 * st + c] = pixx_n * invRho - FN(cs2_2d);
+            dst[4 * st + c] = piyy_n * invRho - FN(cs2_2d);
+            dst[5 * st + c] = pixy * invRho;
+        }
+    FN(bc_passes)(e, dst);
+}
+
 R FN(hlo_recon)(int lattice, const R* m, int i)
 {
     if (lattice == 9) return FN(d2q9_recon)(m[0], m[1], m[2], m[3], m[4], m[5], i);

@@ -1,0 +1,150 @@
+"""Exact polygon cut cells on the GPU (hl_step_polygons) against the oracle's
+hlo_step_polygons: moving polygons (translation and rotation, a body close to
+a face), several steps per call, FP32 and FP64, body forces, and the
+LBM_Solver(cut_cell=True) facade."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as orc
+from openhomelbm_amd.lattice import Lattice
+from openhomelbm_amd.lbm import LBM_Solver
+from tests.helpers import random_state
+
+pytestmark = pytest.mark.gpu
+
+NX, NY = 64, 48
+BC_TYPE = (0, 1, 1, 2)
+BC_VALUE = ((0.04, 0.0, 0.0),) + ((0.0, 0.0, 0.0),) * 5
+
+
+def _poly(center, radius, n, angle):
+    a = angle + 2 * np.pi * np.arange(n) / n
+    return np.stack([center[0] + radius * np.cos(a), center[1] + radius * np.sin(a) * 0.7], 1).astype(np.float32)
+
+
+def _bodies(k):
+    """Two polygons at step k: a pentagon moving and turning, a triangle
+    resting two cells above the bottom face."""
+    pent = _poly((24.3 + 0.35 * k, 23.7 - 0.2 * k), 7.1, 5, 0.3 + 0.05 * k)
+    tri = np.array([[40.2, 1.6], [51.7, 2.9], [45.1, 9.4]], np.float32)
+    v = np.zeros((2, 6, 2), np.float32)
+    v[0, :5], v[1, :3] = pent, tri
+    pos = np.array([[24.3 + 0.35 * k, 23.7 - 0.2 * k], [45.6, 4.6]], np.float32)
+    return v, np.array([5, 3], np.int32), pos
+
+
+R2 = np.array([(7.1 + 1.5) ** 2, 8.5 ** 2], np.float32)
+
+
+def _env(pos, pos_last, dtype=np.float32):
+    env = orc.Env(9, NX, NY, nu=0.015, bc_type=BC_TYPE, bc_value=BC_VALUE)
+    P = np.zeros((2, 8), np.float32)
+    P[:, 0:2], P[:, 2] = pos, 1.0
+    P[:, 4:6], P[:, 6] = pos_last, 1.0
+    env.solids = P
+    return env
+
+
+def _lattice(dtype="f32"):
+    lat = Lattice(9, NX, NY, dtype=dtype, nsolid=2)
+    lat.nu = 0.015
+    lat.bc_type = list(BC_TYPE) + [1, 1]
+    lat.bc_value = [list(v) for v in BC_VALUE]
+    return lat
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_moving_polygons_match_oracle(dtype):
+    prec = np.float32 if dtype == "f32" else np.float64
+    m = random_state(9, (NY, NX), seed=11).astype(prec)
+    lat = _lattice(dtype)
+    lat.set_state(m)
+    ref = m
+    hist = torch.zeros((1, 2, 6), dtype=torch.float64, device="cuda")
+    v_prev, _, pos_prev = _bodies(0)
+    for k in range(4):
+        v, cnt, pos = _bodies(k)
+        env = _env(pos, pos_prev)
+        lat.set_poses(env.solids)
+        lat.step_polygons(1, _dev(v), _dev(v_prev), _dev(cnt), _dev(R2), force_history=hist)
+        ref, F = orc.step_polygons(env, ref, v, v_prev, cnt, R2, precision=dtype)
+        torch.cuda.synchronize()
+        got = lat.get_state(prec)
+        assert np.array_equal(got, ref), (k, np.abs(got - ref).max())
+        np.testing.assert_allclose(hist[0].cpu().numpy(), F, rtol=1e-10, atol=1e-13)
+        assert np.abs(F[:, [0, 1, 5]]).min() > 0
+        v_prev, pos_prev = v, pos
+
+
+def test_several_steps_per_call():
+    """verts_last applies to the first step of the call, later steps see the
+    bodies at rest; each step's forces land in its history row."""
+    m = random_state(9, (NY, NX), seed=12).astype(np.float32)
+    lat = _lattice()
+    lat.set_state(m)
+    v0, _, pos0 = _bodies(0)
+    v, cnt, pos = _bodies(1)
+    env = _env(pos, pos0)
+    lat.set_poses(env.solids)
+    hist = torch.zeros((3, 2, 6), dtype=torch.float64, device="cuda")
+    lat.step_polygons(3, _dev(v), _dev(v0), _dev(cnt), _dev(R2), force_history=hist)
+    torch.cuda.synchronize()
+    ref, Fs = m, []
+    for k in range(3):
+        ref, F = orc.step_polygons(env, ref, v, v0 if k == 0 else v, cnt, R2)
+        Fs.append(F)
+    assert np.array_equal(lat.get_state(np.float32), ref)
+    np.testing.assert_allclose(hist.cpu().numpy(), np.array(Fs), rtol=1e-10, atol=1e-13)
+
+
+def test_step_polygons_argument_errors():
+    lat = _lattice()
+    v, cnt, _ = _bodies(0)
+    with pytest.raises(ValueError):
+        lat.step_polygons(1, _dev(v[:1]), None, _dev(cnt), _dev(R2))  # one solid short
+    with pytest.raises(ValueError):
+        lat.step_polygons(1, _dev(v), None, _dev(cnt.astype(np.int64)), _dev(R2))
+    lat3 = Lattice(27, 8, 8, 8, nsolid=1)
+    with pytest.raises(ValueError):
+        lat3.step_polygons(1, _dev(v[:1]), None, _dev(cnt[:1]), _dev(R2[:1]))
+
+
+def test_lbm_solver_cut_cell_matches_oracle():
+    """LBM_Solver(cut_cell=True): configure_solid polygons stepped with the
+    segment predicate; the facade's vertex tables follow
+    precompute_transformed_segments (last = previous step's vertices)."""
+    s = LBM_Solver(NX, NY, solid_num=1, cut_cell=True)
+    f = s.flows[0]
+    f.vis_shear = 0.015
+    f.bc_type = list(BC_TYPE)
+    f.bc_value = [[0.04, 0.0], [0, 0], [0, 0], [0, 0]]
+    m = random_state(9, (NY, NX), seed=13).astype(np.float32)
+    f.lattice.set_state(m)
+    shape = np.array([[-0.1, -0.05], [0.12, -0.06], [0.08, 0.07], [-0.09, 0.05]], np.float32)
+    f.configure_solid(0, shape, position=(22.0, 24.0), angle=0.2, scale=0.3)
+    ref, prev = m, None
+    for k in range(4):
+        if k >= 2:
+            f.solid_position = [[22.0 + 0.3 * (k - 1), 24.0 - 0.1 * (k - 1)]]
+            f.solid_angle = [0.2 + 0.04 * (k - 1)]
+        s.step()
+        verts, num = f._transformed()
+        v = verts[0, : num[0]].copy()
+        last = v if prev is None else prev
+        pos = f.solid_position.numpy().astype(np.float32)  # the lever arm's origin (the pose table's position)
+        sc = np.float32(0.3) * np.float32(NX)
+        r2 = (np.float32(f.solid_max_radius.numpy()[0]) * sc) * sc
+        env = orc.Env(9, NX, NY, nu=0.015, bc_type=BC_TYPE, bc_value=((0.04, 0, 0),) + ((0, 0, 0),) * 5)
+        env.solids = np.array([[pos[0, 0], pos[0, 1], 1, 0, pos[0, 0], pos[0, 1], 1, 0]], np.float32)
+        ref, F = orc.step_polygons(env, ref, v[None], last[None], [num[0]], [r2])
+        torch.cuda.synchronize()
+        assert np.array_equal(f.lattice.get_state(np.float32), ref), k
+        got = [f.solid_forcex.numpy()[0], f.solid_forcey.numpy()[0], f.torque.numpy()[0]]
+        np.testing.assert_allclose(got, F[0, [0, 1, 5]], rtol=1e-6, atol=1e-9)
+        prev = v
+    assert abs(F[0, 0]) > 0
