@@ -145,11 +145,20 @@ def bench_karman_speed(K: int = 2000, W: int = 50):
     gn = lat.capture(n, force_history=h[:n])
     gn.replay()
     graphn = timed(lambda: [gn.replay() for _ in range(K // n)]) * K / (K // n * n)
+    # the reference's own boundary: the disc as a 96-segment polygon with the
+    # segment-link predicate (hl_step_polygons), one call for K steps
+    ang = 2 * np.pi * np.arange(96) / 96
+    v = torch.from_numpy(np.stack([96 + 8 * np.cos(ang), 65 + 8 * np.sin(ang)], 1)[None].astype(np.float32)).cuda()
+    cnt = torch.tensor([96], dtype=torch.int32, device="cuda")
+    r2 = torch.tensor([(8 + 2 ** 0.5) ** 2], dtype=torch.float32, device="cuda")
+    lat.step_polygons(W, v, v, cnt, r2, force_history=h[:W])
+    cut = timed(lambda: lat.step_polygons(K, v, v, cnt, r2, force_history=h))
     mlups = lambda ms: round(w.ncell / (ms * 1e-3) / 1e6, 1)
     return {"workload": "d2q9_karman_512x128", "steps": K, "us_per_step_per_call": round(per_call * 1e3, 2),
             "us_per_step_one_call": round(one_call * 1e3, 2), "us_per_step_graph1": round(graph1 * 1e3, 2),
             f"us_per_step_graph{n}": round(graphn * 1e3, 2), "mlups_per_call": mlups(per_call),
-            "mlups_one_call": mlups(one_call), "mlups_graph1": mlups(graph1), f"mlups_graph{n}": mlups(graphn)}
+            "mlups_one_call": mlups(one_call), "mlups_graph1": mlups(graph1), f"mlups_graph{n}": mlups(graphn),
+            "us_per_step_cut_cell_96seg": round(cut * 1e3, 2), "mlups_cut_cell_96seg": mlups(cut)}
 
 
 def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
