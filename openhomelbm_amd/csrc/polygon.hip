@@ -10,7 +10,8 @@
 //      per call by k_poly_bbox): cells whose interior projection lies in a
 //      circle are recomputed with the segment predicate and rewritten, face
 //      cells through the same face chain as the gather kernel; interior
-//      cells also emit the momentum exchange of their cut links;
+//      cells also emit the momentum exchange of their cut links. One lane
+//      per link (eight cells per wavefront pass), the vertex tables in LDS;
 //   3. k_sum_partials over the band's fixed number of per-wave FP64 rows.
 // The band kernel is a fixed grid of waves striding over the box, so the
 // FP64 force order does not depend on scheduling. Arithmetic follows the
@@ -62,7 +63,8 @@ __device__ __forceinline__ uint64_t poly_near(const EnvArgs& a, const PolyArgs& 
 // the near solids; returns the solid or -1, with the crossing's position
 // along its segment and the segment's first vertex.
 template <typename T>
-__device__ __forceinline__ int poly_cut(const PolyArgs& p, uint64_t near, int x, int y, int i, T* ratio, int* seg)
+__device__ __forceinline__ int poly_cut(const PolyArgs& p, const float* verts, uint64_t near, int x, int y, int i,
+                                       T* ratio, int* seg)
 {
     T cutcell = T(-1), sr = T(-1);
     int sid = -1, sg = -1;
@@ -72,7 +74,7 @@ __device__ __forceinline__ int poly_cut(const PolyArgs& p, uint64_t near, int x,
         const int s = __builtin_ctzll(near);
         near &= near - 1;
         const int num = poly_count(p, s);
-        const float* v = p.verts + (int64_t)s * p.maxv * 2;
+        const float* v = verts + (int64_t)s * p.maxv * 2;
         for (int n = 0; n < num; ++n) {
             const int m = n + 1 == num ? 0 : n + 1;
             const T cx = T(v[2 * n]), cy = T(v[2 * n + 1]), dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
@@ -152,17 +154,33 @@ __global__ void k_poly_bbox(EnvArgs a, PolyArgs p, int* bbox)
     bbox[3] = (int)y1 + 2;
 }
 
+// LDS for the vertex tables: larger tables are read from global memory.
+constexpr int kPolyLdsFloats = 8192;  // 32 KiB
+
+// One wavefront per workgroup, eight cells per pass: lane 8 c + (i - 1)
+// classifies link i of cell c (segment test against the near solids, wall
+// rule or pull) and keeps that link's momentum exchange; lane 8 c also pulls
+// the rest population, gathers the nine through LDS and collides. Force rows:
+// per solid a wavefront sum per pass, accumulated in pass order (FP64).
 template <typename T, int DT>
 __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial)
 {
+    extern __shared__ float lds_verts[];
     __shared__ double acc[HL_MAX_SOLIDS * 6];
+    __shared__ T pops[8][9];
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
     ST* dst = static_cast<ST*>(a.dst);
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, slot = lane >> 3, i = (lane & 7) + 1;
     const int nx = a.nx, ny = a.ny;
     const int width = a.nsolid * 6;
+    const int nv = a.nsolid * p.maxv * 2;
+    const float* verts = p.verts;
+    if (nv <= kPolyLdsFloats) {
+        for (int k = lane; k < nv; k += 64) lds_verts[k] = p.verts[k];
+        verts = lds_verts;
+    }
     if (partial)
         for (int k = lane; k < width; k += 64) acc[k] = 0.0;
     __syncthreads();
@@ -170,82 +188,84 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
     const int bw = bbox[1] - bbox[0], bh = bbox[3] - bbox[2];
     const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
     const T grid_length = T(1) / T(nx);
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
-        const int64_t t = base + lane;
-        int lsid[9];
-        T lf[9][3];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) lsid[i] = -1;
-        uint64_t mine = 0;
+    for (int64_t base = (int64_t)blockIdx.x * 8; base < n; base += (int64_t)gridDim.x * 8) {
+        const int64_t t = base + slot;
+        int sid = -1;
+        T F[3] = {T(0), T(0), T(0)};
+        bool work = false, interior = false;
+        int x = 0, y = 0;
+        int64_t c = 0;
         if (t < n) {
-            const int x = bx0 + (int)(t % bw), y = by0 + (int)(t / bw);
+            x = bx0 + (int)(t % bw);
+            y = by0 + (int)(t / bw);
             const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
             const uint64_t near = poly_near<T>(a, p, px, py);
             if (near) {
-                const bool interior = px == x && py == y;
-                const int64_t c = (int64_t)px + (int64_t)nx * py;
+                work = true;
+                interior = px == x && py == y;
+                c = (int64_t)px + (int64_t)nx * py;
                 T mc[6];
                 load_cell<T, DT>(a, c, 6, mc);
-                T pop[9];
-#pragma unroll
-                for (int i = 0; i < 9; ++i) {
-                    T ratio = 0;
-                    int seg = -1;
-                    const int sid = i == 0 ? -1 : poly_cut<T>(p, near, px, py, i, &ratio, &seg);
-                    if (sid >= 0) {
-                        T up[2], r[2];
-                        poly_point<T>(a, p, sid, seg, ratio, up, r);
-                        const T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
-                        const T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
-                        const T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
-                        T q = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
-                        if (q < T(0)) q = T(0);
-                        pop[i] = q;
-                        if (partial && interior) {
-                            const int inv = D2INV[i];
-                            const T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
-                            const T Fx = cur_f * (T(D2CX[inv]) - up[0]) - q * (T(D2CX[i]) - up[0]);
-                            const T Fy = cur_f * (T(D2CY[inv]) - up[1]) - q * (T(D2CY[i]) - up[1]);
-                            lsid[i] = sid;
-                            lf[i][0] = Fx;
-                            lf[i][1] = Fy;
-                            lf[i][2] = (r[0] * Fy - r[1] * Fx) * grid_length;
-                            mine |= 1ull << sid;
-                        }
+                T ratio = 0;
+                int seg = -1;
+                sid = poly_cut<T>(p, verts, near, px, py, i, &ratio, &seg);
+                T q;
+                if (sid >= 0) {
+                    T up[2], r[2];
+                    poly_point<T>(a, p, sid, seg, ratio, up, r);
+                    const T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+                    const T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+                    const T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+                    q = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+                    if (q < T(0)) q = T(0);
+                    if (partial && interior) {
+                        const int inv = D2INV[i];
+                        const T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+                        const T Fx = cur_f * (T(D2CX[inv]) - up[0]) - q * (T(D2CX[i]) - up[0]);
+                        const T Fy = cur_f * (T(D2CY[inv]) - up[1]) - q * (T(D2CY[i]) - up[1]);
+                        F[0] = Fx;
+                        F[1] = Fy;
+                        F[2] = (r[0] * Fy - r[1] * Fx) * grid_length;
                     } else {
-                        const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
-                        T m[6];
-#pragma unroll
-                        for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
-                        T q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
-                        if (q < T(0)) q = T(0);
-                        pop[i] = q;
+                        sid = -1;
                     }
-                }
-                T Fx = 0, Fy = 0, out[6];
-                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
-                d2q9_collide<T>(a, pop, Fx, Fy, out);
-                if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
-                const int64_t d = (int64_t)x + (int64_t)nx * y;
+                } else {
+                    const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
+                    T m[6];
 #pragma unroll
-                for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+                    for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                    q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+                    if (q < T(0)) q = T(0);
+                }
+                pops[slot][i] = q;
+                if (i == 1) {  // the rest population: never cut (its link has no length)
+                    T q0 = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], 0);
+                    if (q0 < T(0)) q0 = T(0);
+                    pops[slot][0] = q0;
+                }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (work && i == 1) {
+            T pp[9], Fx = 0, Fy = 0, out[6];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) pp[k] = pops[slot][k];
+            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+            d2q9_collide<T>(a, pp, Fx, Fy, out);
+            if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
+            const int64_t d = (int64_t)x + (int64_t)nx * y;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+        }
         if (partial) {
-            // per solid: this lane's links in ascending order, then the wave
-            uint64_t uni = wave_or(mine);
+            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
             while (uni) {
                 const int s = __builtin_ctzll(uni);
                 uni &= uni - 1;
-                double v[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-                for (int i = 1; i < 9; ++i)
-                    if (lsid[i] == s) {
-                        v[0] += (double)lf[i][0];
-                        v[1] += (double)lf[i][1];
-                        v[2] += (double)lf[i][2];
-                    }
-                const double fx = wave_sum(v[0]), fy = wave_sum(v[1]), tz = wave_sum(v[2]);
+                const bool m = sid == s;
+                const double fx = wave_sum(m ? (double)F[0] : 0.0), fy = wave_sum(m ? (double)F[1] : 0.0);
+                const double tz = wave_sum(m ? (double)F[2] : 0.0);
                 if (lane == 0) {
                     acc[s * 6 + 0] += fx;
                     acc[s * 6 + 1] += fy;
@@ -253,6 +273,9 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
                 }
             }
         }
+        // the lead lanes' reads of pops end before the next pass writes them
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __builtin_amdgcn_wave_barrier();
     }
     if (!partial) return;
     __syncthreads();
@@ -262,7 +285,9 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
 template <typename T, int DT>
 hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, double* partial, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, 0, s, a, p, bbox, partial);
+    const int nv = a.nsolid * p.maxv * 2;
+    const size_t lds = nv <= kPolyLdsFloats ? (size_t)nv * sizeof(float) : 0;
+    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, lds, s, a, p, bbox, partial);
     return hipGetLastError();
 }
 
