@@ -303,7 +303,7 @@ int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const f
     if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
     if (env->nsolid < 1) return fail(HL_E_INVALID, "polygon cut cells need nsolid >= 1");
     if (!verts || !counts || !radius2) return fail(HL_E_INVALID, "verts, counts and radius2 must be non-NULL");
-    if (maxv < 1) return fail(HL_E_INVALID, "maxv must be >= 1, got %d", maxv);
+    if (maxv < 1 || maxv > (1 << 20)) return fail(HL_E_INVALID, "maxv must be in [1, 2^20], got %d", maxv);
     if (nsteps == 0) return HL_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     StreamCtx* ctx = nullptr;

@@ -348,6 +348,8 @@ class Lattice:
         self._tsid_dev = torch.from_numpy(sid).to(self.device)
         if self._mesh_overflow is None:
             self._mesh_overflow = torch.zeros(1, dtype=torch.int32, device=self.device)
+        else:
+            self._mesh_overflow.zero_()  # reports this call only
         b = (C.c_int32 * 6)(*write)
         e = self.env()
         N.check(self.lib.hl_voxelize_mesh(C.byref(e), self.mask.data_ptr(), self._tris_dev.data_ptr(),
@@ -357,7 +359,8 @@ class Lattice:
         self._mask_dirty = tuple(self.mask_box)
 
     def mesh_overflowed(self) -> bool:
-        """True when a mesh voxelization met a ray with more than 64 crossings."""
+        """True when the last mesh voxelization met a ray with more than 64
+        crossings (the cells of that column are then not reliable)."""
         return bool(self._mesh_overflow is not None and int(self._mesh_overflow.item()))
 
     def voxelize_polygons(self, verts: np.ndarray, counts: Sequence[int], box=None, stream=None) -> None:
