@@ -279,6 +279,24 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
                 a.cut_cols = nullptr;
             }
         }
+        if (e == hipErrorNotSupported && env->lattice == HL_D2Q9 && fout && ws) {
+            // one launch: the gather step, this step's force rows and the sum
+            // of the previous step's rows
+            hl::FixArgs fx = hl::fixup_args(env->lattice, a, ws, true, k);
+            if (fx.list && fx.partial) {
+                fx.prev = pend.rows;
+                fx.prev_rows = pend.nrows;
+                fx.prev_out = pend.out;
+                e = hl::step_gather_forces(env->dtype, a, fx, s);
+                if (e == hipSuccess) {
+                    pend.rows = fx.partial;
+                    pend.nrows = fx.nfix;
+                    pend.width = fx.width;
+                    pend.out = fout;
+                    fixed = true;
+                }
+            }
+        }
         if (e == hipErrorNotSupported) {
             e = flush();
             if (e == hipSuccess) e = hl::step_gather(env->lattice, env->dtype, a, s);

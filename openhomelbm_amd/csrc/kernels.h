@@ -43,6 +43,9 @@ void set_tiling(int lattice, TileCfg c);
 TileCfg get_tiling(int lattice);
 
 hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
+// D2Q9 gather step that also writes fx.nfix (set here) force rows of the
+// state it reads into fx.partial and sums fx.prev into fx.prev_out.
+hipError_t step_gather_forces(int dtype, const EnvArgs& a, FixArgs& fx, hipStream_t s);
 // Returns hipErrorNotSupported when no tiled kernel exists for this case.
 // With fx (near-body list built by cut_list, a.cut_bits set by
 // cut_bits_view), the same launch also updates the listed cells and, with

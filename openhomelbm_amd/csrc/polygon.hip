@@ -12,7 +12,8 @@
 //      cells through the same face chain as the gather kernel; interior
 //      cells also emit the momentum exchange of their cut links. One lane
 //      per link (eight cells per wavefront pass), the vertex tables in LDS;
-//   3. k_sum_partials over the band's fixed number of per-wave FP64 rows.
+//   3. the sum of the band's fixed number of per-wave FP64 rows, done by the
+//      next step's band launch (k_sum_partials after the last step).
 // The band kernel is a fixed grid of waves striding over the box, so the
 // FP64 force order does not depend on scheduling. Arithmetic follows the
 // reference's operation order with contraction off.
@@ -163,7 +164,8 @@ constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 // the rest population, gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
 template <typename T, int DT>
-__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial)
+__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial,
+                                                   const double* prev, double* prev_out)
 {
     extern __shared__ float lds_verts[];
     __shared__ double acc[HL_MAX_SOLIDS * 6];
@@ -277,23 +279,33 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();
     }
+    // the previous step's rows: one wave per output, the order of k_sum_partials
+    if (prev)
+        for (int k = blockIdx.x; k < width; k += gridDim.x) {
+            double v = 0.0;
+            for (int r = lane; r < kPolyWaves; r += 64) v += prev[(int64_t)r * width + k];
+            v = wave_sum(v);
+            if (lane == 0) prev_out[k] = v;
+        }
     if (!partial) return;
     __syncthreads();
     for (int k = lane; k < width; k += 64) partial[(int64_t)blockIdx.x * width + k] = acc[k];
 }
 
 template <typename T, int DT>
-hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, double* partial, hipStream_t s)
+hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, double* partial, const double* prev,
+                       double* prev_out, hipStream_t s)
 {
     const int nv = a.nsolid * p.maxv * 2;
     const size_t lds = nv <= kPolyLdsFloats ? (size_t)nv * sizeof(float) : 0;
-    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, lds, s, a, p, bbox, partial);
+    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, lds, s, a, p, bbox, partial, prev, prev_out);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t polygon_workspace_bytes() { return 256 + (size_t)kPolyWaves * HL_MAX_SOLIDS * 6 * sizeof(double); }
+// bbox, then two banks of force rows (a step writes one while summing the other)
+size_t polygon_workspace_bytes() { return 256 + 2 * (size_t)kPolyWaves * HL_MAX_SOLIDS * 6 * sizeof(double); }
 
 hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* verts, const float* verts_last,
                          const int32_t* counts, int maxv, const float* radius2, double* force_history, void* ws,
@@ -305,6 +317,8 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
     hipLaunchKernelGGL(k_poly_bbox, 1, 64, 0, s, a0, p, bbox);
     hipError_t e = hipGetLastError();
     const int width = a0.nsolid * 6;
+    const double* prev = nullptr;  // the previous step's rows, summed by this step's band launch
+    double* prev_out = nullptr;
     for (int k = 0; k < nsteps && e == hipSuccess; ++k) {
         EnvArgs a = a0;
         if (k & 1) {
@@ -319,17 +333,20 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
         // steps after the first see the same vertices twice (bodies at rest)
         p.verts_last = k == 0 ? verts_last : verts;
         double* fout = force_history && width > 0 ? force_history + (int64_t)k * width : nullptr;
+        double* rows = fout ? partial + (size_t)(k & 1) * kPolyWaves * HL_MAX_SOLIDS * 6 : nullptr;
         e = step_gather(HL_D2Q9, dtype, a, s);
         if (e != hipSuccess) break;
         switch (dtype) {
-        case HL_F32: e = launch_band<float, HL_F32>(a, p, bbox, fout ? partial : nullptr, s); break;
-        case HL_F64: e = launch_band<double, HL_F64>(a, p, bbox, fout ? partial : nullptr, s); break;
-        case HL_F16: e = launch_band<float, HL_F16>(a, p, bbox, fout ? partial : nullptr, s); break;
-        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, bbox, fout ? partial : nullptr, s); break;
+        case HL_F32: e = launch_band<float, HL_F32>(a, p, bbox, rows, prev, prev_out, s); break;
+        case HL_F64: e = launch_band<double, HL_F64>(a, p, bbox, rows, prev, prev_out, s); break;
+        case HL_F16: e = launch_band<float, HL_F16>(a, p, bbox, rows, prev, prev_out, s); break;
+        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, bbox, rows, prev, prev_out, s); break;
         default: e = hipErrorInvalidValue;
         }
-        if (e == hipSuccess && fout) e = sum_partials(partial, kPolyWaves, width, fout, s);
+        prev = rows;
+        prev_out = fout;
     }
+    if (e == hipSuccess && prev) e = sum_partials(prev, kPolyWaves, width, prev_out, s);
     return e;
 }
 
