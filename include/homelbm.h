@@ -64,6 +64,9 @@ typedef struct hl_env {
     int32_t bc_type[6];   /* 0 velocity, 1 zero gradient, 2 free slip */
     float bc_value[6][3]; /* face velocity for type 0 */
     float scale[10];      /* 16-bit storage scale per field */
+    uint64_t mask_epoch;  /* 0, or a caller-kept version of mask + mask_box: hl_step calls on one
+                             stream with the same nonzero epoch, mask and box reuse the near-body
+                             list the first of them built (a new epoch whenever the mask changes) */
 } hl_env;
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -45,6 +45,7 @@ class HLEnv(C.Structure):
         ("bc_type", C.c_int32 * 6),
         ("bc_value", (C.c_float * 3) * 6),
         ("scale", C.c_float * 10),
+        ("mask_epoch", C.c_uint64),
     ]
 
 

@@ -137,6 +137,7 @@ class Batch:
             self._h, i, L.mask.data_ptr(), b, caps_dev.data_ptr() if caps_dev is not None else None, ncap,
             poses_dev.data_ptr() if poses_dev is not None else None, T), "hl_batch_set_bodies")
         L.mask_box = box
+        L.mask_changed()
         self._tables[i] = (caps_dev, poses_dev)
         self._graph = None
 
@@ -154,6 +155,10 @@ class Batch:
         return self._active_dev, a
 
     def _advance_mirror(self, a, nsteps):
+        for k, L in enumerate(self.lattices):  # the voxelizer rewrote moving bodies' masks
+            t = self._tables[k]
+            if t is not None and t[0] is not None:
+                L.mask_changed()
         for i, L in enumerate(self.lattices):
             if a is None or a[i]:
                 L.cur ^= nsteps & 1

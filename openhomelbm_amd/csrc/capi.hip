@@ -106,6 +106,12 @@ struct StreamCtx {
     std::mutex mutex;  // held for the whole call that uses this context
     void* ws = nullptr;
     size_t bytes = 0;
+    // the near-body list in ws and what it was built for (hl_step with a
+    // nonzero mask_epoch); any other use of ws clears it
+    bool list_valid = false;
+    uint64_t list_epoch = 0;
+    const void* list_mask = nullptr;
+    int32_t list_key[10] = {0};
 };
 std::mutex g_ctx_mutex;
 std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
@@ -133,6 +139,7 @@ int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
             c->ws = nullptr;
             c->bytes = 0;
         }
+        c->list_valid = false;
         e = hipMalloc(&c->ws, bytes);
         if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
         c->bytes = bytes;
@@ -224,8 +231,24 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
         hl::EnvArgs a0 = make_args(env, cur);
         rc = workspace(ctx, s, hl::forces_workspace_bytes(a0, env->lattice), &ws);
         if (rc) return rc;
-        hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
-        if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
+        // the list depends on the mask, its box and the lattice; a caller that
+        // versions its mask (mask_epoch) gets it rebuilt only when that changes.
+        // Under stream capture it is always recorded, so graphs stay self-contained.
+        const int32_t key[10] = {env->lattice, env->nx, env->ny, env->nz, env->mask_box[0], env->mask_box[1],
+                                 env->mask_box[2], env->mask_box[3], env->mask_box[4], env->mask_box[5]};
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusActive;
+        const bool reuse = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone && ctx->list_valid &&
+                           ctx->list_epoch == env->mask_epoch && ctx->list_mask == env->mask &&
+                           std::memcmp(ctx->list_key, key, sizeof(key)) == 0;
+        if (!reuse) {
+            hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
+            if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
+            ctx->list_valid = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone;
+            ctx->list_epoch = env->mask_epoch;
+            ctx->list_mask = env->mask;
+            std::memcpy(ctx->list_key, key, sizeof(key));
+        }
     }
     // force rows a tiled launch left for the next launch to sum
     struct Pending {
@@ -329,6 +352,7 @@ int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const f
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
+    ctx->list_valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::polygon_workspace_bytes(), &ws);
     if (rc) return rc;
     hl::EnvArgs a = make_args(env, cur);
@@ -351,6 +375,7 @@ int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
+    ctx->list_valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::forces_workspace_bytes(a, env->lattice), &ws);
     if (rc) return rc;
     hipError_t e = hl::forces(env->lattice, env->dtype, a, out, ws, s);
@@ -433,6 +458,7 @@ int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
+    ctx->list_valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::mass_workspace_bytes(a, env->lattice), &ws);
     if (rc) return rc;
     hipError_t e = hl::mass(env->lattice, env->dtype, a, out, ws, s);

@@ -13,6 +13,7 @@ envs/lbm/lbm_func.py:475-500).
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import math
 from typing import Optional, Sequence
 
@@ -29,6 +30,11 @@ DTYPE_CODES = {"f32": N.HL_F32, "f64": N.HL_F64, "f16": N.HL_F16, "bf16": N.HL_B
 #: default 16-bit storage scales: rho-1, u, S (values are O(1e-3..1e-1) in lattice units)
 DEFAULT_SCALE_2D = (256.0, 16.0, 16.0, 1024.0, 1024.0, 1024.0)
 DEFAULT_SCALE_3D = (256.0, 16.0, 16.0, 16.0, 1024.0, 1024.0, 1024.0, 1024.0, 1024.0, 1024.0)
+
+
+# mask versions: unique across every lattice of the process, so a version
+# never names two different masks (hl_env.mask_epoch)
+_EPOCHS = itertools.count(1)
 
 
 def _stream_ptr(stream=None) -> int:
@@ -100,6 +106,7 @@ class Lattice:
         self.solids = torch.zeros((max(nsolid, 1), self.pose_width), dtype=torch.float32, device=self.device)
         self._init_identity_poses()
         self.mask: Optional[torch.Tensor] = None
+        self._mask_epoch = next(_EPOCHS)
         self._mesh_overflow: Optional[torch.Tensor] = None
         self.mask_box = (0, 0, 0, 0, 0, 0)
         self._mask_dirty = None
@@ -143,6 +150,7 @@ class Lattice:
             e.mask = self.mask.data_ptr()
             for k in range(6):
                 e.mask_box[k] = int(self.mask_box[k])
+            e.mask_epoch = self._mask_epoch
         if self.force is not None:
             e.force = self.force.data_ptr()
         e.solids = self.solids.data_ptr() if self.nsolid > 0 else None
@@ -283,6 +291,7 @@ class Lattice:
         self.mask = torch.from_numpy(m.reshape(-1)).to(self.device)
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
+        self.mask_changed()
 
     def voxelize(self, caps: np.ndarray, box=None, stream=None) -> None:
         """Rebuild the mask from capsules [n, 8] (ax, ay, az, bx, by, bz, r, solid_id) on the GPU."""
@@ -301,6 +310,7 @@ class Lattice:
                                      _stream_ptr(stream)), "hl_voxelize")
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
+        self.mask_changed()
 
     def voxelize_device(self, caps: torch.Tensor, box, stream=None) -> None:
         """Rebuild the mask from a device capsule table [n, 8] (FP32) inside a
@@ -316,6 +326,7 @@ class Lattice:
                                      _stream_ptr(stream)), "hl_voxelize")
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
+        self.mask_changed()
 
     def voxelize_mesh(self, tris: np.ndarray, solids: Sequence[int], box=None, merge: bool = False,
                       stream=None) -> None:
@@ -357,6 +368,12 @@ class Lattice:
                                           self._mesh_overflow.data_ptr(), _stream_ptr(stream)), "hl_voxelize_mesh")
         self.mask_box = union_box(self.mask_box, box) if merge else tuple(box)
         self._mask_dirty = tuple(self.mask_box)
+        self.mask_changed()
+
+    def mask_changed(self) -> None:
+        """Give the mask a new version (call after writing self.mask outside
+        this class): steps with an unchanged version reuse the near-body list."""
+        self._mask_epoch = next(_EPOCHS)
 
     def mesh_overflowed(self) -> bool:
         """True when the last mesh voxelization met a ray with more than 64
@@ -393,6 +410,7 @@ class Lattice:
                                               _stream_ptr(stream)), "hl_voxelize_polygons")
         self.mask_box = tuple(box)
         self._mask_dirty = tuple(box)
+        self.mask_changed()
 
     def set_poses(self, poses: np.ndarray) -> None:
         """Pose table [nsolid, 8 | 24] (see include/homelbm.h)."""

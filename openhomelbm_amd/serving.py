@@ -104,6 +104,7 @@ class _Resident:
         self.refresh(0)
         if spec.lattice == 27:
             L.mask = torch.zeros(L.ncell, dtype=torch.uint8, device=L.device)
+            L.mask_changed()
             self.hist = torch.zeros((1, n, 6), dtype=torch.float64, device=L.device)
         self.queue: Deque[Request] = collections.deque()
 
