@@ -107,6 +107,8 @@ class Lattice:
         self._init_identity_poses()
         self.mask: Optional[torch.Tensor] = None
         self._mask_epoch = next(_EPOCHS)
+        self._env_key = None
+        self._env_cache: Optional[N.HLEnv] = None
         self._mesh_overflow: Optional[torch.Tensor] = None
         self.mask_box = (0, 0, 0, 0, 0, 0)
         self._mask_dirty = None
@@ -166,6 +168,20 @@ class Lattice:
             e.scale[f] = float(self.scale[f])
         return e
 
+    def _step_env(self) -> N.HLEnv:
+        """env() for the per-step calls, rebuilt only when a value it holds
+        changed (buffers, mask and its version and box, force, pose table,
+        viscosity, faces, storage scales); the key compares values, so
+        in-place edits of bc_type / bc_value are seen."""
+        key = (self.buf.data_ptr(), self.mask.data_ptr() if self.mask is not None else 0, self._mask_epoch,
+               tuple(self.mask_box), self.force.data_ptr() if self.force is not None else 0, self.solids.data_ptr(),
+               self.nsolid, float(self.nu), tuple(self.bc_type), tuple(tuple(v) for v in self.bc_value),
+               tuple(self.scale))
+        if key != self._env_key:
+            self._env_cache = self.env()
+            self._env_key = key
+        return self._env_cache
+
     def fields(self, which: Optional[int] = None) -> torch.Tensor:
         """Device view of one state as (nfields, (z,) y, x)."""
         b = self.cur if which is None else which
@@ -211,7 +227,7 @@ class Lattice:
         autotuned shape of this lattice's size class when one is recorded."""
         if _tuned and self.lattice == 27:
             autotune.apply(self)
-        e = self.env()
+        e = self._step_env()
         fh = None
         if force_history is not None:
             if (force_history.dtype != torch.float64 or not force_history.is_contiguous()
@@ -249,7 +265,7 @@ class Lattice:
                     or force_history.numel() < nsteps * self.nsolid * 6 or force_history.device != self.device):
                 raise ValueError("force_history must be a contiguous FP64 device tensor [nsteps, nsolid, 6]")
             fh = force_history.data_ptr()
-        e = self.env()
+        e = self._step_env()
         N.check(self.lib.hl_step_polygons(C.byref(e), self.cur, int(nsteps), verts.data_ptr(),
                                           verts_last.data_ptr() if verts_last is not None else None,
                                           counts.data_ptr(), int(verts.shape[1]), radius2.data_ptr(), fh,
