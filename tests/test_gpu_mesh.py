@@ -72,3 +72,36 @@ def test_create_solid_from_mesh_voxelizes_and_feels_drag():
     assert np.array_equal(got, M.voxelize((nz, ny, nx), tris, [0] * len(f)))
     F = fl.solid_force.numpy()[0]
     assert np.isfinite(F).all() and F[0] > 0 and abs(F[1]) < 0.2 * F[0] and abs(F[2]) < 0.2 * F[0]
+
+
+def test_capsule_and_mesh_bodies_in_one_world_and_moving():
+    """A capsule (solid 0) and a mesh (solid 1) in one world: the mask is the
+    capsule voxelization with the mesh merged into its empty cells; after the
+    mesh body moves, its old cells are cleared."""
+    from oracle import oracle as orc
+
+    nx, ny, nz = 40, 26, 24
+    s = LBM_Solver3D(nx, ny, nz, solid_num=2)
+    fl = s.flows[0]
+    s.create_solid_capsule(0, (-0.1, 0.0, 0.0), (0.1, 0.0, 0.0), 0.08, lbm_position=(12.0, 13.0, 12.0),
+                           lbm_scale=1.0)
+    v, f = M.icosphere(1.0, 1)
+    s.create_solid_from_mesh(1, (v, f), lbm_position=(26.3, 12.6, 11.7), lbm_scale=4.5 / nx)
+
+    def want():
+        env = orc.Env(27, nx, ny, nz)
+        base = orc.voxelize(env, fl.capsules)
+        pos = fl.solid_position.numpy()[1]
+        R = quat_to_matrix(fl.solid_quaternion.numpy()[1])
+        tris = M.posed_tris(v * 4.5, f, R, pos)
+        return M.voxelize((nz, ny, nx), tris, [1] * len(f), mask=base, merge=True)
+
+    for k in range(3):
+        if k == 2:
+            s.update_solids_batch(0, [1], np.array([[0.1, -0.05, 0.0]]), np.array([[0.95, 0.0, 0.31, 0.0]]))
+        s.step()
+        torch.cuda.synchronize()
+        got = fl.lattice.mask.view(nz, ny, nx).cpu().numpy()
+        w = want()
+        assert np.array_equal(got, w), k
+        assert set(np.unique(w)) == {0, 1, 2}
