@@ -422,9 +422,21 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     const __amdgpu_buffer_rsrc_t rdst = buffer_rsrc(a.dst, 10 * fbytes);
     const __amdgpu_buffer_rsrc_t rfor = buffer_rsrc(a.force, FORCE ? 3 * (uint32_t)a.stride * 4 : 0);
 
-    // source column of this thread (tile + halo); lanes past the halo keep
-    // an LDS slot of their own out of the way
-    const int sgx = x0 - 1 + tid % NX, sgy = y0 - 1 + tid / NX;
+    // source column of this thread (tile + halo) and its LDS slot (the
+    // column's index in the window); lanes past the halo keep a slot of
+    // their own out of the way. With 64-wide tiles each of the first NY
+    // waves loads one whole window row from x0 (two 128-byte lines per FP32
+    // field when rows start on a line, as for nx a multiple of 32) and the
+    // next 2 * NY threads take the side halo columns.
+    int wsl = tid;
+    if (TX == 64) {
+        if (tid < NY * TX) wsl = (tid / TX) * NX + 1 + tid % TX;
+        else if (tid < NCOL) {
+            const int j = tid - NY * TX;
+            wsl = (j >> 1) * NX + ((j & 1) ? NX - 1 : 0);
+        }
+    }
+    const int sgx = x0 - 1 + wsl % NX, sgy = y0 - 1 + wsl / NX;
     const bool src_lds = tid < NCOL;
     const bool src_ok = src_lds && sgx >= 0 && sgx < nx && sgy >= 0 && sgy < ny;
     const uint32_t vsrc = src_ok ? (uint32_t)(sgy * nx + sgx) * E : kOOB;
@@ -479,13 +491,13 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         float f[27];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            buf[kMid[k]][tid] = hold_mid[k];
-            buf[kUp[k]][tid] = U[k];
+            buf[kMid[k]][wsl] = hold_mid[k];
+            buf[kUp[k]][wsl] = U[k];
         }
         recon27(m, f);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            buf[kDown[k]][tid] = f[kDown[k]];
+            buf[kDown[k]][wsl] = f[kDown[k]];
             U[k] = f[kUp[k]];
             hold_mid[k] = f[kMid[k]];
         }
