@@ -192,6 +192,38 @@ def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
     return out
 
 
+def bench_storage_speed(K: int = 200, W: int = 10):
+    """The same sphere step with 16-bit moment storage (FP32 collision):
+    whole steps with forces and the step kernel alone, per storage type."""
+    import torch
+
+    from openhomelbm_amd import inputs
+
+    w = inputs.d3q27_sphere()
+    out = {"workload": w.name, "steps": K}
+    for dt in ("f16", "bf16"):
+        lat = inputs.build_lattice(w, dt)
+        h = torch.zeros((K, lat.nsolid, 6), dtype=torch.float64, device=lat.device)
+        lat.step(W, force_history=h[:W])
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        lat.step(K, force_history=h)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / K
+        e0.record()
+        lat.step(K)
+        e1.record()
+        torch.cuda.synchronize()
+        kms = e0.elapsed_time(e1) / K
+        out[dt] = {"ms_per_step": round(ms, 4), "mlups": round(w.ncell / (ms * 1e-3) / 1e6, 1),
+                   "kernel_us": round(kms * 1e3, 2), "bytes_per_cell": lat.bytes_per_cell_update,
+                   "gbs_algorithmic": round(lat.bytes_per_cell_update * w.ncell / (kms * 1e-3) / 1e9, 1)}
+        del lat, h
+    return out
+
+
 def bench_coupled(K: int, W: int = 20):
     """Two-way coupled step on the device: a sphere (radius 4, twice the
     fluid density) released in a 64x32x32 D3Q27 channel (inlet 0.05); per step
@@ -405,6 +437,7 @@ def main():
         t0p = time.perf_counter()
         prec = precision.sweep(steps=args.precision_steps, log=log)
         prec["seconds"] = round(time.perf_counter() - t0p, 1)
+        prec["speed"] = bench_storage_speed()
         log(f"[bench] precision sweep {prec}")
 
     coupled = None
