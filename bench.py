@@ -163,8 +163,9 @@ def bench_karman_speed(K: int = 2000, W: int = 50):
 
 def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
     """The reference's API pattern: LBM_Solver(512, 128, nworld) with a square
-    body per world (configure_solid), solver.step() per step; the batched
-    route (every world in one launch sequence) against world by world."""
+    body per world (configure_solid), solver.step() per step, with the mask
+    link rule (cut_cell=False); the batched route (every world in one launch
+    sequence) against world by world."""
     import torch
 
     from openhomelbm_amd.lbm import LBM_Solver
@@ -172,7 +173,7 @@ def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
     sq = np.array([[-0.02, -0.02], [0.02, -0.02], [0.02, 0.02], [-0.02, 0.02]], np.float32)
     out = {"worlds": nworld, "grid": [512, 128], "steps": K}
     for name, batched in (("batched", True), ("per_world", False)):
-        s = LBM_Solver(512, 128, solid_num=1, nworld=nworld, batched=batched)
+        s = LBM_Solver(512, 128, solid_num=1, nworld=nworld, batched=batched, cut_cell=False)
         for f in s.flows:
             f.vis_shear = 0.008
             f.bc_type = [0, 1, 1, 1]
@@ -425,6 +426,8 @@ def main():
 
         t0k = time.perf_counter()
         karm = karman.compare(steps=args.karman_steps, window=min(25000, args.karman_steps // 2), log=log)
+        karm["boundaries"] = karman.compare_boundaries(steps=args.karman_steps,
+                                                       window=min(25000, args.karman_steps // 2))
         karm["speed"] = bench_karman_speed()
         karm["lbm_solver_worlds"] = bench_solver_worlds()
         karm["seconds"] = round(time.perf_counter() - t0k, 1)

@@ -6,13 +6,16 @@ are LatticeField / HostArray handles with .numpy() / .assign() instead of
 Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
 ...) works as with the reference. Differences, all deliberate:
 
-* Bodies are closed polygons (configure_solid, lbm_core.py:172-282). Each
-  step voxelizes them into an integer mask with the reference's even-odd
-  inside test, and the wall rule runs on mask links (mask(x) != mask(x-c_i))
-  instead of segment-link intersections (get_cutcell, lbm_func.py:545-607).
-  LBM_Solver(..., cut_cell=True) (or HomeFlow.cut_cell = True) runs the
-  reference's own segment-link predicate and material-point wall rule
-  instead (hl_step_polygons). Capsule bodies (add_capsule) are an extension.
+* Bodies are closed polygons (configure_solid, lbm_core.py:172-282). By
+  default they take the reference's own boundary: the segment-link
+  predicate and material-point wall rule (get_cutcell, lbm_func.py:545-607;
+  hl_step_polygons). LBM_Solver(..., cut_cell=False) (or HomeFlow.cut_cell
+  = False) voxelizes them into an integer mask with the reference's even-odd
+  inside test and runs the wall rule on mask links (mask(x) != mask(x-c_i)):
+  faster, and several worlds then step in one batched launch sequence, but
+  a different boundary discretization (on the 512x128 cylinder: Strouhal
+  3%, mean drag 16% apart). Capsule bodies (add_capsule, an extension)
+  always use the mask rule.
 * Forces and torques are reduced in FP64 in a fixed order and are
   deterministic; solid_forcex / solid_forcey / torque hold them as FP32 like
   the reference's atomics.
@@ -94,8 +97,9 @@ class HomeFlow:
         self._at_rest = False  # uploaded table has last == current and the solids are unchanged since
         self._forces = torch.zeros((1, max(n_objects, 1), 6), dtype=torch.float64, device=L.device)
         self._box = (0, 0, 0, 0, 0, 0)
-        # cut_cell: the reference's segment-link cut cells (get_cutcell) instead of mask links
-        self.cut_cell = False
+        # cut_cell: polygon bodies take the reference's segment-link cut cells
+        # (get_cutcell); False: the mask link rule
+        self.cut_cell = True
         self._seg_prev: Optional[np.ndarray] = None  # last step's transformed vertices (zeros before the first)
         self._seg_host = None  # (verts, last, counts, radius2) last uploaded
         self._seg_dev = None
@@ -310,7 +314,7 @@ class HomeFlow:
             self._uploaded = table
         self._at_rest = bool(np.array_equal(now, last))
         self._last_pose = now
-        if self.cut_cell:
+        if self._uses_cut_cells():
             self._prepare_segments()
         if not self._solid_dirty:
             return
@@ -336,6 +340,10 @@ class HomeFlow:
             L.voxelize(np.array(caps, np.float32), stream=stream)
         self._solid_dirty = False
 
+    def _uses_cut_cells(self) -> bool:
+        """Polygon bodies stepped with the reference's segment-link cut cells."""
+        return self.cut_cell and self.n_objects > 0 and not self._capsules
+
     def _prepare_segments(self) -> None:
         """Device tables of the exact cut-cell step: this step's transformed
         vertices and the previous step's, as precompute_transformed_segments
@@ -343,8 +351,6 @@ class HomeFlow:
         exactly (0, 0) counts as new, so its last position is the new one),
         and the squared bounding radii (solid_max_radius * scale) * scale in
         FP32 with scale = solid_scale * nx (lbm_func.py:569-574)."""
-        if self._capsules:
-            raise ValueError("cut_cell needs polygon bodies (configure_solid), not capsules")
         f32 = np.float32
         verts, num = self._transformed()
         prev = self._seg_prev
@@ -379,7 +385,7 @@ class HomeFlow:
         self._prepare(stream)
         L = self.lattice
         fh = self._forces if self.n_objects > 0 else None
-        if self.cut_cell and self.n_objects > 0:
+        if self._uses_cut_cells():
             v, vl, cnt, r2 = self._seg_dev
             L.step_polygons(1, v, vl, cnt, r2, force_history=fh, stream=stream)
             return
@@ -390,7 +396,7 @@ class LBM_Solver:
     """Worlds of one size stepped together (envs/lbm/lbm_solver.py:21-141)."""
 
     def __init__(self, nx: int, ny: int, solid_num: int = 1, nworld: int = 1, device: Optional[Any] = None,
-                 dtype: str = "f32", batched: bool = True, cut_cell: bool = False) -> None:
+                 dtype: str = "f32", batched: bool = True, cut_cell: bool = True) -> None:
         if nworld < 1:
             raise ValueError("nworld must be >= 1")
         self._use_batch = bool(batched)
@@ -423,7 +429,7 @@ class LBM_Solver:
         from .batch import MAX_SOLIDS
         return self._use_batch and len(self.flows) > 1 and all(
             f.lattice.dtype == "f32" and f.lattice.force is None and f.lattice.nsolid <= MAX_SOLIDS
-            and not f.cut_cell
+            and not f._uses_cut_cells()
             for f in self.flows)
 
     @staticmethod

@@ -18,7 +18,7 @@ def _square(h):
 
 
 def _setup_2d(nx=48, ny=40, nworld=2):
-    s = LBM_Solver(nx, ny, solid_num=1, nworld=nworld)
+    s = LBM_Solver(nx, ny, solid_num=1, nworld=nworld, cut_cell=False)  # the mask link rule
     for f in s.flows:
         f.vis_shear = 0.02
         f.bc_type = [0, 1, 1, 1]
@@ -175,7 +175,7 @@ def test_lbm_solver3d_pose_update_and_reset():
 
 
 def _moving_worlds(batched, nworld=3, steps=6):
-    s = LBM_Solver(48, 40, solid_num=1, nworld=nworld, batched=batched)
+    s = LBM_Solver(48, 40, solid_num=1, nworld=nworld, batched=batched, cut_cell=False)
     for w, f in enumerate(s.flows):
         f.vis_shear = 0.02 + 0.005 * w
         f.bc_type = [0, 1, 1, 1]

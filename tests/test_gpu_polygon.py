@@ -115,10 +115,11 @@ def test_step_polygons_argument_errors():
 
 
 def test_lbm_solver_cut_cell_matches_oracle():
-    """LBM_Solver(cut_cell=True): configure_solid polygons stepped with the
+    """LBM_Solver (cut_cell=True, the default): configure_solid polygons stepped with the
     segment predicate; the facade's vertex tables follow
     precompute_transformed_segments (last = previous step's vertices)."""
-    s = LBM_Solver(NX, NY, solid_num=1, cut_cell=True)
+    s = LBM_Solver(NX, NY, solid_num=1)  # the reference's boundary is the default
+    assert s.flows[0].cut_cell
     f = s.flows[0]
     f.vis_shear = 0.015
     f.bc_type = list(BC_TYPE)
