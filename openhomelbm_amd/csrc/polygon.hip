@@ -29,7 +29,7 @@ namespace hl {
 
 namespace {
 
-constexpr int kPolyWaves = 256;  // band grid: one row of force partials per wave
+constexpr int kPolyWaves = 1024;  // band grid: one row of force partials per wave
 
 struct PolyArgs {
     const float* verts;       // [nsolid][maxv][2], this step
@@ -76,28 +76,101 @@ __device__ __forceinline__ int poly_cut(const PolyArgs& p, const float* verts, u
         near &= near - 1;
         const int num = poly_count(p, s);
         const float* v = verts + (int64_t)s * p.maxv * 2;
+        T cx = T(v[0]), cy = T(v[1]);  // segment n runs from vertex n to vertex n+1 (mod num)
         for (int n = 0; n < num; ++n) {
             const int m = n + 1 == num ? 0 : n + 1;
-            const T cx = T(v[2 * n]), cy = T(v[2 * n + 1]), dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
+            const T dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
             const T abc = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
             const T abd = (ax - dx) * (by - dy) - (ay - dy) * (bx - dx);
-            if (abc * abd >= T(0)) continue;
-            const T cda = (cx - ax) * (dy - ay) - (cy - ay) * (dx - ax);
-            const T cdb = cda + abc - abd;
-            if (cda * cdb > T(0)) continue;
-            const T t = T(1) - cda / (abd - abc);
-            if (cutcell < T(0) || t < cutcell) {
-                cutcell = t;
-                sid = s;
-                sr = abc / (abc - abd);
-                sg = n;
+            if (!(abc * abd >= T(0))) {  // the reference skips on >= 0 (a NaN product goes on)
+                const T cda = (cx - ax) * (dy - ay) - (cy - ay) * (dx - ax);
+                const T cdb = cda + abc - abd;
+                if (!(cda * cdb > T(0))) {
+                    const T t = T(1) - cda / (abd - abc);
+                    if (cutcell < T(0) || t < cutcell) {
+                        cutcell = t;
+                        sid = s;
+                        sr = abc / (abc - abd);
+                        sg = n;
+                    }
+                }
             }
+            cx = dx;
+            cy = dy;
         }
     }
     if (cutcell >= T(0) && sid >= 0) {
         *ratio = sr;
         *seg = sg;
         return sid;
+    }
+    return -1;
+}
+
+// The same nearest crossing with the segment loop shared by kG lanes: lane g
+// takes the segments n = g, g + kG, ... of the near solids, then the group
+// keeps the smallest t, ties to the earliest (solid, segment) in the loop
+// order. With every crossing at t >= 0 that is exactly the sequential rule
+// (the first crossing is taken, later ones only when strictly nearer); a
+// crossing with t < 0 or NaN makes the group run the sequential loop.
+constexpr int kG = 8;
+constexpr int kCells = 64 / (8 * kG);  // cells per wavefront pass
+
+template <typename T>
+__device__ __forceinline__ int poly_cut_split(const PolyArgs& p, const float* verts, uint64_t near, int x, int y,
+                                              int i, int g, T* ratio, int* seg)
+{
+    T bt = T(-1), bsr = T(-1);
+    int bsid = -1, bsg = -1, bkey = 0x7fffffff, odd = 0;
+    const T bx = T(x), by = T(y);
+    const T ax = bx - T(D2CX[i]), ay = by - T(D2CY[i]);
+    uint64_t nr = near;
+    while (nr) {
+        const int s = __builtin_ctzll(nr);
+        nr &= nr - 1;
+        const int num = poly_count(p, s);
+        const float* v = verts + (int64_t)s * p.maxv * 2;
+        for (int n = g; n < num; n += kG) {
+            const int m = n + 1 == num ? 0 : n + 1;
+            const T cx = T(v[2 * n]), cy = T(v[2 * n + 1]), dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
+            const T abc = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
+            const T abd = (ax - dx) * (by - dy) - (ay - dy) * (bx - dx);
+            if (!(abc * abd >= T(0))) {
+                const T cda = (cx - ax) * (dy - ay) - (cy - ay) * (dx - ax);
+                const T cdb = cda + abc - abd;
+                if (!(cda * cdb > T(0))) {
+                    const T t = T(1) - cda / (abd - abc);
+                    if (!(t >= T(0))) {
+                        odd = 1;
+                    } else if (bt < T(0) || t < bt) {
+                        bt = t;
+                        bsid = s;
+                        bsr = abc / (abc - abd);
+                        bsg = n;
+                        bkey = (s << 20) | n;
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < kG; off <<= 1) {
+        const T ot = __shfl_xor(bt, off), osr = __shfl_xor(bsr, off);
+        const int okey = __shfl_xor(bkey, off), osid = __shfl_xor(bsid, off), osg = __shfl_xor(bsg, off);
+        odd |= __shfl_xor(odd, off);
+        if (ot >= T(0) && (bt < T(0) || ot < bt || (ot == bt && okey < bkey))) {
+            bt = ot;
+            bsr = osr;
+            bkey = okey;
+            bsid = osid;
+            bsg = osg;
+        }
+    }
+    if (odd) return poly_cut<T>(p, verts, near, x, y, i, ratio, seg);
+    if (bt >= T(0) && bsid >= 0) {
+        *ratio = bsr;
+        *seg = bsg;
+        return bsid;
     }
     return -1;
 }
@@ -158,31 +231,32 @@ __global__ void k_poly_bbox(EnvArgs a, PolyArgs p, int* bbox)
 // LDS for the vertex tables: larger tables are read from global memory.
 constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 
-// One wavefront per workgroup, eight cells per pass: lane 8 c + (i - 1)
-// classifies link i of cell c (segment test against the near solids, wall
-// rule or pull) and keeps that link's momentum exchange; lane 8 c also pulls
-// the rest population, gathers the nine through LDS and collides. Force rows:
+// One wavefront per workgroup, kCells cells per pass: the kG lanes
+// 8 kG c + kG (i - 1) + g share the segment test of link i of cell c; lane
+// g = 0 then applies the wall rule or the pull and keeps that link's
+// momentum exchange; the lane of link 1 also pulls the rest population,
+// gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
-template <typename T, int DT>
+template <typename T, int DT, bool LDSV>
 __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial,
                                                    const double* prev, double* prev_out)
 {
     extern __shared__ float lds_verts[];
     __shared__ double acc[HL_MAX_SOLIDS * 6];
-    __shared__ T pops[8][9];
+    __shared__ T pops[kCells][9];
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
     ST* dst = static_cast<ST*>(a.dst);
-    const int lane = threadIdx.x, slot = lane >> 3, i = (lane & 7) + 1;
+    const int lane = threadIdx.x, g = lane % kG, slot = lane / (8 * kG), i = (lane / kG) % 8 + 1;
     const int nx = a.nx, ny = a.ny;
     const int width = a.nsolid * 6;
+    // the vertex tables in LDS when they fit (a compile-time choice, so the
+    // segment loop reads LDS directly rather than through generic pointers)
     const int nv = a.nsolid * p.maxv * 2;
-    const float* verts = p.verts;
-    if (nv <= kPolyLdsFloats) {
+    if (LDSV)
         for (int k = lane; k < nv; k += 64) lds_verts[k] = p.verts[k];
-        verts = lds_verts;
-    }
+    const float* verts = LDSV ? static_cast<const float*>(lds_verts) : p.verts;
     if (partial)
         for (int k = lane; k < width; k += 64) acc[k] = 0.0;
     __syncthreads();
@@ -190,7 +264,7 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
     const int bw = bbox[1] - bbox[0], bh = bbox[3] - bbox[2];
     const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
     const T grid_length = T(1) / T(nx);
-    for (int64_t base = (int64_t)blockIdx.x * 8; base < n; base += (int64_t)gridDim.x * 8) {
+    for (int64_t base = (int64_t)blockIdx.x * kCells; base < n; base += (int64_t)gridDim.x * kCells) {
         const int64_t t = base + slot;
         int sid = -1;
         T F[3] = {T(0), T(0), T(0)};
@@ -203,15 +277,16 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
             const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
             const uint64_t near = poly_near<T>(a, p, px, py);
             if (near) {
-                work = true;
+                work = g == 0;
                 interior = px == x && py == y;
                 c = (int64_t)px + (int64_t)nx * py;
                 T mc[6];
                 load_cell<T, DT>(a, c, 6, mc);
                 T ratio = 0;
                 int seg = -1;
-                sid = poly_cut<T>(p, verts, near, px, py, i, &ratio, &seg);
-                T q;
+                sid = poly_cut_split<T>(p, verts, near, px, py, i, g, &ratio, &seg);
+                if (g != 0) sid = -1;  // lane g = 0 carries the link from here
+                T q = T(0);
                 if (sid >= 0) {
                     T up[2], r[2];
                     poly_point<T>(a, p, sid, seg, ratio, up, r);
@@ -231,7 +306,7 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
                     } else {
                         sid = -1;
                     }
-                } else {
+                } else if (g == 0) {
                     const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
                     T m[6];
 #pragma unroll
@@ -239,8 +314,8 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
                     q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
                     if (q < T(0)) q = T(0);
                 }
-                pops[slot][i] = q;
-                if (i == 1) {  // the rest population: never cut (its link has no length)
+                if (g == 0) pops[slot][i] = q;
+                if (g == 0 && i == 1) {  // the rest population: never cut (its link has no length)
                     T q0 = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], 0);
                     if (q0 < T(0)) q0 = T(0);
                     pops[slot][0] = q0;
@@ -297,8 +372,11 @@ hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, dou
                        double* prev_out, hipStream_t s)
 {
     const int nv = a.nsolid * p.maxv * 2;
-    const size_t lds = nv <= kPolyLdsFloats ? (size_t)nv * sizeof(float) : 0;
-    hipLaunchKernelGGL((k_poly_band<T, DT>), kPolyWaves, 64, lds, s, a, p, bbox, partial, prev, prev_out);
+    if (nv <= kPolyLdsFloats)
+        hipLaunchKernelGGL((k_poly_band<T, DT, true>), kPolyWaves, 64, (size_t)nv * sizeof(float), s, a, p, bbox,
+                           partial, prev, prev_out);
+    else
+        hipLaunchKernelGGL((k_poly_band<T, DT, false>), kPolyWaves, 64, 0, s, a, p, bbox, partial, prev, prev_out);
     return hipGetLastError();
 }
 
