@@ -149,3 +149,54 @@ def test_lbm_solver_cut_cell_matches_oracle():
         np.testing.assert_allclose(got, F[0, [0, 1, 5]], rtol=1e-6, atol=1e-9)
         prev = v
     assert abs(F[0, 0]) > 0
+
+
+def test_random_polygons_bitwise_against_sequential_oracle():
+    """Many polygon sets (3 to 13 vertices, so fewer segments than the eight
+    lanes that share a link's segment loop as well as more; vertices on
+    lattice points so that edges pass through cell centres; overlapping
+    solids): one step each, bitwise against the oracle's sequential loop."""
+    rng = np.random.default_rng(77)
+    nx, ny = 48, 40
+    lat = Lattice(9, nx, ny, nsolid=3)
+    lat.nu = 0.02
+    lat.bc_type = [0, 1, 1, 1, 1, 1]
+    lat.bc_value = [[0.03, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+    for case in range(24):
+        m = random_state(9, (ny, nx), seed=100 + case).astype(np.float32)
+        lat.set_state(m)
+        maxv = 13
+        v = np.zeros((3, maxv, 2), np.float32)
+        vl = np.zeros_like(v)
+        cnt = np.zeros(3, np.int32)
+        pos = np.zeros((3, 2), np.float32)
+        for k in range(3):
+            nvert = int(rng.integers(3, maxv + 1))
+            c = rng.uniform([14, 12], [nx - 14, ny - 12])
+            ang = np.sort(rng.uniform(0, 2 * np.pi, nvert))
+            rad = rng.uniform(3.0, 8.0, nvert)
+            pts = np.stack([c[0] + rad * np.cos(ang), c[1] + rad * np.sin(ang)], 1)
+            if case % 3 == 0:
+                pts = np.round(pts)  # vertices on lattice points
+            v[k, :nvert] = pts
+            vl[k, :nvert] = pts - rng.uniform(-0.2, 0.2, 2)
+            cnt[k] = nvert
+            pos[k] = c
+        r2 = np.array([(9.0 + 1.5) ** 2] * 3, np.float32)
+        env = _env3(nx, ny, pos)
+        lat.set_poses(env.solids)
+        hist = torch.zeros((1, 3, 6), dtype=torch.float64, device="cuda")
+        lat.step_polygons(1, _dev(v), _dev(vl), _dev(cnt), _dev(r2), force_history=hist)
+        ref, F = orc.step_polygons(env, m, v, vl, cnt, r2)
+        torch.cuda.synchronize()
+        assert np.array_equal(lat.get_state(np.float32), ref), case
+        np.testing.assert_allclose(hist[0].cpu().numpy(), F, rtol=1e-10, atol=1e-13)
+
+
+def _env3(nx, ny, pos):
+    env = orc.Env(9, nx, ny, nu=0.02, bc_type=(0, 1, 1, 1), bc_value=((0.03, 0, 0),) + ((0, 0, 0),) * 5)
+    P = np.zeros((len(pos), 8), np.float32)
+    P[:, 0:2], P[:, 2] = pos, 1.0
+    P[:, 4:6], P[:, 6] = pos, 1.0
+    env.solids = P
+    return env
