@@ -6,8 +6,8 @@
 // the crossing's material point (stream_and_collide, lbm_func.py:346-401).
 // Far from bodies that is the plain pull step, so a step here is:
 //   1. the gather kernel over the whole lattice with no mask (plain pull);
-//   2. k_poly_band over the bounding box of the bounding circles (found once
-//      per call by k_poly_bbox): cells whose interior projection lies in a
+//   2. k_poly_band over the bounding box of the bounding circles (each
+//      workgroup finds it first, poly_bbox): cells whose interior projection lies in a
 //      circle are recomputed with the segment predicate and rewritten, face
 //      cells through the same face chain as the gather kernel; interior
 //      cells also emit the momentum exchange of their cut links. One lane
@@ -107,18 +107,18 @@ __device__ __forceinline__ int poly_cut(const PolyArgs& p, const float* verts, u
     return -1;
 }
 
-// The same nearest crossing with the segment loop shared by kG lanes: lane g
-// takes the segments n = g, g + kG, ... of the near solids, then the group
+// The same nearest crossing with the segment loop shared by G lanes (G a
+// power of two up to 8): lane g takes the segments n = g, g + G, ... of the
+// near solids, then the group
 // keeps the smallest t, ties to the earliest (solid, segment) in the loop
 // order. With every crossing at t >= 0 that is exactly the sequential rule
 // (the first crossing is taken, later ones only when strictly nearer); a
 // crossing with t < 0 or NaN makes the group run the sequential loop.
-constexpr int kG = 8;
-constexpr int kCells = 64 / (8 * kG);  // cells per wavefront pass
+constexpr int kMaxG = 8;
 
 template <typename T>
 __device__ __forceinline__ int poly_cut_split(const PolyArgs& p, const float* verts, uint64_t near, int x, int y,
-                                              int i, int g, T* ratio, int* seg)
+                                              int i, int g, int G, T* ratio, int* seg)
 {
     T bt = T(-1), bsr = T(-1);
     int bsid = -1, bsg = -1, bkey = 0x7fffffff, odd = 0;
@@ -130,7 +130,7 @@ __device__ __forceinline__ int poly_cut_split(const PolyArgs& p, const float* ve
         nr &= nr - 1;
         const int num = poly_count(p, s);
         const float* v = verts + (int64_t)s * p.maxv * 2;
-        for (int n = g; n < num; n += kG) {
+        for (int n = g; n < num; n += G) {
             const int m = n + 1 == num ? 0 : n + 1;
             const T cx = T(v[2 * n]), cy = T(v[2 * n + 1]), dx = T(v[2 * m]), dy = T(v[2 * m + 1]);
             const T abc = (ax - cx) * (by - cy) - (ay - cy) * (bx - cx);
@@ -153,8 +153,7 @@ __device__ __forceinline__ int poly_cut_split(const PolyArgs& p, const float* ve
             }
         }
     }
-#pragma unroll
-    for (int off = 1; off < kG; off <<= 1) {
+    for (int off = 1; off < G; off <<= 1) {
         const T ot = __shfl_xor(bt, off), osr = __shfl_xor(bsr, off);
         const int okey = __shfl_xor(bkey, off), osid = __shfl_xor(bsid, off), osg = __shfl_xor(bsg, off);
         odd |= __shfl_xor(odd, off);
@@ -196,9 +195,8 @@ __device__ __forceinline__ void poly_point(const EnvArgs& a, const PolyArgs& p, 
 // Bounding box (half-open [x0, x1) x [y0, y1)) of the cells whose interior
 // projection can lie in a bounding circle; a circle with a non-finite centre
 // or radius takes the whole lattice.
-__global__ void k_poly_bbox(EnvArgs a, PolyArgs p, int* bbox)
+__device__ void poly_bbox(const EnvArgs& a, const PolyArgs& p, int* bbox)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const float hx = (float)(a.nx - 2), hy = (float)(a.ny - 2);
     float x0 = 1e30f, x1 = -1e30f, y0 = 1e30f, y1 = -1e30f;
     for (int s = 0; s < a.nsolid; ++s) {
@@ -220,35 +218,43 @@ __global__ void k_poly_bbox(EnvArgs a, PolyArgs p, int* bbox)
     y0 = fmaxf(y0, 1.0f); y1 = fminf(y1, hy);
     if (x0 > x1 || y0 > y1) {
         bbox[0] = bbox[1] = bbox[2] = bbox[3] = 0;
+        bbox[4] = 1;
         return;
     }
     bbox[0] = (int)x0 - 1;
     bbox[1] = (int)x1 + 2;
     bbox[2] = (int)y0 - 1;
     bbox[3] = (int)y1 + 2;
+    // lanes per link: a small band gets eight lanes sharing each link's
+    // segment loop, a large one one lane per link (the grid stays full)
+    const int64_t cells = (int64_t)(bbox[1] - bbox[0]) * (bbox[3] - bbox[2]);
+    bbox[4] = cells <= 1024 ? 8 : cells <= 4096 ? 4 : cells <= 16384 ? 2 : 1;
 }
 
 // LDS for the vertex tables: larger tables are read from global memory.
 constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 
-// One wavefront per workgroup, kCells cells per pass: the kG lanes
-// 8 kG c + kG (i - 1) + g share the segment test of link i of cell c; lane
+// One wavefront per workgroup, 8 / G cells per pass (G lanes per link, set
+// by poly_bbox from the band's size): the G lanes 8 G c + G (i - 1) + g
+// share the segment test of link i of cell c; lane
 // g = 0 then applies the wall rule or the pull and keeps that link's
 // momentum exchange; the lane of link 1 also pulls the rest population,
 // gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
 template <typename T, int DT, bool LDSV>
-__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const int* bbox, double* partial,
-                                                   const double* prev, double* prev_out)
+__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, double* partial, const double* prev,
+                                                   double* prev_out)
 {
     extern __shared__ float lds_verts[];
+    __shared__ int bbox[5];
+    if (threadIdx.x == 0) poly_bbox(a, p, bbox);  // every workgroup finds the same box
     __shared__ double acc[HL_MAX_SOLIDS * 6];
-    __shared__ T pops[kCells][9];
+    __shared__ T pops[8][9];
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
     ST* dst = static_cast<ST*>(a.dst);
-    const int lane = threadIdx.x, g = lane % kG, slot = lane / (8 * kG), i = (lane / kG) % 8 + 1;
+    const int lane = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     const int width = a.nsolid * 6;
     // the vertex tables in LDS when they fit (a compile-time choice, so the
@@ -259,12 +265,14 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
     const float* verts = LDSV ? static_cast<const float*>(lds_verts) : p.verts;
     if (partial)
         for (int k = lane; k < width; k += 64) acc[k] = 0.0;
-    __syncthreads();
+    __syncthreads();  // bbox, vertex tables and acc are ready
+    const int G = min(max(bbox[4], 1), kMaxG), cells = 8 / G;
+    const int g = lane % G, slot = lane / (8 * G), i = (lane / G) % 8 + 1;
     const int bx0 = bbox[0], by0 = bbox[2];
     const int bw = bbox[1] - bbox[0], bh = bbox[3] - bbox[2];
     const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
     const T grid_length = T(1) / T(nx);
-    for (int64_t base = (int64_t)blockIdx.x * kCells; base < n; base += (int64_t)gridDim.x * kCells) {
+    for (int64_t base = (int64_t)blockIdx.x * cells; base < n; base += (int64_t)gridDim.x * cells) {
         const int64_t t = base + slot;
         int sid = -1;
         T F[3] = {T(0), T(0), T(0)};
@@ -284,7 +292,7 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
                 load_cell<T, DT>(a, c, 6, mc);
                 T ratio = 0;
                 int seg = -1;
-                sid = poly_cut_split<T>(p, verts, near, px, py, i, g, &ratio, &seg);
+                sid = poly_cut_split<T>(p, verts, near, px, py, i, g, G, &ratio, &seg);
                 if (g != 0) sid = -1;  // lane g = 0 carries the link from here
                 T q = T(0);
                 if (sid >= 0) {
@@ -368,15 +376,15 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, const i
 }
 
 template <typename T, int DT>
-hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, const int* bbox, double* partial, const double* prev,
-                       double* prev_out, hipStream_t s)
+hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev, double* prev_out,
+                       hipStream_t s)
 {
     const int nv = a.nsolid * p.maxv * 2;
     if (nv <= kPolyLdsFloats)
-        hipLaunchKernelGGL((k_poly_band<T, DT, true>), kPolyWaves, 64, (size_t)nv * sizeof(float), s, a, p, bbox,
-                           partial, prev, prev_out);
+        hipLaunchKernelGGL((k_poly_band<T, DT, true>), kPolyWaves, 64, (size_t)nv * sizeof(float), s, a, p, partial,
+                           prev, prev_out);
     else
-        hipLaunchKernelGGL((k_poly_band<T, DT, false>), kPolyWaves, 64, 0, s, a, p, bbox, partial, prev, prev_out);
+        hipLaunchKernelGGL((k_poly_band<T, DT, false>), kPolyWaves, 64, 0, s, a, p, partial, prev, prev_out);
     return hipGetLastError();
 }
 
@@ -389,11 +397,9 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
                          const int32_t* counts, int maxv, const float* radius2, double* force_history, void* ws,
                          hipStream_t s)
 {
-    int* bbox = static_cast<int*>(ws);
     double* partial = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
     PolyArgs p{verts, verts_last, counts, maxv, radius2};
-    hipLaunchKernelGGL(k_poly_bbox, 1, 64, 0, s, a0, p, bbox);
-    hipError_t e = hipGetLastError();
+    hipError_t e = hipSuccess;
     const int width = a0.nsolid * 6;
     const double* prev = nullptr;  // the previous step's rows, summed by this step's band launch
     double* prev_out = nullptr;
@@ -415,10 +421,10 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
         e = step_gather(HL_D2Q9, dtype, a, s);
         if (e != hipSuccess) break;
         switch (dtype) {
-        case HL_F32: e = launch_band<float, HL_F32>(a, p, bbox, rows, prev, prev_out, s); break;
-        case HL_F64: e = launch_band<double, HL_F64>(a, p, bbox, rows, prev, prev_out, s); break;
-        case HL_F16: e = launch_band<float, HL_F16>(a, p, bbox, rows, prev, prev_out, s); break;
-        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, bbox, rows, prev, prev_out, s); break;
+        case HL_F32: e = launch_band<float, HL_F32>(a, p, rows, prev, prev_out, s); break;
+        case HL_F64: e = launch_band<double, HL_F64>(a, p, rows, prev, prev_out, s); break;
+        case HL_F16: e = launch_band<float, HL_F16>(a, p, rows, prev, prev_out, s); break;
+        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, rows, prev, prev_out, s); break;
         default: e = hipErrorInvalidValue;
         }
         prev = rows;

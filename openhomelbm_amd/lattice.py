@@ -249,16 +249,21 @@ class Lattice:
             raise ValueError("polygon cut cells are D2Q9 only")
         if self.nsolid < 1:
             raise ValueError("polygon cut cells need at least one solid")
-        vs = [verts, counts, radius2] + ([verts_last] if verts_last is not None else [])
-        for t, dt in zip(vs, (torch.float32, torch.int32, torch.float32, torch.float32)):
-            if t.dtype != dt or not t.is_contiguous() or t.device != self.device:
-                raise ValueError("verts / verts_last / radius2 FP32 and counts int32, contiguous, on the lattice's device")
-        if verts.dim() != 3 or verts.shape[0] != self.nsolid or verts.shape[2] != 2:
-            raise ValueError(f"verts must be [nsolid={self.nsolid}, maxv, 2], got {tuple(verts.shape)}")
-        if verts_last is not None and verts_last.shape != verts.shape:
-            raise ValueError("verts_last must have the shape of verts")
-        if counts.numel() != self.nsolid or radius2.numel() != self.nsolid:
-            raise ValueError("counts and radius2 need one entry per solid")
+        key = tuple((t.data_ptr(), t.dtype, tuple(t.shape)) if t is not None else None
+                    for t in (verts, verts_last, counts, radius2))
+        if key != getattr(self, "_poly_checked", None):  # checks once per set of tables
+            vs = [verts, counts, radius2] + ([verts_last] if verts_last is not None else [])
+            for t, dt in zip(vs, (torch.float32, torch.int32, torch.float32, torch.float32)):
+                if t.dtype != dt or not t.is_contiguous() or t.device != self.device:
+                    raise ValueError("verts / verts_last / radius2 FP32 and counts int32, contiguous, on the "
+                                     "lattice's device")
+            if verts.dim() != 3 or verts.shape[0] != self.nsolid or verts.shape[2] != 2:
+                raise ValueError(f"verts must be [nsolid={self.nsolid}, maxv, 2], got {tuple(verts.shape)}")
+            if verts_last is not None and verts_last.shape != verts.shape:
+                raise ValueError("verts_last must have the shape of verts")
+            if counts.numel() != self.nsolid or radius2.numel() != self.nsolid:
+                raise ValueError("counts and radius2 need one entry per solid")
+            self._poly_checked = key
         fh = None
         if force_history is not None:
             if (force_history.dtype != torch.float64 or not force_history.is_contiguous()

@@ -383,6 +383,10 @@ class HomeFlow:
     def step(self, stream=None) -> None:
         """One time step with the per-solid force of the state it reads."""
         self._prepare(stream)
+        self._launch(stream)
+
+    def _launch(self, stream=None) -> None:
+        """The device work of a step whose tables _prepare has set up."""
         L = self.lattice
         fh = self._forces if self.n_objects > 0 else None
         if self._uses_cut_cells():
@@ -400,6 +404,7 @@ class LBM_Solver:
         if nworld < 1:
             raise ValueError("nworld must be >= 1")
         self._use_batch = bool(batched)
+        self._streams = None  # side streams of the world-by-world route
         self._batch = None
         self._batch_keys = None
         self._batch_boxes = None
@@ -441,13 +446,37 @@ class LBM_Solver:
 
     def step(self) -> None:
         """Advance every world one step (asynchronous): with several FP32 worlds
-        and no body-force field, one batched launch sequence for all of them
-        (bitwise the same as stepping each world alone), else world by world.
+        on the mask rule and no body-force field, one batched launch sequence
+        for all of them (bitwise the same as stepping each world alone), else
+        world by world, the worlds spread over a few concurrent streams.
         The reference captures a graph on the first call; resetting `captured`
         is harmless here."""
         if not self._batchable():
-            for flow in self.flows:
-                flow.step()
+            if len(self.flows) == 1:
+                self.flows[0].step()
+            else:
+                # worlds are independent: their launches go round-robin to two
+                # side streams (default priority, each with its own library
+                # workspace), forked from and joined back into the current
+                # stream; host-side set-up and uploads stay on the current one.
+                # Two: with the current stream that stays within the four
+                # hardware queues HIP gives a process by default (more streams
+                # than queues measured slower)
+                cur = torch.cuda.current_stream(self.device)
+                for flow in self.flows:
+                    flow._prepare()
+                if self._streams is None:
+                    self._streams = [torch.cuda.Stream(device=self.device) for _ in range(min(2, len(self.flows)))]
+                fork = torch.cuda.Event()
+                fork.record(cur)
+                for st in self._streams:
+                    st.wait_event(fork)
+                for w, flow in enumerate(self.flows):
+                    flow._launch(self._streams[w % len(self._streams)])
+                for st in self._streams:
+                    join = torch.cuda.Event()
+                    join.record(st)
+                    cur.wait_event(join)
             self.captured = True
             return
         from .batch import Batch
