@@ -164,16 +164,17 @@ def bench_karman_speed(K: int = 2000, W: int = 50):
 def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
     """The reference's API pattern: LBM_Solver(512, 128, nworld) with a square
     body per world (configure_solid), solver.step() per step: the default
-    exact cut cells (worlds on two side streams), and the mask link rule
-    (cut_cell=False) batched (every world in one launch sequence) and world
-    by world."""
+    exact cut cells (one multi-world call; and, batched=False, world by
+    world on two side streams), and the mask link rule (cut_cell=False)
+    batched (every world in one launch sequence) and world by world."""
     import torch
 
     from openhomelbm_amd.lbm import LBM_Solver
 
     sq = np.array([[-0.02, -0.02], [0.02, -0.02], [0.02, 0.02], [-0.02, 0.02]], np.float32)
     out = {"worlds": nworld, "grid": [512, 128], "steps": K}
-    for name, batched, cut in (("cut_cell", True, True), ("batched", True, False), ("per_world", False, False)):
+    for name, batched, cut in (("cut_cell", True, True), ("cut_cell_streams", False, True), ("batched", True, False),
+                               ("per_world", False, False)):
         s = LBM_Solver(512, 128, solid_num=1, nworld=nworld, batched=batched, cut_cell=cut)
         for f in s.flows:
             f.vis_shear = 0.008

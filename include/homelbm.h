@@ -152,6 +152,18 @@ int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const f
                          const float* verts_last, const int32_t* counts, int32_t maxv, const float* radius2,
                          double* force_history, void* stream);
 
+/* hl_step_polygons for nworld independent worlds of one dtype, all at buffer
+ * `cur`: the world loop of LBM_Solver.step (envs/lbm/lbm_solver.py:97-141)
+ * in one call. Host arrays of nworld entries (maxv, and device pointers for
+ * the others); verts_last, force_history or any of their entries may be
+ * NULL. Per step, one gather and one band launch serve up to eight worlds;
+ * each world's state and force_history are bitwise those of its own
+ * hl_step_polygons call. */
+int32_t hl_step_polygons_worlds(const hl_env* envs, int32_t nworld, int32_t cur, int32_t nsteps,
+                                const float* const* verts, const float* const* verts_last,
+                                const int32_t* const* counts, const int32_t* maxv, const float* const* radius2,
+                                double* const* force_history, void* stream);
+
 /* FP64 sum of rho over all cells of state `cur` into device out[0]. */
 int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream);
 

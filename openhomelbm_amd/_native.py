@@ -21,7 +21,8 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_batch_create", "hl_batch_destroy", "hl_batch_set_bodies", "hl_batch_set_state",
            "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
-           "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons")
+           "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
+           "hl_step_polygons_worlds")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -103,11 +104,13 @@ def load():
     L.hl_body_capsules.argtypes = [I32, I32, P, I32, P, I32, P, P, P, P, P, P, P]
     L.hl_voxelize_mesh.argtypes = [env_p, P, P, P, I32, P, I32, P, P]
     L.hl_step_polygons.argtypes = [env_p, I32, I32, P, P, P, I32, P, P, P]
+    L.hl_step_polygons_worlds.argtypes = [env_p, I32, I32, I32, P, P, P, P, P, P, P]
     for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
                  "hl_set_tiling", "hl_get_tiling", "hl_batch_create", "hl_batch_set_bodies", "hl_batch_set_state",
                  "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
-                 "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons"):
+                 "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
+                 "hl_step_polygons_worlds"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

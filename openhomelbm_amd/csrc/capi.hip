@@ -362,6 +362,53 @@ int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const f
     return HL_OK;
 }
 
+int32_t hl_step_polygons_worlds(const hl_env* envs, int32_t nworld, int32_t cur, int32_t nsteps,
+                                const float* const* verts, const float* const* verts_last,
+                                const int32_t* const* counts, const int32_t* maxv, const float* const* radius2,
+                                double* const* force_history, void* stream)
+{
+    if (!envs) return fail(HL_E_INVALID, "envs is NULL");
+    if (nworld < 1 || nworld > (1 << 16)) return fail(HL_E_INVALID, "nworld must be in [1, 65536], got %d", nworld);
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (!verts || !counts || !maxv || !radius2)
+        return fail(HL_E_INVALID, "verts, counts, maxv and radius2 must be non-NULL");
+    std::vector<hl::EnvArgs> args((size_t)nworld);
+    std::vector<const float*> last((size_t)nworld);
+    std::vector<double*> fh((size_t)nworld);
+    int64_t total_width = 0;
+    for (int32_t w = 0; w < nworld; ++w) {
+        const hl_env* env = envs + w;
+        int32_t rc = validate(env);
+        if (rc) return rc;
+        if (env->lattice != HL_D2Q9) return fail(HL_E_UNSUPPORTED, "world %d: polygon cut cells are D2Q9 only", w);
+        if (env->dtype != envs[0].dtype) return fail(HL_E_INVALID, "world %d: every world needs the same dtype", w);
+        if (env->nsolid < 1) return fail(HL_E_INVALID, "world %d: polygon cut cells need nsolid >= 1", w);
+        if (!verts[w] || !counts[w] || !radius2[w])
+            return fail(HL_E_INVALID, "world %d: verts, counts and radius2 must be non-NULL", w);
+        if (maxv[w] < 1 || maxv[w] > (1 << 20))
+            return fail(HL_E_INVALID, "world %d: maxv must be in [1, 2^20], got %d", w, maxv[w]);
+        args[(size_t)w] = make_args(env, cur);
+        last[(size_t)w] = verts_last && verts_last[w] ? verts_last[w] : verts[w];
+        fh[(size_t)w] = force_history ? force_history[w] : nullptr;
+        if (fh[(size_t)w]) total_width += env->nsolid * 6;
+    }
+    if (nsteps == 0) return HL_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    int32_t rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    void* ws = nullptr;
+    ctx->list_valid = false;  // ws is reused below
+    rc = workspace(ctx, s, hl::polygon_worlds_workspace_bytes(total_width), &ws);
+    if (rc) return rc;
+    hipError_t e = hl::step_polygons_worlds(envs[0].dtype, args.data(), nworld, nsteps, verts, last.data(), counts,
+                                            maxv, radius2, fh.data(), ws, s);
+    if (e != hipSuccess) return hip_fail(e, "polygon worlds step launch");
+    return HL_OK;
+}
+
 int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
 {
     int32_t rc = validate(env);

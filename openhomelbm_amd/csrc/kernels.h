@@ -43,6 +43,15 @@ void set_tiling(int lattice, TileCfg c);
 TileCfg get_tiling(int lattice);
 
 hipError_t step_gather(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
+// Worlds stepped by one launch: their arguments travel in the kernel-argument
+// block (2.8 KB for eight), the world index is a grid dimension.
+constexpr int kWorldChunk = 8;
+struct WorldArgs {
+    EnvArgs a[kWorldChunk];
+};
+// D2Q9 gather step of worlds a[0 .. n) (n <= kWorldChunk), each cell computed
+// exactly as step_gather computes it.
+hipError_t step_gather_worlds(int dtype, const WorldArgs& w, int n, hipStream_t s);
 // D2Q9 gather step that also writes fx.nfix (set here) force rows of the
 // state it reads into fx.partial and sums fx.prev into fx.prev_out.
 hipError_t step_gather_forces(int dtype, const EnvArgs& a, FixArgs& fx, hipStream_t s);
@@ -104,6 +113,15 @@ size_t polygon_workspace_bytes();
 hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* verts, const float* verts_last,
                          const int32_t* counts, int maxv, const float* radius2, double* force_history, void* ws,
                          hipStream_t s);
+// step_polygons of nworld worlds that share nothing but the stream: per step,
+// one gather and one band launch per kWorldChunk worlds; the results are
+// those of step_polygons world by world. Per-world arrays of device
+// pointers (force_history entries may be null); ws at least
+// polygon_worlds_workspace_bytes(sum of nsolid * 6 over the worlds with forces).
+size_t polygon_worlds_workspace_bytes(int64_t total_width);
+hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int nsteps, const float* const* verts,
+                                const float* const* verts_last, const int32_t* const* counts, const int32_t* maxv,
+                                const float* const* radius2, double* const* force_history, void* ws, hipStream_t s);
 
 // Coupling with a rigid-body engine (coupling.hip); see include/homelbm.h.
 hipError_t couple_poses(int lattice, int nworld, int nbody, int nsolid, const float* pos, int pstride,

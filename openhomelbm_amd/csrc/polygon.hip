@@ -21,6 +21,8 @@
 
 #include <math.h>
 
+#include <vector>
+
 #include "gather_post.h"
 #include "kernels.h"
 #include "lbm_device.h"
@@ -242,8 +244,8 @@ constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 // gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
 template <typename T, int DT, bool LDSV>
-__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, double* partial, const double* prev,
-                                                   double* prev_out)
+__device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev,
+                                          double* prev_out)
 {
     extern __shared__ float lds_verts[];
     __shared__ int bbox[5];
@@ -375,6 +377,59 @@ __global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, double*
     for (int k = lane; k < width; k += 64) partial[(int64_t)blockIdx.x * width + k] = acc[k];
 }
 
+template <typename T, int DT, bool LDSV>
+__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, double* partial, const double* prev,
+                                                   double* prev_out)
+{
+    poly_band<T, DT, LDSV>(a, p, partial, prev, prev_out);
+}
+
+// Per-world band arguments of a chunk of worlds (with WorldArgs: 3.3 KB of
+// kernel arguments for eight worlds).
+struct PolyWorlds {
+    PolyArgs p[kWorldChunk];
+    double* rows[kWorldChunk];         // this step's force rows, or null
+    const double* prev[kWorldChunk];   // the previous step's rows, or null
+    double* prev_out[kWorldChunk];     // where their sums go
+};
+static_assert(sizeof(WorldArgs) + sizeof(PolyWorlds) <= 4000, "chunk arguments exceed the kernel-argument block");
+
+// The band of world blockIdx.y: blockIdx.x and gridDim.x play the roles they
+// have in k_poly_band, so every world's rows and sums are those of a
+// launch of its own.
+template <typename T, int DT, bool LDSV>
+__global__ void __launch_bounds__(64) k_poly_band_worlds(WorldArgs w, PolyWorlds p)
+{
+    const int k = blockIdx.y;
+    poly_band<T, DT, LDSV>(w.a[k], p.p[k], p.rows[k], p.prev[k], p.prev_out[k]);
+}
+
+// The closing sums of a chunk (k_sum_partials of each world's last rows):
+// output blockIdx.x of world blockIdx.y.
+__global__ void __launch_bounds__(64) k_sum_rows_worlds(WorldArgs w, PolyWorlds p)
+{
+    const int j = blockIdx.y, k = blockIdx.x, width = w.a[j].nsolid * 6;
+    const double* rows = p.prev[j];
+    if (!rows || k >= width) return;
+    double v = 0.0;
+    for (int r = threadIdx.x; r < kPolyWaves; r += 64) v += rows[(int64_t)r * width + k];
+    v = wave_sum(v);
+    if (threadIdx.x == 0) p.prev_out[j][k] = v;
+}
+
+template <typename T, int DT>
+hipError_t launch_band_worlds(const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
+{
+    int nv = 0;
+    for (int k = 0; k < n; ++k) nv = max(nv, w.a[k].nsolid * p.p[k].maxv * 2);
+    const dim3 grid(kPolyWaves, n);
+    if (nv <= kPolyLdsFloats)
+        hipLaunchKernelGGL((k_poly_band_worlds<T, DT, true>), grid, 64, (size_t)nv * sizeof(float), s, w, p);
+    else
+        hipLaunchKernelGGL((k_poly_band_worlds<T, DT, false>), grid, 64, 0, s, w, p);
+    return hipGetLastError();
+}
+
 template <typename T, int DT>
 hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev, double* prev_out,
                        hipStream_t s)
@@ -431,6 +486,76 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
         prev_out = fout;
     }
     if (e == hipSuccess && prev) e = sum_partials(prev, kPolyWaves, width, prev_out, s);
+    return e;
+}
+
+size_t polygon_worlds_workspace_bytes(int64_t total_width)
+{
+    return 256 + (size_t)(total_width > 0 ? total_width : 0) * 2 * kPolyWaves * sizeof(double);
+}
+
+hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int nsteps, const float* const* verts,
+                                const float* const* verts_last, const int32_t* const* counts, const int32_t* maxv,
+                                const float* const* radius2, double* const* force_history, void* ws, hipStream_t s)
+{
+    if (nworld < 1 || nsteps < 0) return hipErrorInvalidValue;
+    // a world with forces keeps two banks of kPolyWaves rows of its width,
+    // as step_polygons keeps them in its own workspace; worlds in order
+    double* rows0 = reinterpret_cast<double*>(static_cast<char*>(ws) + 256);
+    std::vector<size_t> at((size_t)nworld);
+    size_t off = 0;
+    for (int wi = 0; wi < nworld; ++wi) {
+        at[(size_t)wi] = off;
+        if (force_history && force_history[wi]) off += (size_t)2 * kPolyWaves * a0[wi].nsolid * 6;
+    }
+    hipError_t e = hipSuccess;
+    for (int k = 0; k <= nsteps && e == hipSuccess; ++k) {
+        for (int c0 = 0; c0 < nworld && e == hipSuccess; c0 += kWorldChunk) {
+            const int n = nworld - c0 < kWorldChunk ? nworld - c0 : kWorldChunk;
+            WorldArgs w{};
+            PolyWorlds p{};
+            int maxwidth = 0;
+            for (int j = 0; j < n; ++j) {
+                const int wi = c0 + j;
+                EnvArgs a = a0[wi];
+                if (k & 1) {
+                    a.src = a0[wi].dst;
+                    a.dst = const_cast<void*>(a0[wi].src);
+                }
+                a.mask = nullptr;
+                a.cut_bits = nullptr;
+                a.cut_cols = nullptr;
+                a.box[0] = a.box[2] = a.box[4] = 1 << 29;
+                a.box[1] = a.box[3] = a.box[5] = -(1 << 29);
+                w.a[j] = a;
+                const int width = a.nsolid * 6;
+                maxwidth = width > maxwidth ? width : maxwidth;
+                p.p[j] = PolyArgs{verts[wi], k == 0 ? verts_last[wi] : verts[wi], counts[wi], maxv[wi], radius2[wi]};
+                double* fh = force_history && width > 0 ? force_history[wi] : nullptr;
+                double* wr = rows0 + at[(size_t)wi];
+                const size_t bank = (size_t)kPolyWaves * width;
+                p.rows[j] = fh && k < nsteps ? wr + (size_t)(k & 1) * bank : nullptr;
+                p.prev[j] = fh && k > 0 ? wr + (size_t)((k - 1) & 1) * bank : nullptr;
+                p.prev_out[j] = fh && k > 0 ? fh + (int64_t)(k - 1) * width : nullptr;
+            }
+            if (k == nsteps) {  // the last step's rows: summed by a launch of their own
+                if (maxwidth > 0) {
+                    hipLaunchKernelGGL(k_sum_rows_worlds, dim3(maxwidth, n), 64, 0, s, w, p);
+                    e = hipGetLastError();
+                }
+                continue;
+            }
+            e = step_gather_worlds(dtype, w, n, s);
+            if (e != hipSuccess) break;
+            switch (dtype) {
+            case HL_F32: e = launch_band_worlds<float, HL_F32>(w, p, n, s); break;
+            case HL_F64: e = launch_band_worlds<double, HL_F64>(w, p, n, s); break;
+            case HL_F16: e = launch_band_worlds<float, HL_F16>(w, p, n, s); break;
+            case HL_BF16: e = launch_band_worlds<float, HL_BF16>(w, p, n, s); break;
+            default: e = hipErrorInvalidValue;
+            }
+        }
+    }
     return e;
 }
 

@@ -245,37 +245,45 @@ class Lattice:
         verts / verts_last device FP32 [nsolid, maxv, 2] transformed vertices of
         this and the previous step, counts device int32 [nsolid], radius2
         device FP32 [nsolid] squared bounding radii (lattice units)."""
-        if self.lattice != 9:
-            raise ValueError("polygon cut cells are D2Q9 only")
-        if self.nsolid < 1:
-            raise ValueError("polygon cut cells need at least one solid")
-        key = tuple((t.data_ptr(), t.dtype, tuple(t.shape)) if t is not None else None
-                    for t in (verts, verts_last, counts, radius2))
-        if key != getattr(self, "_poly_checked", None):  # checks once per set of tables
-            vs = [verts, counts, radius2] + ([verts_last] if verts_last is not None else [])
-            for t, dt in zip(vs, (torch.float32, torch.int32, torch.float32, torch.float32)):
-                if t.dtype != dt or not t.is_contiguous() or t.device != self.device:
-                    raise ValueError("verts / verts_last / radius2 FP32 and counts int32, contiguous, on the "
-                                     "lattice's device")
-            if verts.dim() != 3 or verts.shape[0] != self.nsolid or verts.shape[2] != 2:
-                raise ValueError(f"verts must be [nsolid={self.nsolid}, maxv, 2], got {tuple(verts.shape)}")
-            if verts_last is not None and verts_last.shape != verts.shape:
-                raise ValueError("verts_last must have the shape of verts")
-            if counts.numel() != self.nsolid or radius2.numel() != self.nsolid:
-                raise ValueError("counts and radius2 need one entry per solid")
-            self._poly_checked = key
-        fh = None
-        if force_history is not None:
-            if (force_history.dtype != torch.float64 or not force_history.is_contiguous()
-                    or force_history.numel() < nsteps * self.nsolid * 6 or force_history.device != self.device):
-                raise ValueError("force_history must be a contiguous FP64 device tensor [nsteps, nsolid, 6]")
-            fh = force_history.data_ptr()
+        self._check_polygons(verts, verts_last, counts, radius2)
+        fh = self._history_ptr(force_history, nsteps)
         e = self._step_env()
         N.check(self.lib.hl_step_polygons(C.byref(e), self.cur, int(nsteps), verts.data_ptr(),
                                           verts_last.data_ptr() if verts_last is not None else None,
                                           counts.data_ptr(), int(verts.shape[1]), radius2.data_ptr(), fh,
                                           _stream_ptr(stream)), "hl_step_polygons")
         self.cur ^= nsteps & 1
+
+    def _check_polygons(self, verts, verts_last, counts, radius2) -> None:
+        """The table checks of step_polygons, once per set of tables."""
+        if self.lattice != 9:
+            raise ValueError("polygon cut cells are D2Q9 only")
+        if self.nsolid < 1:
+            raise ValueError("polygon cut cells need at least one solid")
+        key = tuple((t.data_ptr(), t.dtype, tuple(t.shape)) if t is not None else None
+                    for t in (verts, verts_last, counts, radius2))
+        if key == getattr(self, "_poly_checked", None):
+            return
+        vs = [verts, counts, radius2] + ([verts_last] if verts_last is not None else [])
+        for t, dt in zip(vs, (torch.float32, torch.int32, torch.float32, torch.float32)):
+            if t.dtype != dt or not t.is_contiguous() or t.device != self.device:
+                raise ValueError("verts / verts_last / radius2 FP32 and counts int32, contiguous, on the "
+                                 "lattice's device")
+        if verts.dim() != 3 or verts.shape[0] != self.nsolid or verts.shape[2] != 2:
+            raise ValueError(f"verts must be [nsolid={self.nsolid}, maxv, 2], got {tuple(verts.shape)}")
+        if verts_last is not None and verts_last.shape != verts.shape:
+            raise ValueError("verts_last must have the shape of verts")
+        if counts.numel() != self.nsolid or radius2.numel() != self.nsolid:
+            raise ValueError("counts and radius2 need one entry per solid")
+        self._poly_checked = key
+
+    def _history_ptr(self, force_history: Optional[torch.Tensor], nsteps: int):
+        if force_history is None:
+            return None
+        if (force_history.dtype != torch.float64 or not force_history.is_contiguous()
+                or force_history.numel() < nsteps * self.nsolid * 6 or force_history.device != self.device):
+            raise ValueError("force_history must be a contiguous FP64 device tensor [nsteps, nsolid, 6]")
+        return force_history.data_ptr()
 
     def capture(self, nsteps: int, kernel: int = N.HL_KERNEL_AUTO,
                 force_history: Optional[torch.Tensor] = None) -> "StepGraph":
@@ -518,3 +526,51 @@ class StepGraph:
         current stream, asynchronously)."""
         self.graphs[self.lat.cur].replay()
         self.lat.cur ^= self.nsteps & 1
+
+
+def step_polygons_worlds(lattices: Sequence["Lattice"], nsteps: int, tables, force_histories=None,
+                         stream=None) -> None:
+    """Lattice.step_polygons of several D2Q9 worlds of one dtype, all at the
+    same buffer, in one library call (hl_step_polygons_worlds): the world loop
+    of LBM_Solver.step (envs/lbm/lbm_solver.py:97-141). tables[w] =
+    (verts, verts_last or None, counts, radius2) of world w; force_histories
+    None or one tensor (or None) per world. Each world ends bitwise where
+    its own step_polygons call would."""
+    lats = list(lattices)
+    n = len(lats)
+    if n == 0:
+        return
+    if len(tables) != n or (force_histories is not None and len(force_histories) != n):
+        raise ValueError("one table set (and force history) per world")
+    cur = lats[0].cur
+    for L in lats:
+        if L.cur != cur:
+            raise ValueError("the worlds must share dtype, device and current buffer")
+    hists = [None] * n if force_histories is None else list(force_histories)
+    envs = [L._step_env() for L in lats]
+    hold = lats[0]
+    cached = getattr(hold, "_worlds_args", None)
+    # the same objects as last call (the cache holds them, so no new object
+    # can take an old one's identity): the checks and argument arrays stand
+    same = cached is not None and cached[0] == nsteps and len(cached[1]) == n and all(
+        a is b for a, b in zip(cached[1], lats)) and all(a is b for a, b in zip(cached[2], envs)) and all(
+        x is y for a, b in zip(cached[3], tables) for x, y in zip(a, b)) and all(
+        a is b for a, b in zip(cached[4], hists))
+    if not same:
+        for L, (v, vl, cnt, r2) in zip(lats, tables):
+            if L.dtype_code != lats[0].dtype_code or L.device != lats[0].device:
+                raise ValueError("the worlds must share dtype, device and current buffer")
+            L._check_polygons(v, vl, cnt, r2)
+        fhs = [L._history_ptr(h, nsteps) for L, h in zip(lats, hists)]
+        P = C.c_void_p
+        args = ((N.HLEnv * n)(*envs), (P * n)(*[t[0].data_ptr() for t in tables]),
+                (P * n)(*[t[1].data_ptr() if t[1] is not None else None for t in tables]),
+                (P * n)(*[t[2].data_ptr() for t in tables]), (C.c_int32 * n)(*[int(t[0].shape[1]) for t in tables]),
+                (P * n)(*[t[3].data_ptr() for t in tables]), (P * n)(*fhs))
+        cached = (nsteps, list(lats), envs, [tuple(t) for t in tables], hists, args)  # tables' tensors held
+        hold._worlds_args = cached
+    a = cached[5]
+    N.check(lats[0].lib.hl_step_polygons_worlds(a[0], n, cur, int(nsteps), a[1], a[2], a[3], a[4], a[5], a[6],
+                                                _stream_ptr(stream)), "hl_step_polygons_worlds")
+    for L in lats:
+        L.cur ^= nsteps & 1

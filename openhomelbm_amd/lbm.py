@@ -9,7 +9,8 @@ Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
 * Bodies are closed polygons (configure_solid, lbm_core.py:172-282). By
   default they take the reference's own boundary: the segment-link
   predicate and material-point wall rule (get_cutcell, lbm_func.py:545-607;
-  hl_step_polygons). LBM_Solver(..., cut_cell=False) (or HomeFlow.cut_cell
+  hl_step_polygons; several such worlds of an LBM_Solver step in one
+  hl_step_polygons_worlds call). LBM_Solver(..., cut_cell=False) (or HomeFlow.cut_cell
   = False) voxelizes them into an integer mask with the reference's even-odd
   inside test and runs the wall rule on mask links (mask(x) != mask(x-c_i)):
   faster, and several worlds then step in one batched launch sequence, but
@@ -34,7 +35,7 @@ import torch
 
 from . import _native as N
 from .arrays import ForceField, HostArray, LatticeField, as_numpy
-from .lattice import Lattice
+from .lattice import Lattice, step_polygons_worlds
 
 ML_FLUID, ML_SOLID, ML_WALL = 0, 1, 2
 
@@ -437,6 +438,12 @@ class LBM_Solver:
             and not f._uses_cut_cells()
             for f in self.flows)
 
+    def _polygon_worlds(self) -> bool:
+        """Several worlds, all on the exact cut cells, of one dtype and at
+        the same buffer: the multi-world polygon route."""
+        return self._use_batch and len(self.flows) > 1 and all(f._uses_cut_cells() for f in self.flows) and len(
+            {(f.lattice.dtype_code, f.lattice.cur) for f in self.flows}) == 1
+
     @staticmethod
     def _descriptor_key(L):
         """What a batch keeps of a world's descriptor, except the mask box
@@ -454,6 +461,13 @@ class LBM_Solver:
         if not self._batchable():
             if len(self.flows) == 1:
                 self.flows[0].step()
+            elif self._polygon_worlds():
+                # every world on the exact cut cells: one library call, one
+                # gather and one band launch per eight worlds
+                for flow in self.flows:
+                    flow._prepare()
+                step_polygons_worlds([f.lattice for f in self.flows], 1, [f._seg_dev for f in self.flows],
+                                     [f._forces for f in self.flows])
             else:
                 # worlds are independent: their launches go round-robin to two
                 # side streams (default priority, each with its own library

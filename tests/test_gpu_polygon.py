@@ -202,10 +202,10 @@ def _env3(nx, ny, pos):
     return env
 
 
-def _polygon_worlds(nworld, steps=5):
+def _polygon_worlds(nworld, steps=5, batched=True):
     """nworld worlds with moving polygons (the default exact boundary) and a
     runtime inlet change; returns the solver and the per-step forces."""
-    s = LBM_Solver(NX, NY, solid_num=1, nworld=nworld)
+    s = LBM_Solver(NX, NY, solid_num=1, nworld=nworld, batched=batched)
     shape = np.array([[-0.1, -0.05], [0.12, -0.06], [0.08, 0.07], [-0.09, 0.05]], np.float32)
     for w, f in enumerate(s.flows):
         f.vis_shear = 0.015 + 0.003 * w
@@ -224,11 +224,16 @@ def _polygon_worlds(nworld, steps=5):
     return s, np.array(forces)
 
 
-def test_worlds_on_side_streams_equal_each_world_alone():
-    """Several cut-cell worlds step on concurrent side streams; every world's
-    state and forces equal those of the same world stepped in a solver of its own."""
-    s, F = _polygon_worlds(5)
-    assert s._streams is not None and len(s._streams) == 2
+@pytest.mark.parametrize("batched", [True, False])
+def test_worlds_equal_each_world_alone(batched):
+    """Several cut-cell worlds step in one multi-world call (batched) or on
+    concurrent side streams; every world's state and forces equal those of
+    the same world stepped in a solver of its own."""
+    s, F = _polygon_worlds(5, batched=batched)
+    if batched:
+        assert s._streams is None
+    else:
+        assert s._streams is not None and len(s._streams) == 2
     for w in range(5):
         alone = LBM_Solver(NX, NY, solid_num=1, nworld=1)
         f = alone.flows[0]
@@ -248,3 +253,80 @@ def test_worlds_on_side_streams_equal_each_world_alone():
         torch.cuda.synchronize()
         assert np.array_equal(f.lattice.get_state(np.float32), s.flows[w].lattice.get_state(np.float32)), w
     assert np.abs(F).max() > 0
+
+
+def _world_tables(w, k):
+    """World w's bodies at step k: world w has 1 + w % 2 solids, shifted and
+    turned by w (vertex tables of 6 or 7 rows)."""
+    v, cnt, pos = _bodies(k)
+    ns = 1 + w % 2
+    dx = 0.37 * (w % 5)
+    vv = np.zeros((ns, 6 + w % 2, 2), np.float32)
+    vv[:, :6] = v[:ns]
+    vv[:, :, 0] += np.where(np.arange(vv.shape[1]) < np.r_[cnt[:ns]][:, None], dx, 0.0)
+    p = pos[:ns].copy()
+    p[:, 0] += dx
+    return vv, cnt[:ns].copy(), p
+
+
+def test_step_polygons_worlds_equal_per_world_calls():
+    """hl_step_polygons_worlds over 11 worlds (a chunk of eight and one of
+    three) with 1 or 2 solids, differing vertex rows, viscosities and faces,
+    moving bodies, force histories on some worlds only, three steps per
+    call and two calls: every world's state and force rows are bitwise
+    those of its own hl_step_polygons calls."""
+    from openhomelbm_amd.lattice import step_polygons_worlds
+    nw, nsteps = 11, 3
+    many, alone = [], []
+    for w in range(nw):
+        ns = 1 + w % 2
+        for group in (many, alone):
+            lat = Lattice(9, NX, NY, dtype="f32", nsolid=ns)
+            lat.nu = 0.012 + 0.001 * w
+            lat.bc_type = [0, 1, 1, 2 if w % 3 else 1, 1, 1]
+            lat.bc_value = [[0.03 + 0.002 * w, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+            lat.set_state(random_state(9, (NY, NX), seed=70 + w).astype(np.float32))
+            group.append(lat)
+    for call in range(2):
+        tables, hists, hist_alone = [], [], []
+        for w in range(nw):
+            v, cnt, pos = _world_tables(w, call + 1)
+            vl, _, pos_last = _world_tables(w, call)
+            ns = len(cnt)
+            P = np.zeros((ns, 8), np.float32)
+            P[:, 0:2], P[:, 2] = pos, 1.0
+            P[:, 4:6], P[:, 6] = pos_last, 1.0
+            r2 = np.full(ns, 9.5 ** 2, np.float32)
+            many[w].set_poses(P)
+            alone[w].set_poses(P)
+            tables.append((_dev(v), _dev(vl), _dev(cnt), _dev(r2)))
+            with_forces = w % 4 != 3
+            h = torch.zeros((nsteps, ns, 6), dtype=torch.float64, device="cuda") if with_forces else None
+            ha = torch.zeros((nsteps, ns, 6), dtype=torch.float64, device="cuda") if with_forces else None
+            hists.append(h)
+            hist_alone.append(ha)
+            alone[w].step_polygons(nsteps, *tables[w], force_history=ha)
+        step_polygons_worlds(many, nsteps, tables, hists)
+        torch.cuda.synchronize()
+        for w in range(nw):
+            assert many[w].cur == alone[w].cur == (call + 1) * nsteps % 2
+            assert np.array_equal(many[w].get_state(np.float32), alone[w].get_state(np.float32)), (call, w)
+            if hists[w] is not None:
+                got, ref = hists[w].cpu().numpy(), hist_alone[w].cpu().numpy()
+                assert np.array_equal(got, ref), (call, w)
+                assert np.abs(ref[:, :, [0, 1, 5]]).min() > 0
+
+
+def test_step_polygons_worlds_errors():
+    from openhomelbm_amd.lattice import step_polygons_worlds
+    a, b = _lattice(), _lattice()
+    v, cnt, _ = _bodies(0)
+    t = (_dev(v), None, _dev(cnt), _dev(R2))
+    b.cur = 1
+    with pytest.raises(ValueError, match="current buffer"):
+        step_polygons_worlds([a, b], 1, [t, t])
+    b.cur = 0
+    with pytest.raises(ValueError, match="one table set"):
+        step_polygons_worlds([a, b], 1, [t])
+    with pytest.raises(ValueError, match="counts int32"):
+        step_polygons_worlds([a, b], 1, [t, (t[0], None, t[2].float(), t[3])])
