@@ -243,6 +243,107 @@ constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 // momentum exchange; the lane of link 1 also pulls the rest population,
 // gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
+// The band with one lane per cell, for bands too large to share links
+// among lanes: lane c of a pass takes all eight links of its cell in turn,
+// each with the arithmetic of the lane-per-link path (so the same state),
+// and the per-solid wave sums of each link's momentum exchange go into acc
+// (lane 0) link by link.
+template <typename T, int DT>
+__device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs& p, const float* verts,
+                                                double* partial, double* acc, int64_t n, int bw, int bx0, int by0)
+{
+    using S = Storage<DT>;
+    using ST = typename S::type;
+    const ST* src = static_cast<const ST*>(a.src);
+    ST* dst = static_cast<ST*>(a.dst);
+    const int lane = threadIdx.x;
+    const int nx = a.nx, ny = a.ny;
+    const T grid_length = T(1) / T(nx);
+    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+        const int64_t t = base + lane;
+        int x = 0, y = 0, px = 0, py = 0;
+        uint64_t near = 0;
+        if (t < n) {
+            x = bx0 + (int)(t % bw);
+            y = by0 + (int)(t / bw);
+            px = min(max(x, 1), nx - 2);
+            py = min(max(y, 1), ny - 2);
+            near = poly_near<T>(a, p, px, py);
+        }
+        const bool interior = px == x && py == y;
+        const int64_t c = (int64_t)px + (int64_t)nx * py;
+        T mc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        if (near) load_cell<T, DT>(a, c, 6, mc);
+        T pp[9];
+#pragma unroll
+        for (int i = 1; i < 9; ++i) {
+            int sid = -1;
+            T F[3] = {T(0), T(0), T(0)};
+            T q = T(0);
+            if (near) {
+                T ratio = 0;
+                int seg = -1;
+                sid = poly_cut_split<T>(p, verts, near, px, py, i, 0, 1, &ratio, &seg);
+                if (sid >= 0) {
+                    T up[2], r[2];
+                    poly_point<T>(a, p, sid, seg, ratio, up, r);
+                    const T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+                    const T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+                    const T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+                    q = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+                    if (q < T(0)) q = T(0);
+                    if (partial && interior) {
+                        const int inv = D2INV[i];
+                        const T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+                        const T Fx = cur_f * (T(D2CX[inv]) - up[0]) - q * (T(D2CX[i]) - up[0]);
+                        const T Fy = cur_f * (T(D2CY[inv]) - up[1]) - q * (T(D2CY[i]) - up[1]);
+                        F[0] = Fx;
+                        F[1] = Fy;
+                        F[2] = (r[0] * Fy - r[1] * Fx) * grid_length;
+                    } else {
+                        sid = -1;
+                    }
+                } else {
+                    const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
+                    T m[6];
+#pragma unroll
+                    for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                    q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+                    if (q < T(0)) q = T(0);
+                }
+            }
+            pp[i] = q;
+            if (partial) {
+                uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
+                while (uni) {
+                    const int s = __builtin_ctzll(uni);
+                    uni &= uni - 1;
+                    const bool m = sid == s;
+                    const double fx = wave_sum(m ? (double)F[0] : 0.0), fy = wave_sum(m ? (double)F[1] : 0.0);
+                    const double tz = wave_sum(m ? (double)F[2] : 0.0);
+                    if (lane == 0) {
+                        acc[s * 6 + 0] += fx;
+                        acc[s * 6 + 1] += fy;
+                        acc[s * 6 + 5] += tz;
+                    }
+                }
+            }
+        }
+        if (near) {
+            T q0 = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], 0);
+            if (q0 < T(0)) q0 = T(0);
+            pp[0] = q0;
+            T Fx = 0, Fy = 0, out[6];
+            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+            d2q9_collide<T>(a, pp, Fx, Fy, out);
+            if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
+            const int64_t d = (int64_t)x + (int64_t)nx * y;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+        }
+    }
+}
+
 template <typename T, int DT, bool LDSV>
 __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev,
                                           double* prev_out)
@@ -274,6 +375,9 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
     const int bw = bbox[1] - bbox[0], bh = bbox[3] - bbox[2];
     const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
     const T grid_length = T(1) / T(nx);
+    if (G == 1)  // a large band: one lane per cell
+        poly_band_cells<T, DT>(a, p, verts, partial, acc, n, bw, bx0, by0);
+    else
     for (int64_t base = (int64_t)blockIdx.x * cells; base < n; base += (int64_t)gridDim.x * cells) {
         const int64_t t = base + slot;
         int sid = -1;

@@ -193,6 +193,49 @@ def test_random_polygons_bitwise_against_sequential_oracle():
         np.testing.assert_allclose(hist[0].cpu().numpy(), F, rtol=1e-10, atol=1e-13)
 
 
+@pytest.mark.parametrize("radius,dtype", [(6.0, "f32"), (22.0, "f32"), (45.0, "f32"), (1000.0, "f32"),
+                                          (1000.0, "f64")])
+def test_every_band_layout_matches_oracle(radius, dtype):
+    """Bounding radii whose band takes each lane layout on a 200x104 lattice
+    (radius 6: 775 band cells, eight lanes per link; 22: 3,591 cells, four;
+    45: 11,118 cells, two; 1000: the whole lattice, one lane per cell, as
+    the reference's radius usually gives): three
+    steps with two moving polygons, bitwise against the oracle, forces to
+    1e-10."""
+    nx, ny = 200, 104
+    prec = np.float32 if dtype == "f32" else np.float64
+    lat = Lattice(9, nx, ny, dtype=dtype, nsolid=2)
+    lat.nu = 0.02
+    lat.bc_type = [0, 1, 1, 1, 1, 1]
+    lat.bc_value = [[0.03, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+    m = random_state(9, (ny, nx), seed=5).astype(prec)
+    lat.set_state(m)
+    ref = m
+
+    def bodies(k):
+        v = np.zeros((2, 7, 2), np.float32)
+        v[0] = _poly((98.4 + 0.3 * k, 52.2 - 0.15 * k), 8.3, 7, 0.1 * k)
+        v[1, :4] = _poly((109.6, 47.1 + 0.2 * k), 4.2, 4, 0.4)
+        pos = np.array([[98.4 + 0.3 * k, 52.2 - 0.15 * k], [109.6, 47.1 + 0.2 * k]], np.float32)
+        return v, np.array([7, 4], np.int32), pos
+
+    r2 = np.array([radius ** 2, radius ** 2], np.float32)
+    v_prev, _, pos_prev = bodies(0)
+    for k in range(3):
+        v, cnt, pos = bodies(k)
+        env = _env3(nx, ny, pos)
+        env.solids[:, 4:6] = pos_prev
+        lat.set_poses(env.solids)
+        hist = torch.zeros((1, 2, 6), dtype=torch.float64, device="cuda")
+        lat.step_polygons(1, _dev(v), _dev(v_prev), _dev(cnt), _dev(r2), force_history=hist)
+        ref, F = orc.step_polygons(env, ref, v, v_prev, cnt, r2, precision=dtype)
+        torch.cuda.synchronize()
+        assert np.array_equal(lat.get_state(prec), ref), k
+        np.testing.assert_allclose(hist[0].cpu().numpy(), F, rtol=1e-10, atol=1e-13)
+        assert np.abs(F[:, [0, 1, 5]]).min() > 0
+        v_prev, pos_prev = v, pos
+
+
 def _env3(nx, ny, pos):
     env = orc.Env(9, nx, ny, nu=0.02, bc_type=(0, 1, 1, 1), bc_value=((0.03, 0, 0),) + ((0, 0, 0),) * 5)
     P = np.zeros((len(pos), 8), np.float32)
