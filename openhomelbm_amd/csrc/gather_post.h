@@ -322,4 +322,29 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
     }
 }
 
+// Cell (x, y) of a D2Q9 gather step (x, y inside the lattice): a face cell
+// takes the update of its interior projection, then the face chain.
+template <typename T, int DT>
+__device__ __forceinline__ void gather_d2q9_at(const EnvArgs& a, int x, int y)
+{
+    const int px = min(max(x, 1), a.nx - 2), py = min(max(y, 1), a.ny - 2);
+    T m[6];
+    d2q9_post<T, DT>(a, px, py, m);
+    if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, a.nx, a.ny, 1, a.bc_type, a.bc_value);
+    using ST = typename Storage<DT>::type;
+    ST* dst = static_cast<ST*>(a.dst);
+    const int64_t c = (int64_t)x + (int64_t)a.nx * y;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+}
+
+// Cell (region x0 + bx, y0 + by) of a D2Q9 gather step.
+template <typename T, int DT>
+__device__ __forceinline__ void gather_d2q9_cell(const EnvArgs& a, int bx, int by)
+{
+    const int x = a.region[0] + bx, y = a.region[2] + by;
+    if (x >= a.region[1] || y >= a.region[3]) return;
+    gather_d2q9_at<T, DT>(a, x, y);
+}
+
 }  // namespace hl

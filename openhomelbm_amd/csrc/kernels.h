@@ -49,9 +49,6 @@ constexpr int kWorldChunk = 8;
 struct WorldArgs {
     EnvArgs a[kWorldChunk];
 };
-// D2Q9 gather step of worlds a[0 .. n) (n <= kWorldChunk), each cell computed
-// exactly as step_gather computes it.
-hipError_t step_gather_worlds(int dtype, const WorldArgs& w, int n, hipStream_t s);
 // D2Q9 gather step that also writes fx.nfix (set here) force rows of the
 // state it reads into fx.partial and sums fx.prev into fx.prev_out.
 hipError_t step_gather_forces(int dtype, const EnvArgs& a, FixArgs& fx, hipStream_t s);

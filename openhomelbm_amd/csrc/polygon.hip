@@ -5,7 +5,8 @@
 // envs/lbm/lbm_func.py:545-607) and, on a crossing, applies the wall rule at
 // the crossing's material point (stream_and_collide, lbm_func.py:346-401).
 // Far from bodies that is the plain pull step, so a step here is:
-//   1. the gather kernel over the whole lattice with no mask (plain pull);
+//   1. the gather kernel with no mask (plain pull) over the cells whose
+//      interior projection lies in no bounding circle (k_poly_gather);
 //   2. k_poly_band over the bounding box of the bounding circles (each
 //      workgroup finds it first, poly_bbox): cells whose interior projection lies in a
 //      circle are recomputed with the segment predicate and rewritten, face
@@ -244,10 +245,11 @@ constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 // gathers the nine through LDS and collides. Force rows:
 // per solid a wavefront sum per pass, accumulated in pass order (FP64).
 // The band with one lane per cell, for bands too large to share links
-// among lanes: lane c of a pass takes all eight links of its cell in turn,
-// each with the arithmetic of the lane-per-link path (so the same state),
-// and the per-solid wave sums of each link's momentum exchange go into acc
-// (lane 0) link by link.
+// among lanes: lane c of a pass pulls all eight links of its cell, then
+// tests them in turn and replaces the cut ones by the wall rule, each value
+// with the arithmetic of the lane-per-link path (so the same state); the
+// per-solid wave sums of each link's momentum exchange go into acc (lane 0)
+// link by link.
 template <typename T, int DT>
 __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs& p, const float* verts,
                                                 double* partial, double* acc, int64_t n, int bw, int bx0, int by0)
@@ -273,18 +275,32 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
         const bool interior = px == x && py == y;
         const int64_t c = (int64_t)px + (int64_t)nx * py;
         T mc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        if (near) load_cell<T, DT>(a, c, 6, mc);
         T pp[9];
+        if (near) {
+            // every link's pull first, so that their loads issue together;
+            // a cut link's value is replaced below
+            load_cell<T, DT>(a, c, 6, mc);
+#pragma unroll
+            for (int i = 1; i < 9; ++i) {
+                const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
+                T m[6];
+#pragma unroll
+                for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                T q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
+                if (q < T(0)) q = T(0);
+                pp[i] = q;
+            }
+        }
 #pragma unroll
         for (int i = 1; i < 9; ++i) {
             int sid = -1;
             T F[3] = {T(0), T(0), T(0)};
-            T q = T(0);
             if (near) {
                 T ratio = 0;
                 int seg = -1;
                 sid = poly_cut_split<T>(p, verts, near, px, py, i, 0, 1, &ratio, &seg);
                 if (sid >= 0) {
+                    T q;
                     T up[2], r[2];
                     poly_point<T>(a, p, sid, seg, ratio, up, r);
                     const T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
@@ -292,6 +308,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
                     const T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
                     q = d2q9_recon<T>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
                     if (q < T(0)) q = T(0);
+                    pp[i] = q;
                     if (partial && interior) {
                         const int inv = D2INV[i];
                         const T cur_f = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
@@ -303,16 +320,8 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
                     } else {
                         sid = -1;
                     }
-                } else {
-                    const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
-                    T m[6];
-#pragma unroll
-                    for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
-                    q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
-                    if (q < T(0)) q = T(0);
                 }
             }
-            pp[i] = q;
             if (partial) {
                 uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
                 while (uni) {
@@ -521,6 +530,69 @@ __global__ void __launch_bounds__(64) k_sum_rows_worlds(WorldArgs w, PolyWorlds 
     if (threadIdx.x == 0) p.prev_out[j][k] = v;
 }
 
+// The gather step of the cells the band leaves alone: a cell whose interior
+// projection lies in a bounding circle is written by the band launch that
+// follows (the band's box holds every such cell), so it is skipped here.
+template <typename T, int DT>
+__device__ __forceinline__ void poly_gather_cell(const EnvArgs& a, const PolyArgs& p, int bx, int by)
+{
+    const int x = a.region[0] + bx, y = a.region[2] + by;
+    if (x >= a.region[1] || y >= a.region[3]) return;
+    const int px = min(max(x, 1), a.nx - 2), py = min(max(y, 1), a.ny - 2);
+    if (poly_near<T>(a, p, px, py)) return;
+    gather_d2q9_at<T, DT>(a, x, y);
+}
+
+template <typename T, int DT>
+__global__ void __launch_bounds__(256) k_poly_gather(EnvArgs a, PolyArgs p)
+{
+    poly_gather_cell<T, DT>(a, p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
+}
+
+// World blockIdx.z of the chunk; the grid covers its largest region.
+template <typename T, int DT>
+__global__ void __launch_bounds__(256) k_poly_gather_worlds(WorldArgs w, PolyWorlds p)
+{
+    const int k = blockIdx.z;
+    poly_gather_cell<T, DT>(w.a[k], p.p[k], blockIdx.x * blockDim.x + threadIdx.x,
+                            blockIdx.y * blockDim.y + threadIdx.y);
+}
+
+hipError_t launch_poly_gather(int dtype, const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
+{
+    int wmax = 0, hmax = 0;
+    for (int k = 0; k < n; ++k) {
+        const int* r = w.a[k].region;
+        wmax = max(wmax, r[1] - r[0]);
+        hmax = max(hmax, r[3] - r[2]);
+    }
+    if (wmax <= 0 || hmax <= 0) return hipSuccess;
+    const dim3 block(64, 4, 1), grid((wmax + 63) / 64, (hmax + 3) / 4, n);
+    switch (dtype) {
+    case HL_F32: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_F32>), grid, block, 0, s, w, p); break;
+    case HL_F64: hipLaunchKernelGGL((k_poly_gather_worlds<double, HL_F64>), grid, block, 0, s, w, p); break;
+    case HL_F16: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_F16>), grid, block, 0, s, w, p); break;
+    case HL_BF16: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_BF16>), grid, block, 0, s, w, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_poly_gather(int dtype, const EnvArgs& a, const PolyArgs& p, hipStream_t s)
+{
+    const int w = a.region[1] - a.region[0], h = a.region[3] - a.region[2];
+    if (w <= 0 || h <= 0) return hipSuccess;
+    const dim3 block(64, 4, 1), grid((w + 63) / 64, (h + 3) / 4, 1);
+    switch (dtype) {
+    case HL_F32: hipLaunchKernelGGL((k_poly_gather<float, HL_F32>), grid, block, 0, s, a, p); break;
+    case HL_F64: hipLaunchKernelGGL((k_poly_gather<double, HL_F64>), grid, block, 0, s, a, p); break;
+    case HL_F16: hipLaunchKernelGGL((k_poly_gather<float, HL_F16>), grid, block, 0, s, a, p); break;
+    case HL_BF16: hipLaunchKernelGGL((k_poly_gather<float, HL_BF16>), grid, block, 0, s, a, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <typename T, int DT>
 hipError_t launch_band_worlds(const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
 {
@@ -577,7 +649,7 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
         p.verts_last = k == 0 ? verts_last : verts;
         double* fout = force_history && width > 0 ? force_history + (int64_t)k * width : nullptr;
         double* rows = fout ? partial + (size_t)(k & 1) * kPolyWaves * HL_MAX_SOLIDS * 6 : nullptr;
-        e = step_gather(HL_D2Q9, dtype, a, s);
+        e = launch_poly_gather(dtype, a, p, s);
         if (e != hipSuccess) break;
         switch (dtype) {
         case HL_F32: e = launch_band<float, HL_F32>(a, p, rows, prev, prev_out, s); break;
@@ -649,7 +721,7 @@ hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int ns
                 }
                 continue;
             }
-            e = step_gather_worlds(dtype, w, n, s);
+            e = launch_poly_gather(dtype, w, p, n, s);
             if (e != hipSuccess) break;
             switch (dtype) {
             case HL_F32: e = launch_band_worlds<float, HL_F32>(w, p, n, s); break;

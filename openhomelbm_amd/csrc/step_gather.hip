@@ -19,35 +19,10 @@
 
 namespace hl {
 
-// Cell (region x0 + bx, y0 + by) of a D2Q9 gather step.
-template <typename T, int DT>
-__device__ __forceinline__ void gather_d2q9_cell(const EnvArgs& a, int bx, int by)
-{
-    const int x = a.region[0] + bx, y = a.region[2] + by;
-    if (x >= a.region[1] || y >= a.region[3]) return;
-    const int px = min(max(x, 1), a.nx - 2), py = min(max(y, 1), a.ny - 2);
-    T m[6];
-    d2q9_post<T, DT>(a, px, py, m);
-    if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, a.nx, a.ny, 1, a.bc_type, a.bc_value);
-    using ST = typename Storage<DT>::type;
-    ST* dst = static_cast<ST*>(a.dst);
-    const int64_t c = (int64_t)x + (int64_t)a.nx * y;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
-}
-
 template <typename T, int DT>
 __global__ void __launch_bounds__(256) k_gather_d2q9(EnvArgs a)
 {
     gather_d2q9_cell<T, DT>(a, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
-}
-
-// World blockIdx.z of w; the grid covers the largest region of the chunk.
-template <typename T, int DT>
-__global__ void __launch_bounds__(256) k_gather_d2q9_worlds(WorldArgs w)
-{
-    gather_d2q9_cell<T, DT>(w.a[blockIdx.z], blockIdx.x * blockDim.x + threadIdx.x,
-                            blockIdx.y * blockDim.y + threadIdx.y);
 }
 
 template <typename T, int DT>
@@ -179,27 +154,6 @@ hipError_t step_gather_forces(int dtype, const EnvArgs& a, FixArgs& fx, hipStrea
     case HL_F64: hipLaunchKernelGGL((k_gather_d2q9_fx<double, HL_F64>), grid, block, 0, s, a, fx); break;
     case HL_F16: hipLaunchKernelGGL((k_gather_d2q9_fx<float, HL_F16>), grid, block, 0, s, a, fx); break;
     case HL_BF16: hipLaunchKernelGGL((k_gather_d2q9_fx<float, HL_BF16>), grid, block, 0, s, a, fx); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t step_gather_worlds(int dtype, const WorldArgs& w, int n, hipStream_t s)
-{
-    if (n < 1 || n > kWorldChunk) return hipErrorInvalidValue;
-    int wmax = 0, hmax = 0;
-    for (int k = 0; k < n; ++k) {
-        const int* r = w.a[k].region;
-        wmax = max(wmax, r[1] - r[0]);
-        hmax = max(hmax, r[3] - r[2]);
-    }
-    if (wmax <= 0 || hmax <= 0) return hipSuccess;
-    const dim3 block(64, 4, 1), grid((wmax + 63) / 64, (hmax + 3) / 4, n);
-    switch (dtype) {
-    case HL_F32: hipLaunchKernelGGL((k_gather_d2q9_worlds<float, HL_F32>), grid, block, 0, s, w); break;
-    case HL_F64: hipLaunchKernelGGL((k_gather_d2q9_worlds<double, HL_F64>), grid, block, 0, s, w); break;
-    case HL_F16: hipLaunchKernelGGL((k_gather_d2q9_worlds<float, HL_F16>), grid, block, 0, s, w); break;
-    case HL_BF16: hipLaunchKernelGGL((k_gather_d2q9_worlds<float, HL_BF16>), grid, block, 0, s, w); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
