@@ -4,20 +4,24 @@
 // the polygon bodies whose bounding circle holds the cell (get_cutcell,
 // envs/lbm/lbm_func.py:545-607) and, on a crossing, applies the wall rule at
 // the crossing's material point (stream_and_collide, lbm_func.py:346-401).
-// Far from bodies that is the plain pull step, so a step here is:
-//   1. the gather kernel with no mask (plain pull) over the cells whose
-//      interior projection lies in no bounding circle (k_poly_gather);
-//   2. k_poly_band over the bounding box of the bounding circles (each
-//      workgroup finds it first, poly_bbox): cells whose interior projection lies in a
-//      circle are recomputed with the segment predicate and rewritten, face
-//      cells through the same face chain as the gather kernel; interior
-//      cells also emit the momentum exchange of their cut links. One lane
-//      per link (eight cells per wavefront pass), the vertex tables in LDS;
-//   3. the sum of the band's fixed number of per-wave FP64 rows, done by the
-//      next step's band launch (k_sum_partials after the last step).
-// The band kernel is a fixed grid of waves striding over the box, so the
-// FP64 force order does not depend on scheduling. Arithmetic follows the
-// reference's operation order with contraction off.
+// Far from bodies that is the plain pull step, so a step here is one
+// launch of k_poly_step, whose workgroups take one of two roles:
+//   1. the band (the first kPolyWaves workgroups, one wave each) over the
+//      bounding box of the bounding circles (each workgroup finds it first,
+//      poly_bbox): cells whose interior projection lies in a circle are
+//      computed with the segment predicate, face cells through the same
+//      face chain as the gather kernel; interior cells also emit the
+//      momentum exchange of their cut links. G = 8, 4 or 2 lanes per link
+//      (8 / G cells per wavefront pass) or, for a large box, one lane per
+//      cell; the vertex tables in LDS;
+//   2. the plain pull (gather with no mask) of every other cell, 64 cells
+//      of one row per workgroup.
+// The two roles write disjoint cells and read only the source state. The
+// band's fixed number of per-wave FP64 rows is summed by the next step's
+// band workgroups (k_sum_partials after the last step). The band strides
+// over the box with a fixed number of waves, so the FP64 force order does
+// not depend on scheduling. Arithmetic follows the reference's operation
+// order with contraction off.
 #pragma clang fp contract(off)
 
 #include <math.h>
@@ -252,7 +256,8 @@ constexpr int kPolyLdsFloats = 8192;  // 32 KiB
 // link by link.
 template <typename T, int DT>
 __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs& p, const float* verts,
-                                                double* partial, double* acc, int64_t n, int bw, int bx0, int by0)
+                                                double* partial, double* acc, int64_t n, int bw, int bx0, int by0,
+                                                int wg, int nwg)
 {
     using S = Storage<DT>;
     using ST = typename S::type;
@@ -261,7 +266,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
     const int lane = threadIdx.x;
     const int nx = a.nx, ny = a.ny;
     const T grid_length = T(1) / T(nx);
-    for (int64_t base = (int64_t)blockIdx.x * 64; base < n; base += (int64_t)gridDim.x * 64) {
+    for (int64_t base = (int64_t)wg * 64; base < n; base += (int64_t)nwg * 64) {
         const int64_t t = base + lane;
         int x = 0, y = 0, px = 0, py = 0;
         uint64_t near = 0;
@@ -355,7 +360,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
 
 template <typename T, int DT, bool LDSV>
 __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev,
-                                          double* prev_out)
+                                          double* prev_out, int wg, int nwg)
 {
     extern __shared__ float lds_verts[];
     __shared__ int bbox[5];
@@ -385,9 +390,9 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
     const int64_t n = bw > 0 && bh > 0 ? (int64_t)bw * bh : 0;
     const T grid_length = T(1) / T(nx);
     if (G == 1)  // a large band: one lane per cell
-        poly_band_cells<T, DT>(a, p, verts, partial, acc, n, bw, bx0, by0);
+        poly_band_cells<T, DT>(a, p, verts, partial, acc, n, bw, bx0, by0, wg, nwg);
     else
-    for (int64_t base = (int64_t)blockIdx.x * cells; base < n; base += (int64_t)gridDim.x * cells) {
+    for (int64_t base = (int64_t)wg * cells; base < n; base += (int64_t)nwg * cells) {
         const int64_t t = base + slot;
         int sid = -1;
         T F[3] = {T(0), T(0), T(0)};
@@ -479,7 +484,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
     }
     // the previous step's rows: one wave per output, the order of k_sum_partials
     if (prev)
-        for (int k = blockIdx.x; k < width; k += gridDim.x) {
+        for (int k = wg; k < width; k += nwg) {
             double v = 0.0;
             for (int r = lane; r < kPolyWaves; r += 64) v += prev[(int64_t)r * width + k];
             v = wave_sum(v);
@@ -487,14 +492,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
         }
     if (!partial) return;
     __syncthreads();
-    for (int k = lane; k < width; k += 64) partial[(int64_t)blockIdx.x * width + k] = acc[k];
-}
-
-template <typename T, int DT, bool LDSV>
-__global__ void __launch_bounds__(64) k_poly_band(EnvArgs a, PolyArgs p, double* partial, const double* prev,
-                                                   double* prev_out)
-{
-    poly_band<T, DT, LDSV>(a, p, partial, prev, prev_out);
+    for (int k = lane; k < width; k += 64) partial[(int64_t)wg * width + k] = acc[k];
 }
 
 // Per-world band arguments of a chunk of worlds (with WorldArgs: 3.3 KB of
@@ -506,16 +504,6 @@ struct PolyWorlds {
     double* prev_out[kWorldChunk];     // where their sums go
 };
 static_assert(sizeof(WorldArgs) + sizeof(PolyWorlds) <= 4000, "chunk arguments exceed the kernel-argument block");
-
-// The band of world blockIdx.y: blockIdx.x and gridDim.x play the roles they
-// have in k_poly_band, so every world's rows and sums are those of a
-// launch of its own.
-template <typename T, int DT, bool LDSV>
-__global__ void __launch_bounds__(64) k_poly_band_worlds(WorldArgs w, PolyWorlds p)
-{
-    const int k = blockIdx.y;
-    poly_band<T, DT, LDSV>(w.a[k], p.p[k], p.rows[k], p.prev[k], p.prev_out[k]);
-}
 
 // The closing sums of a chunk (k_sum_partials of each world's last rows):
 // output blockIdx.x of world blockIdx.y.
@@ -530,9 +518,9 @@ __global__ void __launch_bounds__(64) k_sum_rows_worlds(WorldArgs w, PolyWorlds 
     if (threadIdx.x == 0) p.prev_out[j][k] = v;
 }
 
-// The gather step of the cells the band leaves alone: a cell whose interior
-// projection lies in a bounding circle is written by the band launch that
-// follows (the band's box holds every such cell), so it is skipped here.
+// The plain pull of the cells the band leaves alone: a cell whose interior
+// projection lies in a bounding circle is the band's (the band's box holds
+// every such cell), so it is skipped here.
 template <typename T, int DT>
 __device__ __forceinline__ void poly_gather_cell(const EnvArgs& a, const PolyArgs& p, int bx, int by)
 {
@@ -543,79 +531,70 @@ __device__ __forceinline__ void poly_gather_cell(const EnvArgs& a, const PolyArg
     gather_d2q9_at<T, DT>(a, x, y);
 }
 
-template <typename T, int DT>
-__global__ void __launch_bounds__(256) k_poly_gather(EnvArgs a, PolyArgs p)
+// One step in one launch: the band and the plain pull write disjoint cells
+// and both only read the source state, so workgroups 0 .. kPolyWaves - 1
+// run the band (workgroup b in the role of band wave b of kPolyWaves) and
+// the rest pull 64 cells of one row each (gw workgroups per row).
+template <typename T, int DT, bool LDSV>
+__device__ __forceinline__ void poly_step(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev,
+                                          double* prev_out, int b, int gw)
 {
-    poly_gather_cell<T, DT>(a, p, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y);
+    if (b < kPolyWaves) {
+        poly_band<T, DT, LDSV>(a, p, partial, prev, prev_out, b, kPolyWaves);
+        return;
+    }
+    const int g = b - kPolyWaves;
+    poly_gather_cell<T, DT>(a, p, (g % gw) * 64 + (int)threadIdx.x, g / gw);
 }
 
-// World blockIdx.z of the chunk; the grid covers its largest region.
-template <typename T, int DT>
-__global__ void __launch_bounds__(256) k_poly_gather_worlds(WorldArgs w, PolyWorlds p)
+template <typename T, int DT, bool LDSV>
+__global__ void __launch_bounds__(64) k_poly_step(EnvArgs a, PolyArgs p, double* partial, const double* prev,
+                                                   double* prev_out, int gw)
 {
-    const int k = blockIdx.z;
-    poly_gather_cell<T, DT>(w.a[k], p.p[k], blockIdx.x * blockDim.x + threadIdx.x,
-                            blockIdx.y * blockDim.y + threadIdx.y);
+    poly_step<T, DT, LDSV>(a, p, partial, prev, prev_out, blockIdx.x, gw);
 }
 
-hipError_t launch_poly_gather(int dtype, const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
+// World blockIdx.y of the chunk; the grid covers its largest region.
+template <typename T, int DT, bool LDSV>
+__global__ void __launch_bounds__(64) k_poly_step_worlds(WorldArgs w, PolyWorlds p, int gw)
 {
-    int wmax = 0, hmax = 0;
+    const int k = blockIdx.y;
+    poly_step<T, DT, LDSV>(w.a[k], p.p[k], p.rows[k], p.prev[k], p.prev_out[k], blockIdx.x, gw);
+}
+
+template <typename T, int DT>
+hipError_t launch_step_worlds(const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
+{
+    int nv = 0, wmax = 0, hmax = 0;
     for (int k = 0; k < n; ++k) {
+        nv = max(nv, w.a[k].nsolid * p.p[k].maxv * 2);
         const int* r = w.a[k].region;
         wmax = max(wmax, r[1] - r[0]);
         hmax = max(hmax, r[3] - r[2]);
     }
-    if (wmax <= 0 || hmax <= 0) return hipSuccess;
-    const dim3 block(64, 4, 1), grid((wmax + 63) / 64, (hmax + 3) / 4, n);
-    switch (dtype) {
-    case HL_F32: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_F32>), grid, block, 0, s, w, p); break;
-    case HL_F64: hipLaunchKernelGGL((k_poly_gather_worlds<double, HL_F64>), grid, block, 0, s, w, p); break;
-    case HL_F16: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_F16>), grid, block, 0, s, w, p); break;
-    case HL_BF16: hipLaunchKernelGGL((k_poly_gather_worlds<float, HL_BF16>), grid, block, 0, s, w, p); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_poly_gather(int dtype, const EnvArgs& a, const PolyArgs& p, hipStream_t s)
-{
-    const int w = a.region[1] - a.region[0], h = a.region[3] - a.region[2];
-    if (w <= 0 || h <= 0) return hipSuccess;
-    const dim3 block(64, 4, 1), grid((w + 63) / 64, (h + 3) / 4, 1);
-    switch (dtype) {
-    case HL_F32: hipLaunchKernelGGL((k_poly_gather<float, HL_F32>), grid, block, 0, s, a, p); break;
-    case HL_F64: hipLaunchKernelGGL((k_poly_gather<double, HL_F64>), grid, block, 0, s, a, p); break;
-    case HL_F16: hipLaunchKernelGGL((k_poly_gather<float, HL_F16>), grid, block, 0, s, a, p); break;
-    case HL_BF16: hipLaunchKernelGGL((k_poly_gather<float, HL_BF16>), grid, block, 0, s, a, p); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-template <typename T, int DT>
-hipError_t launch_band_worlds(const WorldArgs& w, const PolyWorlds& p, int n, hipStream_t s)
-{
-    int nv = 0;
-    for (int k = 0; k < n; ++k) nv = max(nv, w.a[k].nsolid * p.p[k].maxv * 2);
-    const dim3 grid(kPolyWaves, n);
+    const int gw = (max(wmax, 0) + 63) / 64;
+    const dim3 grid(kPolyWaves + gw * max(hmax, 0), n);
     if (nv <= kPolyLdsFloats)
-        hipLaunchKernelGGL((k_poly_band_worlds<T, DT, true>), grid, 64, (size_t)nv * sizeof(float), s, w, p);
+        hipLaunchKernelGGL((k_poly_step_worlds<T, DT, true>), grid, 64, (size_t)nv * sizeof(float), s, w, p,
+                           max(gw, 1));
     else
-        hipLaunchKernelGGL((k_poly_band_worlds<T, DT, false>), grid, 64, 0, s, w, p);
+        hipLaunchKernelGGL((k_poly_step_worlds<T, DT, false>), grid, 64, 0, s, w, p, max(gw, 1));
     return hipGetLastError();
 }
 
 template <typename T, int DT>
-hipError_t launch_band(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev, double* prev_out,
+hipError_t launch_step(const EnvArgs& a, const PolyArgs& p, double* partial, const double* prev, double* prev_out,
                        hipStream_t s)
 {
     const int nv = a.nsolid * p.maxv * 2;
+    const int w = max(a.region[1] - a.region[0], 0), h = max(a.region[3] - a.region[2], 0);
+    const int gw = (w + 63) / 64;
+    const dim3 grid(kPolyWaves + gw * h);
     if (nv <= kPolyLdsFloats)
-        hipLaunchKernelGGL((k_poly_band<T, DT, true>), kPolyWaves, 64, (size_t)nv * sizeof(float), s, a, p, partial,
-                           prev, prev_out);
+        hipLaunchKernelGGL((k_poly_step<T, DT, true>), grid, 64, (size_t)nv * sizeof(float), s, a, p, partial, prev,
+                           prev_out, max(gw, 1));
     else
-        hipLaunchKernelGGL((k_poly_band<T, DT, false>), kPolyWaves, 64, 0, s, a, p, partial, prev, prev_out);
+        hipLaunchKernelGGL((k_poly_step<T, DT, false>), grid, 64, 0, s, a, p, partial, prev, prev_out, max(gw, 1));
     return hipGetLastError();
 }
 
@@ -649,13 +628,11 @@ hipError_t step_polygons(int dtype, const EnvArgs& a0, int nsteps, const float* 
         p.verts_last = k == 0 ? verts_last : verts;
         double* fout = force_history && width > 0 ? force_history + (int64_t)k * width : nullptr;
         double* rows = fout ? partial + (size_t)(k & 1) * kPolyWaves * HL_MAX_SOLIDS * 6 : nullptr;
-        e = launch_poly_gather(dtype, a, p, s);
-        if (e != hipSuccess) break;
         switch (dtype) {
-        case HL_F32: e = launch_band<float, HL_F32>(a, p, rows, prev, prev_out, s); break;
-        case HL_F64: e = launch_band<double, HL_F64>(a, p, rows, prev, prev_out, s); break;
-        case HL_F16: e = launch_band<float, HL_F16>(a, p, rows, prev, prev_out, s); break;
-        case HL_BF16: e = launch_band<float, HL_BF16>(a, p, rows, prev, prev_out, s); break;
+        case HL_F32: e = launch_step<float, HL_F32>(a, p, rows, prev, prev_out, s); break;
+        case HL_F64: e = launch_step<double, HL_F64>(a, p, rows, prev, prev_out, s); break;
+        case HL_F16: e = launch_step<float, HL_F16>(a, p, rows, prev, prev_out, s); break;
+        case HL_BF16: e = launch_step<float, HL_BF16>(a, p, rows, prev, prev_out, s); break;
         default: e = hipErrorInvalidValue;
         }
         prev = rows;
@@ -721,13 +698,11 @@ hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int ns
                 }
                 continue;
             }
-            e = launch_poly_gather(dtype, w, p, n, s);
-            if (e != hipSuccess) break;
             switch (dtype) {
-            case HL_F32: e = launch_band_worlds<float, HL_F32>(w, p, n, s); break;
-            case HL_F64: e = launch_band_worlds<double, HL_F64>(w, p, n, s); break;
-            case HL_F16: e = launch_band_worlds<float, HL_F16>(w, p, n, s); break;
-            case HL_BF16: e = launch_band_worlds<float, HL_BF16>(w, p, n, s); break;
+            case HL_F32: e = launch_step_worlds<float, HL_F32>(w, p, n, s); break;
+            case HL_F64: e = launch_step_worlds<double, HL_F64>(w, p, n, s); break;
+            case HL_F16: e = launch_step_worlds<float, HL_F16>(w, p, n, s); break;
+            case HL_BF16: e = launch_step_worlds<float, HL_BF16>(w, p, n, s); break;
             default: e = hipErrorInvalidValue;
             }
         }
