@@ -13,9 +13,10 @@ Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
   hl_step_polygons_worlds call). LBM_Solver(..., cut_cell=False) (or HomeFlow.cut_cell
   = False) voxelizes them into an integer mask with the reference's even-odd
   inside test and runs the wall rule on mask links (mask(x) != mask(x-c_i)):
-  faster, and several worlds then step in one batched launch sequence, but
-  a different boundary discretization (on the 512x128 cylinder: Strouhal
-  3%, mean drag 16% apart). Capsule bodies (add_capsule, an extension)
+  several FP32 worlds then step in one batched launch sequence (faster
+  when the bounding circles cover the lattice), but a different boundary
+  discretization (on the 512x128 cylinder: Strouhal 3%, mean drag 16%
+  apart). Capsule bodies (add_capsule, an extension)
   always use the mask rule.
 * Forces and torques are reduced in FP64 in a fixed order and are
   deterministic; solid_forcex / solid_forcey / torque hold them as FP32 like
