@@ -364,7 +364,6 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
 {
     extern __shared__ float lds_verts[];
     __shared__ int bbox[5];
-    if (threadIdx.x == 0) poly_bbox(a, p, bbox);  // every workgroup finds the same box
     __shared__ double acc[HL_MAX_SOLIDS * 6];
     __shared__ T pops[8][9];
     using S = Storage<DT>;
@@ -375,10 +374,16 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
     const int nx = a.nx, ny = a.ny;
     const int width = a.nsolid * 6;
     // the vertex tables in LDS when they fit (a compile-time choice, so the
-    // segment loop reads LDS directly rather than through generic pointers)
+    // segment loop reads LDS directly rather than through generic pointers);
+    // the first 64 values are loaded before the box is found, so that the
+    // two sets of loads overlap
     const int nv = a.nsolid * p.maxv * 2;
-    if (LDSV)
-        for (int k = lane; k < nv; k += 64) lds_verts[k] = p.verts[k];
+    const float v0 = LDSV && lane < nv ? p.verts[lane] : 0.0f;
+    if (threadIdx.x == 0) poly_bbox(a, p, bbox);  // every workgroup finds the same box
+    if (LDSV) {
+        if (lane < nv) lds_verts[lane] = v0;
+        for (int k = lane + 64; k < nv; k += 64) lds_verts[k] = p.verts[k];
+    }
     const float* verts = LDSV ? static_cast<const float*>(lds_verts) : p.verts;
     if (partial)
         for (int k = lane; k < width; k += 64) acc[k] = 0.0;
