@@ -373,3 +373,39 @@ def test_step_polygons_worlds_errors():
         step_polygons_worlds([a, b], 1, [t])
     with pytest.raises(ValueError, match="counts int32"):
         step_polygons_worlds([a, b], 1, [t, (t[0], None, t[2].float(), t[3])])
+
+
+def test_step_polygons_worlds_of_different_sizes():
+    """Worlds of different lattice sizes in one chunk (the launch covers
+    the largest, every world bounds its own cells): each equals its own
+    step_polygons call bitwise, state and forces, over two steps."""
+    from openhomelbm_amd.lattice import step_polygons_worlds
+    sizes = [(64, 48), (40, 40), (72, 30)]
+    many, alone, tables, hists, hist_alone = [], [], [], [], []
+    for w, (nx, ny) in enumerate(sizes):
+        for group in (many, alone):
+            lat = Lattice(9, nx, ny, dtype="f32", nsolid=1)
+            lat.nu = 0.015
+            lat.bc_type = [0, 1, 1, 2, 1, 1]
+            lat.bc_value = [[0.04, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+            lat.set_state(random_state(9, (ny, nx), seed=90 + w).astype(np.float32))
+            group.append(lat)
+        c = (nx * 0.4 + 0.3, ny * 0.5 - 0.2)
+        v = _poly(c, 6.3, 6, 0.2 + w)[None]
+        vl = _poly((c[0] - 0.3, c[1] + 0.1), 6.3, 6, 0.15 + w)[None]
+        P = np.zeros((1, 8), np.float32)
+        P[0, 0:2], P[0, 2] = c, 1.0
+        P[0, 4:6], P[0, 6] = (c[0] - 0.3, c[1] + 0.1), 1.0
+        for lat in (many[w], alone[w]):
+            lat.set_poses(P)
+        t = (_dev(v), _dev(vl), _dev(np.array([6], np.int32)), _dev(np.array([8.0 ** 2], np.float32)))
+        tables.append(t)
+        hists.append(torch.zeros((2, 1, 6), dtype=torch.float64, device="cuda"))
+        hist_alone.append(torch.zeros((2, 1, 6), dtype=torch.float64, device="cuda"))
+        alone[w].step_polygons(2, *t, force_history=hist_alone[w])
+    step_polygons_worlds(many, 2, tables, hists)
+    torch.cuda.synchronize()
+    for w in range(len(sizes)):
+        assert np.array_equal(many[w].get_state(np.float32), alone[w].get_state(np.float32)), w
+        assert np.array_equal(hists[w].cpu().numpy(), hist_alone[w].cpu().numpy()), w
+        assert np.abs(hist_alone[w].cpu().numpy()[:, 0, [0, 1, 5]]).min() > 0
