@@ -15,6 +15,7 @@ from __future__ import annotations
 import ctypes as C
 import itertools
 import math
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -550,10 +551,11 @@ def step_polygons_worlds(lattices: Sequence["Lattice"], nsteps: int, tables, for
     envs = [L._step_env() for L in lats]
     hold = lats[0]
     cached = getattr(hold, "_worlds_args", None)
-    # the same objects as last call (the cache holds them, so no new object
-    # can take an old one's identity): the checks and argument arrays stand
+    # the same objects as last call (the cache holds them, or a weak
+    # reference that a new object cannot answer to): the checks and argument
+    # arrays stand
     same = cached is not None and cached[0] == nsteps and len(cached[1]) == n and all(
-        a is b for a, b in zip(cached[1], lats)) and all(a is b for a, b in zip(cached[2], envs)) and all(
+        a() is b for a, b in zip(cached[1], lats)) and all(a is b for a, b in zip(cached[2], envs)) and all(
         x is y for a, b in zip(cached[3], tables) for x, y in zip(a, b)) and all(
         a is b for a, b in zip(cached[4], hists))
     if not same:
@@ -567,7 +569,9 @@ def step_polygons_worlds(lattices: Sequence["Lattice"], nsteps: int, tables, for
                 (P * n)(*[t[1].data_ptr() if t[1] is not None else None for t in tables]),
                 (P * n)(*[t[2].data_ptr() for t in tables]), (C.c_int32 * n)(*[int(t[0].shape[1]) for t in tables]),
                 (P * n)(*[t[3].data_ptr() for t in tables]), (P * n)(*fhs))
-        cached = (nsteps, list(lats), envs, [tuple(t) for t in tables], hists, args)  # tables' tensors held
+        # the worlds by weak reference (the first one holds the cache: no
+        # cycle keeps their buffers alive), tables' tensors held
+        cached = (nsteps, [weakref.ref(L) for L in lats], envs, [tuple(t) for t in tables], hists, args)
         hold._worlds_args = cached
     a = cached[5]
     N.check(lats[0].lib.hl_step_polygons_worlds(a[0], n, cur, int(nsteps), a[1], a[2], a[3], a[4], a[5], a[6],
