@@ -86,3 +86,41 @@ def test_batch_rejects_unsupported_environments():
     assert lib.hl_batch_step(None, None, 1, None, 0, None) == -1
     assert lib.hl_batch_replay(None, None) == -1
     assert "NULL" in lib.hl_last_error().decode()
+
+
+def _d2q9_env(nsolid=1, dtype=0):
+    e = N.HLEnv()
+    e.lattice, e.dtype, e.nx, e.ny, e.nz = 9, dtype, 8, 8, 1
+    e.stride = 64
+    e.moments[0] = 0x1000
+    e.moments[1] = 0x2000
+    e.nu = 0.1
+    e.nsolid = nsolid
+    e.solids = 0x3000 if nsolid else None
+    return e
+
+
+def test_polygon_worlds_arguments_rejected_before_any_gpu_call():
+    """hl_step_polygons_worlds checks every world's descriptor and tables
+    and names the world at fault, before any device work."""
+    lib = N.load()
+    P = C.c_void_p
+
+    def call(envs, n, verts=(0x10, 0x10), counts=(0x20, 0x20), maxv=(4, 4), r2=(0x30, 0x30), cur=0, nsteps=1):
+        arr = (N.HLEnv * len(envs))(*envs) if envs is not None else None
+        return lib.hl_step_polygons_worlds(arr, n, cur, nsteps, (P * 2)(*verts), None, (P * 2)(*counts),
+                                           (C.c_int32 * 2)(*maxv), (P * 2)(*r2), None, None)
+
+    two = [_d2q9_env(), _d2q9_env()]
+    assert call(None, 2) == -1 and "envs is NULL" in lib.hl_last_error().decode()
+    assert call(two, 0) == -1 and "nworld" in lib.hl_last_error().decode()
+    assert call(two, 2, cur=2) == -1 and "cur" in lib.hl_last_error().decode()
+    assert call(two, 2, nsteps=-1) == -1 and "nsteps" in lib.hl_last_error().decode()
+    d3 = _d2q9_env()
+    d3.lattice, d3.nz, d3.stride = 27, 8, 512
+    assert call([_d2q9_env(), d3], 2) == -2 and "world 1" in lib.hl_last_error().decode()
+    assert call([_d2q9_env(), _d2q9_env(dtype=1)], 2) == -1 and "same dtype" in lib.hl_last_error().decode()
+    assert call([_d2q9_env(), _d2q9_env(nsolid=0)], 2) == -1 and "nsolid" in lib.hl_last_error().decode()
+    assert call(two, 2, maxv=(4, 0)) == -1 and "world 1: maxv" in lib.hl_last_error().decode()
+    assert call(two, 2, verts=(0x10, 0)) == -1 and "world 1: verts" in lib.hl_last_error().decode()
+    assert call(two, 2, nsteps=0) == 0  # nothing to do: no device work
