@@ -369,13 +369,12 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
 }
 
 // out[k] = sum over blocks b of partial[b][k]: each lane sums a fixed strided
-// subset in order, then a wavefront butterfly; one wave per output.
+// subset in order (lane_rows_sum), then a wavefront butterfly; one wave per
+// output.
 __global__ void __launch_bounds__(64) k_sum_partials(const double* partial, int nblocks, int width, double* out)
 {
     const int k = blockIdx.x;
-    double v = 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 64) v += partial[(int64_t)b * width + k];
-    v = wave_sum(v);
+    const double v = wave_sum(lane_rows_sum(partial, nblocks, width, k, threadIdx.x));
     if (threadIdx.x == 0) out[k] = v;
 }
 

@@ -332,6 +332,25 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// A lane's share of the sum of column k over nrows rows of `width`: rows
+// lane, lane + 64, ... added in that order, eight loads issued before their
+// adds (the same sum as the plain strided loop, without a load latency
+// per row).
+__device__ __forceinline__ double lane_rows_sum(const double* rows, int nrows, int width, int k, int lane)
+{
+    double v = 0.0;
+    int r = lane;
+    for (; r + 64 * 7 < nrows; r += 64 * 8) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) t[u] = rows[(int64_t)(r + 64 * u) * width + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += t[u];
+    }
+    for (; r < nrows; r += 64) v += rows[(int64_t)r * width + k];
+    return v;
+}
+
 __device__ __forceinline__ uint64_t wave_or(uint64_t v)
 {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);

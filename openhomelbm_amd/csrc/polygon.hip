@@ -38,6 +38,22 @@ namespace {
 
 constexpr int kPolyWaves = 1024;  // band grid: one row of force partials per wave
 
+// A lane's share of the sum of column k over the kPolyWaves rows: rows
+// lane, lane + 64, ... added in that order (the order of k_sum_partials),
+// every load issued before the first add.
+__device__ __forceinline__ double rows_lane_sum(const double* rows, int width, int k, int lane)
+{
+    static_assert(kPolyWaves % 64 == 0, "whole rows per lane");
+    constexpr int kPer = kPolyWaves / 64;
+    double t[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) t[j] = rows[(int64_t)(lane + 64 * j) * width + k];
+    double v = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) v += t[j];
+    return v;
+}
+
 struct PolyArgs {
     const float* verts;       // [nsolid][maxv][2], this step
     const float* verts_last;  // same layout, previous step
@@ -490,9 +506,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
     // the previous step's rows: one wave per output, the order of k_sum_partials
     if (prev)
         for (int k = wg; k < width; k += nwg) {
-            double v = 0.0;
-            for (int r = lane; r < kPolyWaves; r += 64) v += prev[(int64_t)r * width + k];
-            v = wave_sum(v);
+            const double v = wave_sum(rows_lane_sum(prev, width, k, lane));
             if (lane == 0) prev_out[k] = v;
         }
     if (!partial) return;
@@ -517,9 +531,7 @@ __global__ void __launch_bounds__(64) k_sum_rows_worlds(WorldArgs w, PolyWorlds 
     const int j = blockIdx.y, k = blockIdx.x, width = w.a[j].nsolid * 6;
     const double* rows = p.prev[j];
     if (!rows || k >= width) return;
-    double v = 0.0;
-    for (int r = threadIdx.x; r < kPolyWaves; r += 64) v += rows[(int64_t)r * width + k];
-    v = wave_sum(v);
+    const double v = wave_sum(rows_lane_sum(rows, width, k, threadIdx.x));
     if (threadIdx.x == 0) p.prev_out[j][k] = v;
 }
 

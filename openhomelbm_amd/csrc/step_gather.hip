@@ -108,9 +108,7 @@ __global__ void __launch_bounds__(256) k_gather_d2q9_fx(EnvArgs a, FixArgs fx)
     }
     if (fx.prev) {
         for (int k = gw; k < fx.width; k += nwaves) {
-            double v = 0.0;
-            for (int r = lane; r < fx.prev_rows; r += 64) v += fx.prev[(int64_t)r * fx.width + k];
-            v = wave_sum(v);
+            const double v = wave_sum(lane_rows_sum(fx.prev, fx.prev_rows, fx.width, k, lane));
             if (lane == 0) fx.prev_out[k] = v;
         }
     }

@@ -386,9 +386,7 @@ __device__ void sum_previous(const FixArgs& fx, int blk)
     constexpr int W = NT / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int k = blk * W + wave; k < fx.width; k += (int)gridDim.x * W) {
-        double v = 0.0;
-        for (int r = lane; r < fx.prev_rows; r += 64) v += fx.prev[(int64_t)r * fx.width + k];
-        v = wave_sum(v);
+        const double v = wave_sum(lane_rows_sum(fx.prev, fx.prev_rows, fx.width, k, lane));
         if (lane == 0) fx.prev_out[k] = v;
     }
 }
