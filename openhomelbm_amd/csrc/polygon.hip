@@ -521,6 +521,11 @@ struct PolyWorlds {
     double* rows[kWorldChunk];         // this step's force rows, or null
     const double* prev[kWorldChunk];   // the previous step's rows, or null
     double* prev_out[kWorldChunk];     // where their sums go
+    // the previous chunk's rows of the call's last step and where their sums
+    // go (null: none), summed by this launch's band workgroups of world 0
+    const double* xrows[kWorldChunk];
+    double* xout[kWorldChunk];
+    int32_t xwidth[kWorldChunk];
 };
 static_assert(sizeof(WorldArgs) + sizeof(PolyWorlds) <= 4000, "chunk arguments exceed the kernel-argument block");
 
@@ -577,6 +582,20 @@ __global__ void __launch_bounds__(64) k_poly_step_worlds(WorldArgs w, PolyWorlds
 {
     const int k = blockIdx.y;
     poly_step<T, DT, LDSV>(w.a[k], p.p[k], p.rows[k], p.prev[k], p.prev_out[k], blockIdx.x, gw);
+    if (k != 0 || blockIdx.x >= kPolyWaves) return;
+    // output o of the previous chunk's columns (worlds in order) by band
+    // workgroup o mod kPolyWaves, in the order of k_sum_partials
+    int base = 0;
+    for (int j = 0; j < kWorldChunk; ++j) {
+        const double* rows = p.xrows[j];
+        const int wd = rows ? p.xwidth[j] : 0;
+        for (int o = (int)blockIdx.x; o < base + wd; o += kPolyWaves) {
+            if (o < base) continue;
+            const double v = wave_sum(rows_lane_sum(rows, wd, o - base, threadIdx.x));
+            if (threadIdx.x == 0) p.xout[j][o - base] = v;
+        }
+        base += wd;
+    }
 }
 
 template <typename T, int DT>
@@ -679,11 +698,18 @@ hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int ns
         if (force_history && force_history[wi]) off += (size_t)2 * kPolyWaves * a0[wi].nsolid * 6;
     }
     hipError_t e = hipSuccess;
+    PolyWorlds before{};  // the previous chunk's arguments in this step
     for (int k = 0; k <= nsteps && e == hipSuccess; ++k) {
         for (int c0 = 0; c0 < nworld && e == hipSuccess; c0 += kWorldChunk) {
             const int n = nworld - c0 < kWorldChunk ? nworld - c0 : kWorldChunk;
             WorldArgs w{};
             PolyWorlds p{};
+            if (k == nsteps - 1 && c0 > 0)  // the call's last step: sum the previous chunk's rows here
+                for (int j = 0; j < kWorldChunk; ++j) {
+                    p.xrows[j] = before.rows[j];
+                    p.xout[j] = before.xout[j];
+                    p.xwidth[j] = before.xwidth[j];
+                }
             int maxwidth = 0;
             for (int j = 0; j < n; ++j) {
                 const int wi = c0 + j;
@@ -708,8 +734,16 @@ hipError_t step_polygons_worlds(int dtype, const EnvArgs* a0, int nworld, int ns
                 p.prev[j] = fh && k > 0 ? wr + (size_t)((k - 1) & 1) * bank : nullptr;
                 p.prev_out[j] = fh && k > 0 ? fh + (int64_t)(k - 1) * width : nullptr;
             }
-            if (k == nsteps) {  // the last step's rows: summed by a launch of their own
-                if (maxwidth > 0) {
+            // for the next chunk: where this chunk's rows of this step go
+            before = p;
+            for (int j = 0; j < kWorldChunk; ++j) {
+                const int wi = c0 + j;
+                const int width = j < n ? w.a[j].nsolid * 6 : 0;
+                before.xout[j] = p.rows[j] ? force_history[wi] + (int64_t)k * width : nullptr;
+                before.xwidth[j] = width;
+            }
+            if (k == nsteps) {  // the last chunk's last rows: summed by a launch of their own
+                if (maxwidth > 0 && c0 + kWorldChunk >= nworld) {
                     hipLaunchKernelGGL(k_sum_rows_worlds, dim3(maxwidth, n), 64, 0, s, w, p);
                     e = hipGetLastError();
                 }
