@@ -13,8 +13,13 @@
  * reference's LBM_Solver / LBM_Solver3D / HomeFlow / HomeFlow3D API.
  *
  * Moment storage (owned by the caller, e.g. PyTorch tensors): two ping-pong
- * buffers, structure-of-arrays, cell (x, y, z) at x + nx*(y + ny*z), field f
- * at offset f*stride. Field order:
+ * buffers; field f of cell (x, y, z) at f*stride + x + rpitch*(y + ny*z).
+ * Two layouts:
+ *   structure of arrays (rpitch = nx, or 0): stride >= nx*ny*nz;
+ *   row-interleaved (D3Q27 only; rpitch = 10*nx, stride = nx): the ten fields
+ *     of one x-row are consecutive, so a row of cells with all its fields is
+ *     one contiguous span (fewer concurrent HBM streams per step).
+ * Field order:
  *   D2Q9  : rho, ux, uy, Sxx, Syy, Sxy
  *   D3Q27 : rho, ux, uy, uz, Sxx, Syy, Szz, Sxy, Sxz, Syz
  * 16-bit storage holds (rho - 1)*scale[0] and value*scale[f] for f > 0.
@@ -54,11 +59,12 @@ typedef struct hl_env {
     int32_t dtype;        /* HL_F32, HL_F64, HL_F16, HL_BF16 */
     int32_t nx, ny, nz;   /* nz = 1 for D2Q9; every extent >= 3 */
     int32_t nsolid;       /* solids in the pose table (<= HL_MAX_SOLIDS) */
-    int64_t stride;       /* elements between moment fields (>= nx*ny*nz) */
+    int64_t stride;       /* elements between moment fields (see the layouts above) */
     void* moments[2];     /* ping-pong moment buffers */
     const uint8_t* mask;  /* nullable; 0 = fluid, k > 0 = solid k-1 */
     int32_t mask_box[6];  /* x0, x1, y0, y1, z0, z1 (half-open): mask is 0 outside */
-    const float* force;   /* nullable per-cell body force, [dim][stride] FP32 */
+    const float* force;   /* nullable per-cell body force, FP32, component d of cell
+                             x + nx*(y + ny*z) at d*force_stride + cell */
     const float* solids;  /* device pose table: 8 floats/solid (2D), 24 (3D) */
     float nu;             /* kinematic viscosity (lattice units) */
     int32_t bc_type[6];   /* 0 velocity, 1 zero gradient, 2 free slip */
@@ -67,6 +73,9 @@ typedef struct hl_env {
     uint64_t mask_epoch;  /* 0, or a caller-kept version of mask + mask_box: hl_step calls on one
                              stream with the same nonzero epoch, mask and box reuse the near-body
                              list the first of them built (a new epoch whenever the mask changes) */
+    int64_t rpitch;       /* elements between consecutive x-rows of a field: 0 or nx (structure of
+                             arrays) or 10*nx with stride = nx (row-interleaved, D3Q27) */
+    int64_t force_stride; /* elements between force components (>= nx*ny*nz); 0 = stride */
 } hl_env;
 
 /* Library version (major*10000 + minor*100 + patch). */

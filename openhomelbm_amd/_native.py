@@ -47,6 +47,8 @@ class HLEnv(C.Structure):
         ("bc_value", (C.c_float * 3) * 6),
         ("scale", C.c_float * 10),
         ("mask_epoch", C.c_uint64),
+        ("rpitch", C.c_int64),
+        ("force_stride", C.c_int64),
     ]
 
 

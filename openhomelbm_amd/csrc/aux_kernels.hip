@@ -32,9 +32,10 @@ __global__ void k_fill(void* buf, int64_t stride, int64_t n, int nf, EnvArgs a, 
     ST* p = static_cast<ST*>(buf);
     const double m[10] = {m0, m1, m2, m3, m4, m5, m6, m7, m8, m9};
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t so = cell_of(a, c);
         for (int f = 0; f < nf; ++f) {
-            if (DT == HL_F64) Storage<DT>::template store<double>(p, f * stride + c, m[f], f, a.scale[f]);
-            else Storage<DT>::template store<float>(p, f * stride + c, (float)m[f], f, a.scale[f]);
+            if (DT == HL_F64) Storage<DT>::template store<double>(p, f * stride + so, m[f], f, a.scale[f]);
+            else Storage<DT>::template store<float>(p, f * stride + so, (float)m[f], f, a.scale[f]);
         }
     }
 }
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
         }
         if (act) {
             T mc[10];
-            load_cell<T, DT>(a, c, Q == 9 ? 6 : 10, mc);
+            load_cell<T, DT>(a, cell_at(a, px, py, pz), Q == 9 ? 6 : 10, mc);
             const T pop = Q == 9 ? d2q9_pull<T, DT>(a, px, py, i, mc, true, mval_c)
                                  : d3q27_pull<T, DT>(a, px, py, pz, i, mc, true, mval_c);
             pops[h][i] = pop;
@@ -332,17 +333,17 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
         if (store && cell && i == 0) {
             using ST = typename Storage<DT>::type;
             ST* dst = static_cast<ST*>(a.dst);
-            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+            const int64_t d = cell_at(a, x, y, z);
             if (Q == 9) {
                 T Fx = 0, Fy = 0, m[6];
-                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); }
                 d2q9_collide<T>(a, pops[h], Fx, Fy, m);
                 if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
 #pragma unroll
                 for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
             } else {
                 T Fx = 0, Fy = 0, Fz = 0, m[10];
-                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); Fz = T(a.force[2 * a.stride + c]); }
+                if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); Fz = T(a.force[2 * a.fs + c]); }
                 d3q27_collide_sums<T>(a, sums[h], Fx, Fy, Fz, m);
                 if (px != x || py != y || pz != z) bc_chain<T, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
 #pragma unroll
@@ -533,14 +534,14 @@ hipError_t forces(int lattice, int dtype, const EnvArgs& a, double* out, void* w
 size_t mass_workspace_bytes(const EnvArgs& a, int lattice) { return (size_t)kMaxReduceBlocks * sizeof(double); }
 
 template <int DT>
-__global__ void __launch_bounds__(kReduceBlock) k_mass(const void* srcv, float inv_scale0, int64_t n, double* partial)
+__global__ void __launch_bounds__(kReduceBlock) k_mass(EnvArgs a, int64_t n, double* partial)
 {
     __shared__ double wsum[kWaves];
     using ST = typename Storage<DT>::type;
-    const ST* src = static_cast<const ST*>(srcv);
+    const ST* src = static_cast<const ST*>(a.src);
     double v = 0.0;
     for (int64_t c = blockIdx.x * (int64_t)kReduceBlock + threadIdx.x; c < n; c += (int64_t)gridDim.x * kReduceBlock)
-        v += Storage<DT>::template load<double>(src, c, 0, inv_scale0);
+        v += Storage<DT>::template load<double>(src, cell_of(a, c), 0, a.inv_scale[0]);
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -557,10 +558,10 @@ hipError_t mass(int lattice, int dtype, const EnvArgs& a, double* out, void* ws,
     const int nb = reduce_blocks(n);
     double* part = static_cast<double*>(ws);
     switch (dtype) {
-    case HL_F32: hipLaunchKernelGGL(k_mass<HL_F32>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
-    case HL_F64: hipLaunchKernelGGL(k_mass<HL_F64>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
-    case HL_F16: hipLaunchKernelGGL(k_mass<HL_F16>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
-    case HL_BF16: hipLaunchKernelGGL(k_mass<HL_BF16>, nb, kReduceBlock, 0, s, a.src, a.inv_scale[0], n, part); break;
+    case HL_F32: hipLaunchKernelGGL(k_mass<HL_F32>, nb, kReduceBlock, 0, s, a, n, part); break;
+    case HL_F64: hipLaunchKernelGGL(k_mass<HL_F64>, nb, kReduceBlock, 0, s, a, n, part); break;
+    case HL_F16: hipLaunchKernelGGL(k_mass<HL_F16>, nb, kReduceBlock, 0, s, a, n, part); break;
+    case HL_BF16: hipLaunchKernelGGL(k_mass<HL_BF16>, nb, kReduceBlock, 0, s, a, n, part); break;
     default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();

@@ -47,7 +47,21 @@ int32_t validate(const hl_env* e, bool need_moments = true)
     if (e->lattice == HL_D3Q27 && e->nz < 3) return fail(HL_E_INVALID, "D3Q27 needs nz >= 3, got %d", e->nz);
     const int64_t n = (int64_t)e->nx * e->ny * e->nz;
     if (n > ((int64_t)1 << 40)) return fail(HL_E_INVALID, "grid too large");
-    if (e->stride < n) return fail(HL_E_INVALID, "stride %lld < cell count %lld", (long long)e->stride, (long long)n);
+    const int64_t nf = nfields(e->lattice);
+    if (e->rpitch == 0 || e->rpitch == e->nx) {
+        if (e->stride < n) return fail(HL_E_INVALID, "stride %lld < cell count %lld", (long long)e->stride, (long long)n);
+    } else if (e->rpitch == nf * e->nx) {
+        if (e->lattice != HL_D3Q27) return fail(HL_E_UNSUPPORTED, "the row-interleaved layout is for D3Q27 lattices");
+        if (e->stride != e->nx)
+            return fail(HL_E_INVALID, "row-interleaved layout needs stride == nx, got %lld", (long long)e->stride);
+    } else {
+        return fail(HL_E_INVALID, "rpitch %lld is neither nx (%d) nor %lld * nx", (long long)e->rpitch, e->nx,
+                    (long long)nf);
+    }
+    if (e->force) {
+        const int64_t fs = e->force_stride ? e->force_stride : e->stride;
+        if (fs < n) return fail(HL_E_INVALID, "force stride %lld < cell count %lld", (long long)fs, (long long)n);
+    }
     if (need_moments && (!e->moments[0] || !e->moments[1])) return fail(HL_E_INVALID, "moment buffers are NULL");
     if (e->nsolid < 0 || e->nsolid > HL_MAX_SOLIDS) return fail(HL_E_INVALID, "nsolid %d outside [0, %d]", e->nsolid, HL_MAX_SOLIDS);
     if (e->nsolid > 0 && !e->solids) return fail(HL_E_INVALID, "nsolid > 0 but the pose table is NULL");
@@ -74,6 +88,8 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
     std::memset(&a, 0, sizeof(a));
     a.nx = e->nx; a.ny = e->ny; a.nz = e->nz; a.nsolid = e->nsolid;
     a.stride = e->stride;
+    a.rp = e->rpitch ? e->rpitch : e->nx;
+    a.fs = e->force_stride ? e->force_stride : e->stride;
     a.src = e->moments[cur & 1];
     a.dst = e->moments[(cur & 1) ^ 1];
     a.mask = e->mask;

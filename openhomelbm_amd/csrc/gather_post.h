@@ -87,12 +87,13 @@ __device__ __forceinline__ void d2q9_collide(const EnvArgs& a, const T* pop, T F
     out[5] = pixy * invRho;
 }
 
+// The nf moments of the cell at field offset so (cell_at) of the source state.
 template <typename T, int DT>
-__device__ __forceinline__ void load_cell(const EnvArgs& a, int64_t c, int nf, T* mc)
+__device__ __forceinline__ void load_cell(const EnvArgs& a, int64_t so, int nf, T* mc)
 {
     using S = Storage<DT>;
     const typename S::type* src = static_cast<const typename S::type*>(a.src);
-    for (int f = 0; f < nf; ++f) mc[f] = S::template load<T>(src, f * a.stride + c, f, a.inv_scale[f]);
+    for (int f = 0; f < nf; ++f) mc[f] = S::template load<T>(src, f * a.stride + so, f, a.inv_scale[f]);
 }
 
 template <typename T, int DT>
@@ -108,7 +109,7 @@ __device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[
 #pragma unroll
     for (int i = 0; i < 9; ++i) pop[i] = d2q9_pull<T, DT>(a, x, y, i, mc, masked, mval_c);
     T Fx = 0, Fy = 0;
-    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); }
     d2q9_collide<T>(a, pop, Fx, Fy, out);
 }
 
@@ -124,11 +125,12 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
     const int64_t st = a.stride;
     const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]), wi = d3w<T>(i);
     const int64_t s = (int64_t)(x - D3CX[i]) + (int64_t)a.nx * ((int64_t)(y - D3CY[i]) + (int64_t)a.ny * (z - D3CZ[i]));
+    const int64_t so = cell_at(a, x - D3CX[i], y - D3CY[i], z - D3CZ[i]);
     // the source moments are loaded in the same round trip as the mask test
     // (the source cell is inside the lattice either way)
     T m[10];
 #pragma unroll
-    for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+    for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + so, f, a.inv_scale[f]);
     int cut = 0, sid = -1;
     if (masked) {
         const int mval_s = a.mask[s];
@@ -265,15 +267,14 @@ __device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z
 {
     const int64_t c = (int64_t)x + (int64_t)a.nx * ((int64_t)y + (int64_t)a.ny * z);
     T mc[10];
-#pragma unroll
-    for (int f = 0; f < 10; ++f) mc[f] = Storage<DT>::template load<T>(static_cast<const typename Storage<DT>::type*>(a.src), f * a.stride + c, f, a.inv_scale[f]);
+    load_cell<T, DT>(a, cell_at(a, x, y, z), 10, mc);
     const bool masked = a.mask != nullptr && in_box_halo(a, x, y, z);
     const int mval_c = masked ? a.mask[c] : 0;
     T pop[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) pop[i] = d3q27_pull<T, DT>(a, x, y, z, i, mc, masked, mval_c);
     T Fx = 0, Fy = 0, Fz = 0;
-    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); Fz = T(a.force[2 * a.stride + c]); }
+    if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); Fz = T(a.force[2 * a.fs + c]); }
     d3q27_collide<T>(a, pop, Fx, Fy, Fz, out);
 }
 

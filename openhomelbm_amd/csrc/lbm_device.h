@@ -377,7 +377,9 @@ __device__ __forceinline__ __attribute__((address_space(1))) T* as_global(T* p)
 // kernel argument and a graph node parameter).
 struct EnvArgs {
     int32_t nx, ny, nz, nsolid;
-    int64_t stride;
+    int64_t stride;  // elements between moment fields of one cell
+    int64_t rp;      // elements between consecutive x-rows of a field (nx, or 10 nx row-interleaved)
+    int64_t fs;      // elements between body-force components (linear cell index)
     const void* src;
     void* dst;
     const uint8_t* mask;
@@ -400,6 +402,19 @@ struct EnvArgs {
     const uint64_t* cut_cols;
     int32_t cut_colwords;
 };
+
+// Offset of cell (x, y, z) within a moment field: x + rp (y + ny z). The
+// mask and the body force use the linear index x + nx (y + ny z) instead.
+__device__ __forceinline__ int64_t cell_at(const EnvArgs& a, int x, int y, int z)
+{
+    return (int64_t)x + a.rp * ((int64_t)y + (int64_t)a.ny * z);
+}
+
+// cell_at of the cell with linear index c.
+__device__ __forceinline__ int64_t cell_of(const EnvArgs& a, int64_t c)
+{
+    return a.rp == a.nx ? c : c % a.nx + a.rp * (c / a.nx);
+}
 
 __device__ __forceinline__ bool fixup_owns(const EnvArgs& a, int x, int y, int z)
 {

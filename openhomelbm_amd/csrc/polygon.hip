@@ -364,7 +364,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
             if (q0 < T(0)) q0 = T(0);
             pp[0] = q0;
             T Fx = 0, Fy = 0, out[6];
-            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); }
             d2q9_collide<T>(a, pp, Fx, Fy, out);
             if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
             const int64_t d = (int64_t)x + (int64_t)nx * y;
@@ -477,7 +477,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
             T pp[9], Fx = 0, Fy = 0, out[6];
 #pragma unroll
             for (int k = 0; k < 9; ++k) pp[k] = pops[slot][k];
-            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.stride + c]); }
+            if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); }
             d2q9_collide<T>(a, pp, Fx, Fy, out);
             if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
             const int64_t d = (int64_t)x + (int64_t)nx * y;

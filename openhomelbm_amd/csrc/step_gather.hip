@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(256) k_gather_d3q27(EnvArgs a)
     if (px != x || py != y || pz != z) bc_chain<T, 3>(m, x, y, z, a.nx, a.ny, a.nz, a.bc_type, a.bc_value);
     using ST = typename Storage<DT>::type;
     ST* dst = static_cast<ST*>(a.dst);
-    const int64_t c = (int64_t)x + (int64_t)a.nx * ((int64_t)y + (int64_t)a.ny * z);
+    const int64_t c = cell_at(a, x, y, z);
 #pragma unroll
     for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
 }

@@ -265,6 +265,12 @@ template <> struct BufIO<HL_BF16> {
     }
 };
 
+// Elements one state spans: the last field's offset plus one field's extent.
+__host__ __device__ __forceinline__ int64_t state_elems(const EnvArgs& a)
+{
+    return 9 * a.stride + a.nx + a.rp * ((int64_t)a.ny * a.nz - 1);
+}
+
 // Bits [o, o + 64) of a column's cut bitstring (nw words; bits outside it are 0).
 __device__ __forceinline__ uint64_t column_window(const uint64_t* col, int nw, int o)
 {
@@ -324,7 +330,7 @@ __device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int
         }
         if (cell && i < 27) {
             float mc[10];
-            load_cell<float, DT>(a, c, 10, mc);
+            load_cell<float, DT>(a, cell_at(a, px, py, pz), 10, mc);
             pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c);
             // force of this lane's link (interior cells only: a face cell repeats its projection)
             if (width && i > 0 && px == x && py == y && pz == z) {
@@ -343,12 +349,12 @@ __device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int
         wave_sync();
         if (cell && i == 0) {
             float Fx = 0, Fy = 0, Fz = 0, m[10];
-            if (a.force) { Fx = a.force[c]; Fy = a.force[a.stride + c]; Fz = a.force[2 * a.stride + c]; }
+            if (a.force) { Fx = a.force[c]; Fy = a.force[a.fs + c]; Fz = a.force[2 * a.fs + c]; }
             d3q27_collide_sums<float>(a, sums[wave][h], Fx, Fy, Fz, m);
             if (px != x || py != y || pz != z) bc_chain<float, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
             using ST = typename Storage<DT>::type;
             ST* dst = static_cast<ST*>(a.dst);
-            const int64_t d = (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+            const int64_t d = cell_at(a, x, y, z);
 #pragma unroll
             for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + d, m[f], f, a.scale[f]);
         }
@@ -415,10 +421,12 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     const int ze = min(zb + zc, nz - 1);  // destination planes [zb, ze), at most 64
 
     // byte offsets: per field and per plane (scalar), per column (lane)
-    const uint32_t fbytes = (uint32_t)a.stride * E, pbytes = (uint32_t)(nx * ny) * E;
-    const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(a.src, 10 * fbytes);
-    const __amdgpu_buffer_rsrc_t rdst = buffer_rsrc(a.dst, 10 * fbytes);
-    const __amdgpu_buffer_rsrc_t rfor = buffer_rsrc(a.force, FORCE ? 3 * (uint32_t)a.stride * 4 : 0);
+    const int rp = (int)a.rp;
+    const uint32_t fbytes = (uint32_t)a.stride * E, pbytes = (uint32_t)(rp * ny) * E;
+    const uint32_t sbytes = (uint32_t)state_elems(a) * E;
+    const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(a.src, sbytes);
+    const __amdgpu_buffer_rsrc_t rdst = buffer_rsrc(a.dst, sbytes);
+    const __amdgpu_buffer_rsrc_t rfor = buffer_rsrc(a.force, FORCE ? 3 * (uint32_t)a.fs * 4 : 0);
 
     // source column of this thread (tile + halo) and its LDS slot (the
     // column's index in the window); lanes past the halo keep a slot of
@@ -437,14 +445,14 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     const int sgx = x0 - 1 + wsl % NX, sgy = y0 - 1 + wsl / NX;
     const bool src_lds = tid < NCOL;
     const bool src_ok = src_lds && sgx >= 0 && sgx < nx && sgy >= 0 && sgy < ny;
-    const uint32_t vsrc = src_ok ? (uint32_t)(sgy * nx + sgx) * E : kOOB;
+    const uint32_t vsrc = src_ok ? (uint32_t)(sgy * rp + sgx) * E : kOOB;
 
     // destination cell of this thread
     const int dx = x0 + tid % TX, dy = y0 + tid / TX;
     const bool dst_ok = tid < TX * TY && dx < nx && dy < ny;
     const int px = min(max(dx, 1), nx - 2), py = min(max(dy, 1), ny - 2);
     const int cp = dst_ok ? (py - y0 + 1) * NX + (px - x0 + 1) : 0;
-    const uint32_t vdst = dst_ok ? (uint32_t)(dy * nx + dx) * E : kOOB;
+    const uint32_t vdst = dst_ok ? (uint32_t)(dy * rp + dx) * E : kOOB;
     const uint32_t vfor = dst_ok ? (uint32_t)(py * nx + px) * 4 : kOOB;
 
     const float omega = 1.0f / (a.nu * 3.0f + 0.5f);
@@ -467,7 +475,7 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     auto load_force = [&](int z, float* F) {
         const uint32_t sz = (uint32_t)z * (uint32_t)(nx * ny) * 4;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) F[q] = BufIO<HL_F32>::load(rfor, vfor, sz + q * (uint32_t)a.stride * 4, 0, 1.0f);
+        for (int q = 0; q < 3; ++q) F[q] = BufIO<HL_F32>::load(rfor, vfor, sz + q * (uint32_t)a.fs * 4, 0, 1.0f);
     };
     auto store_plane = [&](int z, uint32_t v, const float* m) {
         const uint32_t sz = (uint32_t)z * pbytes;
@@ -629,7 +637,8 @@ hipError_t launch_tiled(const EnvArgs& a, int zc, FixArgs* fxp, hipStream_t s)
     if ((a.nx % TX) == 1 || (a.nx % TX) == 2 || (a.ny % TY) == 1 || (a.ny % TY) == 2) return hipErrorNotSupported;
     if (zc < 1 || zc > 64) return hipErrorInvalidValue;
     // buffer offsets are 32-bit with the out-of-range marker at 2^31
-    if ((uint64_t)a.stride * 10 * BufIO<DT>::size >= (1ull << 31)) return hipErrorNotSupported;
+    if ((uint64_t)state_elems(a) * BufIO<DT>::size >= (1ull << 31) || (a.force && (uint64_t)a.fs * 3 * 4 >= (1ull << 31)))
+        return hipErrorNotSupported;
     constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
     const int gx = (a.nx + TX - 1) / TX, gy = (a.ny + TY - 1) / TY, gz = (a.nz - 2 + zc - 1) / zc;
     const int ntile = gx * gy * gz;

@@ -4,11 +4,16 @@ the HIP step / force / voxelizer entry points of libhomelbm.
 This is the layer the reference-compatible facades (openhomelbm_amd.lbm and
 openhomelbm_amd.lbm3d) and the batched / served engines are built on.
 
-Layout: moments live in one tensor ``buf[2, nfields, stride]`` (two ping-pong
-states, structure-of-arrays, x fastest). ``cur`` names the state that holds
-the present time level; a step writes the other one and flips ``cur`` (the
-reference instead swaps array handles with the Swap_Mom kernels,
-envs/lbm/lbm_func.py:475-500).
+Layout: moments live in one tensor with two ping-pong states. D2Q9 lattices
+use structure of arrays, ``buf[2, nfields, stride]`` (x fastest). D3Q27
+lattices default to the row-interleaved layout ``buf[2, nz * ny, nfields,
+nx]``: the ten fields of one x-row are consecutive, so the step kernel reads
+and writes one contiguous span per row of cells instead of ten streams 16 MB
+apart (the structure-of-arrays layout stays available, ``layout="soa"``).
+``fields()`` presents either as a (nfields, (z,) y, x) view. ``cur`` names the
+state that holds the present time level; a step writes the other one and
+flips ``cur`` (the reference instead swaps array handles with the Swap_Mom
+kernels, envs/lbm/lbm_func.py:475-500).
 """
 from __future__ import annotations
 
@@ -77,10 +82,13 @@ class Lattice:
         dtype: moment storage, "f32" (default), "f64", "f16" or "bf16".
         nsolid: number of rigid solids in the pose table.
         device: torch device (default the current CUDA device).
+        layout: "rows" (row-interleaved, the D3Q27 default) or "soa"
+            (structure of arrays, the only D2Q9 layout).
     """
 
     def __init__(self, lattice: int, nx: int, ny: int, nz: int = 1, dtype: str = "f32", nsolid: int = 0,
-                 device: Optional[torch.device] = None, scale: Optional[Sequence[float]] = None):
+                 device: Optional[torch.device] = None, scale: Optional[Sequence[float]] = None,
+                 layout: Optional[str] = None):
         if lattice not in (9, 27):
             raise ValueError(f"lattice must be 9 or 27, got {lattice}")
         if lattice == 9 and nz != 1:
@@ -98,9 +106,21 @@ class Lattice:
         self.dtype_code = DTYPE_CODES[dtype]
         self.nfields = 6 if lattice == 9 else 10
         self.ncell = nx * ny * nz
-        self.stride = (self.ncell + 63) // 64 * 64
+        if layout is None:
+            layout = "rows" if lattice == 27 else "soa"
+        if layout not in ("rows", "soa") or (layout == "rows" and lattice != 27):
+            raise ValueError(f"layout must be 'soa', or 'rows' for D3Q27 lattices; got {layout!r}")
+        self.layout = layout
+        # body-force components (and the SoA moment fields) are this many elements apart
+        self.fstride = (self.ncell + 63) // 64 * 64
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
-        self.buf = torch.zeros((2, self.nfields, self.stride), dtype=_TORCH_DTYPE[self.dtype_code], device=self.device)
+        tdt = _TORCH_DTYPE[self.dtype_code]
+        if layout == "rows":
+            self.stride, self.rpitch = nx, self.nfields * nx
+            self.buf = torch.zeros((2, nz * ny, self.nfields, nx), dtype=tdt, device=self.device)
+        else:
+            self.stride, self.rpitch = self.fstride, nx
+            self.buf = torch.zeros((2, self.nfields, self.stride), dtype=tdt, device=self.device)
         self.cur = 0
         self.nsolid = nsolid
         self.pose_width = 8 if lattice == 9 else 24
@@ -147,6 +167,8 @@ class Lattice:
         e.nx, e.ny, e.nz = self.nx, self.ny, self.nz
         e.nsolid = self.nsolid
         e.stride = self.stride
+        e.rpitch = self.rpitch
+        e.force_stride = self.fstride
         e.moments[0] = self.buf[0].data_ptr()
         e.moments[1] = self.buf[1].data_ptr()
         if self.mask is not None:
@@ -184,8 +206,11 @@ class Lattice:
         return self._env_cache
 
     def fields(self, which: Optional[int] = None) -> torch.Tensor:
-        """Device view of one state as (nfields, (z,) y, x)."""
+        """Device view of one state as (nfields, (z,) y, x) (a strided view in
+        the row-interleaved layout: writes through it reach the state)."""
         b = self.cur if which is None else which
+        if self.layout == "rows":
+            return self.buf[b].view(self.nz, self.ny, self.nfields, self.nx).permute(2, 0, 1, 3)
         return self.buf[b, :, : self.ncell].view((self.nfields,) + self.shape)
 
     # ------------------------------------------------------------- state
@@ -202,7 +227,7 @@ class Lattice:
             t = (t * sc).to(torch.float32)
         dev = t.to(device=self.device, dtype=self.buf.dtype)
         for b in (0, 1):
-            self.buf[b, :, : self.ncell].copy_(dev.reshape(self.nfields, self.ncell))
+            self.fields(b).copy_(dev.reshape((self.nfields,) + self.shape))
 
     def get_state(self, dtype=np.float64) -> np.ndarray:
         """Host copy of the current moments, decoded, in oracle layout."""
@@ -453,7 +478,7 @@ class Lattice:
             return
         d = 2 if self.lattice == 9 else 3
         f = np.ascontiguousarray(force, dtype=np.float32).reshape(d, self.ncell)
-        t = torch.zeros((d, self.stride), dtype=torch.float32)
+        t = torch.zeros((d, self.fstride), dtype=torch.float32)
         t[:, : self.ncell] = torch.from_numpy(f)
         self.force = t.to(self.device)
 

@@ -236,3 +236,31 @@ def test_graph_replays_equal_eager_steps(lattice, nsteps):
         assert torch.equal(ha, hb), r
         _ = (torch.randn(1000, device=a.device) * 2).sum().item()  # eager work between replays
         a.mass()
+
+
+@pytest.mark.parametrize("kernel,dtype", [(N.HL_KERNEL_TILED, "f32"), (N.HL_KERNEL_GATHER, "f32"),
+                                          (N.HL_KERNEL_TILED, "bf16"), (N.HL_KERNEL_GATHER, "f64")])
+def test_row_interleaved_layout_equals_structure_of_arrays(kernel, dtype):
+    """The two moment layouts address the same cells: stepping, forces, mass,
+    near-body cells, body force and images agree bitwise between them."""
+    env, caps, poses, force = _tiled_scene(13)
+    m0 = random_state(27, SHAPE, seed=47)
+    out = []
+    for layout in ("soa", "rows"):
+        lat = Lattice(27, env.nx, env.ny, env.nz, dtype=dtype, nsolid=poses.shape[0], layout=layout)
+        lat.nu = env.nu
+        lat.bc_type = list(env.bc_type)
+        lat.bc_value = [list(v) for v in env.bc_value]
+        lat.set_poses(poses)
+        lat.voxelize(caps)
+        lat.set_force(force)
+        lat.set_state(m0)
+        h = torch.zeros((3, lat.nsolid, 6), dtype=torch.float64, device=lat.device)
+        lat.step(3, kernel=kernel, force_history=h)
+        img = lat.render_plane(N.HL_RENDER_VORTICITY, normal=0, index=20)
+        proj = lat.render_projection(N.HL_PROJECT_MAX_SPEED, 2)
+        out.append((lat.get_state(), h.cpu().numpy(), lat.mass(), lat.forces().cpu().numpy(),
+                    img.cpu().numpy(), proj.cpu().numpy()))
+    assert out[1][0].shape == out[0][0].shape
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(a), np.asarray(b))

@@ -124,3 +124,33 @@ def test_polygon_worlds_arguments_rejected_before_any_gpu_call():
     assert call(two, 2, maxv=(4, 0)) == -1 and "world 1: maxv" in lib.hl_last_error().decode()
     assert call(two, 2, verts=(0x10, 0)) == -1 and "world 1: verts" in lib.hl_last_error().decode()
     assert call(two, 2, nsteps=0) == 0  # nothing to do: no device work
+
+
+@pytest.mark.parametrize("lattice,nz,rpitch,stride,force_stride,code,msg", [
+    (27, 8, 80, 8, 0, 0, None),           # row-interleaved: accepted
+    (27, 8, 8, 512, 0, 0, None),          # structure of arrays, explicit row pitch
+    (27, 8, 80, 512, 0, -1, "stride == nx"),
+    (27, 8, 24, 8, 0, -1, "rpitch"),
+    (9, 1, 48, 8, 0, -2, "D3Q27"),
+    (27, 8, 80, 8, 100, -1, "force stride"),
+])
+def test_moment_layouts_are_validated(lattice, nz, rpitch, stride, force_stride, code, msg):
+    """rpitch selects structure of arrays (nx) or the row-interleaved layout
+    (nfields * nx with stride = nx); anything else is rejected before any
+    device work, as is a force stride shorter than the lattice."""
+    lib = N.load()
+    e = N.HLEnv()
+    e.lattice, e.dtype, e.nx, e.ny, e.nz = lattice, 0, 8, 8, nz
+    e.stride, e.rpitch, e.force_stride = stride, rpitch, force_stride
+    e.moments[0] = 0x1000
+    e.moments[1] = 0x2000
+    e.nu = 0.1
+    if force_stride:
+        e.force = 0x4000
+    # an accepted layout reaches the next check (cur), still before any device work
+    rc = lib.hl_step(C.byref(e), 0 if code else 2, 1, 0, None, None)
+    err = lib.hl_last_error().decode()
+    if code:
+        assert rc == code and msg in err, err
+    else:
+        assert rc == -1 and "cur must be" in err, err
