@@ -122,6 +122,24 @@ int32_t hl_voxelize(const hl_env* env, uint8_t* mask, const float* caps, int32_t
 int32_t hl_set_tiling(int32_t lattice, int32_t tx, int32_t ty, int32_t zc);
 int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc);
 
+/* Step-kernel timer: with max_launches > 0, later hl_step calls record a
+ * HIP event pair on their stream around one step-kernel launch in every
+ * `every` (up to max_launches recorded launches; not while the stream is
+ * being captured); max_launches = 0 stops recording. A recorded launch does
+ * not overlap its neighbours, so sampling (every > 1) keeps the timed
+ * sequence close to an unrecorded one. hl_kernel_timer_read waits for the
+ * recorded events, returns the summed launch durations (ms) and their count,
+ * and starts a new count. Measurement only: no reference counterpart. */
+int32_t hl_kernel_timer(int32_t max_launches, int32_t every);
+int32_t hl_kernel_timer_read(double* total_ms, int32_t* launches);
+
+/* Copy state `from` into the other buffer: a plain streaming kernel over the
+ * whole state span (16-byte loads and stores). Duplicates a state into both
+ * ping-pong buffers, as the reference's init writes both (InitFlow3D,
+ * lbm_func_3d.py:22-83); also the HBM streaming ceiling a step is measured
+ * against (it moves the step's algorithmic bytes). */
+int32_t hl_copy_state(const hl_env* env, int32_t from, void* stream);
+
 /* Even-odd polygon voxelizer (D2Q9): device verts [npoly][maxv][2] closed
  * polygons in lattice coordinates, counts [npoly]; mask = index + 1 of the
  * lowest polygon containing the cell centre, 0 elsewhere in the box.

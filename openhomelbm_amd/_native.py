@@ -22,7 +22,7 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
-           "hl_step_polygons_worlds")
+           "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -107,12 +107,15 @@ def load():
     L.hl_voxelize_mesh.argtypes = [env_p, P, P, P, I32, P, I32, P, P]
     L.hl_step_polygons.argtypes = [env_p, I32, I32, P, P, P, I32, P, P, P]
     L.hl_step_polygons_worlds.argtypes = [env_p, I32, I32, I32, P, P, P, P, P, P, P]
+    L.hl_kernel_timer.argtypes = [I32, I32]
+    L.hl_kernel_timer_read.argtypes = [C.POINTER(C.c_double), PI32]
+    L.hl_copy_state.argtypes = [env_p, I32, P]
     for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
                  "hl_set_tiling", "hl_get_tiling", "hl_batch_create", "hl_batch_set_bodies", "hl_batch_set_state",
                  "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
-                 "hl_step_polygons_worlds"):
+                 "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")
@@ -128,6 +131,20 @@ def check(rc: int, what: str) -> None:
 
 def set_tiling(lattice: int, tx: int, ty: int, zc: int) -> None:
     check(load().hl_set_tiling(lattice, tx, ty, zc), "hl_set_tiling")
+
+
+def kernel_timer(max_launches: int, every: int = 1) -> None:
+    """Record HIP events around one step-kernel launch in every `every`, up to
+    max_launches recorded launches (0 stops)."""
+    check(load().hl_kernel_timer(int(max_launches), int(every)), "hl_kernel_timer")
+
+
+def kernel_timer_read():
+    """(summed step-kernel launch time in ms, launches) since the timer was
+    armed or last read; waits for the recorded launches."""
+    ms, n = C.c_double(), C.c_int32()
+    check(load().hl_kernel_timer_read(C.byref(ms), C.byref(n)), "hl_kernel_timer_read")
+    return ms.value, n.value
 
 
 def get_tiling(lattice: int):

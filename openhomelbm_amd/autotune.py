@@ -26,17 +26,21 @@ ZCHUNKS = (8, 16, 32, 64)
 #: OPENHOMELBM_TUNING
 DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
+#: the shape a lattice of an untuned size class runs with
+DEFAULT_TILE = (64, 8, 16)
+
 _table: Optional[Dict[str, Tuple[int, int, int]]] = None
-_active: Dict[int, Tuple[int, int, int]] = {}
 
 
-def size_class(lattice: int, nx: int, ny: int, nz: int = 1) -> str:
-    """Size class key: lattice and cell count bucket (powers of 4 cells)."""
+def size_class(lattice: int, nx: int, ny: int, nz: int = 1, layout: str = "rows") -> str:
+    """Size class key: lattice, cell count bucket (powers of 4 cells) and,
+    for the structure-of-arrays D3Q27 layout, a ':soa' suffix."""
     n = nx * ny * nz
     b = 0
     while (1 << (2 * (b + 1))) <= n:
         b += 1
-    return f"d{2 if lattice == 9 else 3}q{lattice}:4^{b}"
+    suffix = ":soa" if lattice == 27 and layout == "soa" else ""
+    return f"d{2 if lattice == 9 else 3}q{lattice}:4^{b}{suffix}"
 
 
 def table_path() -> str:
@@ -69,21 +73,21 @@ def save_table(entries: Dict[str, dict], path: Optional[str] = None) -> str:
     return p
 
 
-def tuned_tile(lattice: int, nx: int, ny: int, nz: int) -> Optional[Tuple[int, int, int]]:
+def tuned_tile(lattice: int, nx: int, ny: int, nz: int, layout: str = "rows") -> Optional[Tuple[int, int, int]]:
     if _table is None:
         load_table()
-    return _table.get(size_class(lattice, nx, ny, nz))
+    return _table.get(size_class(lattice, nx, ny, nz, layout))
 
 
 def apply(lat) -> None:
-    """Set the tiled kernel's shape for this lattice's size class (if tuned)."""
+    """Set the tiled kernel's shape for this lattice's size class: the tuned
+    entry, or DEFAULT_TILE for a class with none (so the shape never depends
+    on which lattice stepped before)."""
     if lat.lattice != 27:
         return
-    t = tuned_tile(lat.lattice, lat.nx, lat.ny, lat.nz)
-    if t is None or _active.get(27) == t:
-        return
-    N.set_tiling(27, *t)
-    _active[27] = t
+    t = tuned_tile(lat.lattice, lat.nx, lat.ny, lat.nz, lat.layout) or DEFAULT_TILE
+    if N.get_tiling(27) != tuple(t):
+        N.set_tiling(27, *t)
 
 
 def candidates() -> Iterable[Tuple[int, int, int]]:
@@ -108,7 +112,6 @@ def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[
     try:
         for t in (shapes or candidates()):
             N.set_tiling(27, *t)
-            _active[27] = t
             lat.step(warmup, N.HL_KERNEL_TILED, _tuned=False)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             torch.cuda.synchronize()
@@ -125,9 +128,36 @@ def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[
     best = min(timings, key=timings.get)
     tile = [int(x) for x in best.split("x")]
     N.set_tiling(27, *tile)
-    _active[27] = tuple(tile)
     entry = {"tile": tile, "us_per_step": round(timings[best], 2),
              "timings": {k: round(v, 2) for k, v in timings.items()}}
     if save:
-        save_table({size_class(lat.lattice, lat.nx, lat.ny, lat.nz): entry}, path)
+        save_table({size_class(lat.lattice, lat.nx, lat.ny, lat.nz, lat.layout): entry}, path)
     return entry
+
+
+def main(argv=None) -> None:
+    """Tune the size classes of the shipped workloads and write the table:
+    python -m openhomelbm_amd.autotune [--out PATH] [--steps K]."""
+    import argparse
+    import sys
+
+    from .lattice import Lattice
+
+    p = argparse.ArgumentParser()
+    p.add_argument("--out", default=None, help="table path (default: the shipped table)")
+    p.add_argument("--steps", type=int, default=20)
+    a = p.parse_args(argv)
+    # one lattice per size class: 96^3 (4^9), 128^3 (4^10), the 256x128x128 sphere (4^11)
+    for shape in ((96, 96, 96), (128, 128, 128), (256, 128, 128)):
+        lat = Lattice(27, *shape)
+        lat.nu = 0.01
+        lat.bc_type = [0, 1, 1, 1, 1, 1]
+        lat.bc_value = [[0.05, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+        lat.init_uniform((0.05, 0.0, 0.0))
+        e = tune(lat, steps=a.steps, path=a.out, log=lambda m: print(m, file=sys.stderr, flush=True))
+        print(f"{size_class(27, *shape)}: {e['tile']} {e['us_per_step']} us", flush=True)
+        del lat
+
+
+if __name__ == "__main__":
+    main()

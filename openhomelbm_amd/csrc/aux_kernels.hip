@@ -55,6 +55,40 @@ hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, con
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------- state copy
+
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+__global__ void __launch_bounds__(256) k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t copy_state(int lattice, int dtype, const EnvArgs& a, hipStream_t s)
+{
+    const int64_t esize = dtype == HL_F64 ? 8 : (dtype == HL_F32 ? 4 : 2);
+    const int64_t bytes = state_elems(a, lattice == HL_D2Q9 ? 6 : 10) * esize;
+    const uint8_t* src = static_cast<const uint8_t*>(a.src);
+    uint8_t* dst = static_cast<uint8_t*>(a.dst);
+    int64_t body = 0;
+    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) % 16 == 0) {
+        body = bytes / 16 * 16;
+        if (body > 0) {
+            const int64_t n = body / 16, want = (n + 255) / 256;
+            hipLaunchKernelGGL(k_copy16, (int)(want < 2048 ? want : 2048), 256, 0, s,
+                               reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
+        }
+    }
+    if (bytes > body) {
+        const int64_t n = bytes - body, want = (n + 255) / 256;
+        hipLaunchKernelGGL(k_copy1, (int)(want < 2048 ? want : 2048), 256, 0, s, src + body, dst + body, n);
+    }
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------- voxelizer
 
 // Cell centre p is inside capsule (a, b, r) iff its squared distance to the

@@ -164,6 +164,38 @@ int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
     return HL_OK;
 }
 
+// Step-kernel timer (hl_kernel_timer): event pairs recorded around the step
+// kernel launches of hl_step, read back by hl_kernel_timer_read.
+struct KernelTimer {
+    std::mutex mutex;
+    std::vector<hipEvent_t> start, end;
+    size_t used = 0;
+    int32_t cap = 0;
+    int32_t every = 1;   // record one launch in every `every`
+    int64_t seen = 0;    // launches since the timer was armed
+};
+KernelTimer g_timer;
+
+// Records one launch's start event on s; returns the slot or -1.
+int timer_begin(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lock(g_timer.mutex);
+    if (g_timer.used >= (size_t)g_timer.cap) return -1;
+    if (g_timer.seen++ % g_timer.every != 0) return -1;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return -1;
+    const int k = (int)g_timer.used;
+    if (hipEventRecord(g_timer.start[k], s) != hipSuccess) return -1;
+    return k;
+}
+
+void timer_end(int k, hipStream_t s)
+{
+    if (k < 0) return;
+    std::lock_guard<std::mutex> lock(g_timer.mutex);
+    if (hipEventRecord(g_timer.end[k], s) == hipSuccess) g_timer.used = (size_t)k + 1;
+}
+
 // Uniform-flow moments in working precision T, in the operation order of
 // set_uniform_flow_kernel (karman_lbm_env.py:15-53) and
 // set_uniform_flow_3d_kernel (karman_lbm_env_3d.py:15-64).
@@ -300,7 +332,9 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
                     fx.prev_out = pend.out;
                 }
             }
+            const int tk = timer_begin(s);
             e = hl::step_tiled(env->lattice, env->dtype, a, pfx, s);
+            timer_end(e == hipSuccess ? tk : -1, s);
             if (e == hipErrorNotSupported && kernel == HL_KERNEL_TILED)
                 return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d dtype %d", env->lattice, env->dtype);
             if (e == hipSuccess) {
@@ -326,7 +360,9 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
                 fx.prev = pend.rows;
                 fx.prev_rows = pend.nrows;
                 fx.prev_out = pend.out;
+                const int tk = timer_begin(s);
                 e = hl::step_gather_forces(env->dtype, a, fx, s);
+                timer_end(e == hipSuccess ? tk : -1, s);
                 if (e == hipSuccess) {
                     pend.rows = fx.partial;
                     pend.nrows = fx.nfix;
@@ -338,7 +374,11 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
         }
         if (e == hipErrorNotSupported) {
             e = flush();
-            if (e == hipSuccess) e = hl::step_gather(env->lattice, env->dtype, a, s);
+            if (e == hipSuccess) {
+                const int tk = timer_begin(s);
+                e = hl::step_gather(env->lattice, env->dtype, a, s);
+                timer_end(e == hipSuccess ? tk : -1, s);
+            }
         }
         if (e == hipSuccess && !fixed && fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
         if (e != hipSuccess) return hip_fail(e, "step kernel launch");
@@ -481,6 +521,57 @@ int32_t hl_set_tiling(int32_t lattice, int32_t tx, int32_t ty, int32_t zc)
         return fail(HL_E_UNSUPPORTED, "no tiled kernel for lattice %d with tile %dx%dx%d", lattice, tx, ty, zc);
     hl::set_tiling(lattice, hl::TileCfg{tx, ty, zc});
     return HL_OK;
+}
+
+int32_t hl_kernel_timer(int32_t max_launches, int32_t every)
+{
+    if (max_launches < 0 || max_launches > (1 << 20)) return fail(HL_E_INVALID, "max_launches must be in [0, 2^20]");
+    if (every < 1) return fail(HL_E_INVALID, "every must be >= 1");
+    std::lock_guard<std::mutex> lock(g_timer.mutex);
+    while ((int32_t)g_timer.start.size() < max_launches) {
+        hipEvent_t a = nullptr, b = nullptr;
+        hipError_t e = hipEventCreate(&a);
+        if (e == hipSuccess) e = hipEventCreate(&b);
+        if (e != hipSuccess) {
+            if (a) (void)hipEventDestroy(a);
+            return hip_fail(e, "hipEventCreate");
+        }
+        g_timer.start.push_back(a);
+        g_timer.end.push_back(b);
+    }
+    g_timer.cap = max_launches;
+    g_timer.every = every;
+    g_timer.used = 0;
+    g_timer.seen = 0;
+    return HL_OK;
+}
+
+int32_t hl_kernel_timer_read(double* total_ms, int32_t* launches)
+{
+    if (!total_ms || !launches) return fail(HL_E_INVALID, "NULL output");
+    std::lock_guard<std::mutex> lock(g_timer.mutex);
+    double sum = 0.0;
+    for (size_t k = 0; k < g_timer.used; ++k) {
+        hipError_t e = hipEventSynchronize(g_timer.end[k]);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, g_timer.start[k], g_timer.end[k]);
+        if (e != hipSuccess) return hip_fail(e, "kernel timer read");
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (int32_t)g_timer.used;
+    g_timer.used = 0;
+    return HL_OK;
+}
+
+int32_t hl_copy_state(const hl_env* env, int32_t from, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (from != 0 && from != 1) return fail(HL_E_INVALID, "from must be 0 or 1");
+    hl::EnvArgs a = make_args(env, from);
+    hipError_t e = hl::copy_state(env->lattice, env->dtype, a, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? HL_OK : hip_fail(e, "copy kernel launch");
 }
 
 int32_t hl_get_tiling(int32_t lattice, int32_t* tx, int32_t* ty, int32_t* zc)

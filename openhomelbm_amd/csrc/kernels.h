@@ -65,6 +65,8 @@ hipError_t sum_partials(const double* partial, int rows, int width, double* out,
 
 
 hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, const double m[10], hipStream_t s);
+// a.src -> a.dst over the whole state span (plain streaming copy).
+hipError_t copy_state(int lattice, int dtype, const EnvArgs& a, hipStream_t s);
 hipError_t voxelize(int lattice, const EnvArgs& a, uint8_t* mask, const float* caps, int ncap, const int32_t box[6],
                     hipStream_t s);
 hipError_t voxelize_polygons(const EnvArgs& a, uint8_t* mask, const float* verts, const int32_t* counts, int npoly,

@@ -410,6 +410,12 @@ __device__ __forceinline__ int64_t cell_at(const EnvArgs& a, int x, int y, int z
     return (int64_t)x + a.rp * ((int64_t)y + (int64_t)a.ny * z);
 }
 
+// Elements one state spans: the last field's offset plus one field's extent.
+__host__ __device__ __forceinline__ int64_t state_elems(const EnvArgs& a, int nf = 10)
+{
+    return (nf - 1) * a.stride + a.nx + a.rp * ((int64_t)a.ny * a.nz - 1);
+}
+
 // cell_at of the cell with linear index c.
 __device__ __forceinline__ int64_t cell_of(const EnvArgs& a, int64_t c)
 {

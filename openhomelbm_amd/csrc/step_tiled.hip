@@ -265,12 +265,6 @@ template <> struct BufIO<HL_BF16> {
     }
 };
 
-// Elements one state spans: the last field's offset plus one field's extent.
-__host__ __device__ __forceinline__ int64_t state_elems(const EnvArgs& a)
-{
-    return 9 * a.stride + a.nx + a.rp * ((int64_t)a.ny * a.nz - 1);
-}
-
 // Bits [o, o + 64) of a column's cut bitstring (nw words; bits outside it are 0).
 __device__ __forceinline__ uint64_t column_window(const uint64_t* col, int nw, int o)
 {
@@ -297,7 +291,7 @@ __device__ __forceinline__ void wave_sync()
 // workgroup (summed by the next launch). lds: the launch's LDS, used before
 // the workgroup's tiled pass.
 template <int DT, int NT>
-__device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int blk)
+__device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int blk)
 {
     constexpr int W = NT / 64;
     float(*pops)[2][32] = reinterpret_cast<float(*)[2][32]>(lds);
@@ -387,7 +381,7 @@ __device__ void fixup_block(const EnvArgs& a, const FixArgs& fx, float* lds, int
 // butterfly combining them (the order of sum_partials, so the sums agree
 // bitwise).
 template <int NT>
-__device__ void sum_previous(const FixArgs& fx, int blk)
+__device__ __forceinline__ void sum_previous(const FixArgs& fx, int blk)
 {
     constexpr int W = NT / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

@@ -512,6 +512,12 @@ class Lattice:
                                               _stream_ptr(stream)), "hl_render_projection")
         return out
 
+    def copy_state(self, stream=None) -> None:
+        """Copy the current state into the other buffer (a plain streaming
+        kernel over the state span; both buffers then hold the present state)."""
+        e = self._step_env()
+        N.check(self.lib.hl_copy_state(C.byref(e), self.cur, _stream_ptr(stream)), "hl_copy_state")
+
     def mass(self, stream=None) -> float:
         e = self.env()
         N.check(self.lib.hl_mass(C.byref(e), self.cur, self._mass_out.data_ptr(), _stream_ptr(stream)), "hl_mass")

@@ -110,6 +110,21 @@ class HomeFlow:
         self._solid_dirty = True
 
     @property
+    def cut_cell(self) -> bool:
+        """Polygon bodies take the reference's segment-link cut cells (True)
+        or the mask link rule (False); switching it between steps rebuilds
+        the tables the next step needs."""
+        return self._cut_cell
+
+    @cut_cell.setter
+    def cut_cell(self, value) -> None:
+        value = bool(value)
+        if getattr(self, "_cut_cell", None) != value:
+            self._cut_cell = value
+            self._solid_dirty = True
+            self._at_rest = False
+
+    @property
     def lattice(self) -> Lattice:
         if self._lat is None:
             raise RuntimeError("HomeFlow.Initialize(nx, ny, n_objects) has not been called")

@@ -126,10 +126,12 @@ class Server:
 
     Args:
         hbm_budget: bytes of device memory the resident environments may use.
-        streams: streams over which D3Q27 environments are spread per tick.
+        streams: side streams over which D3Q27 environments are spread per
+            tick (default 2: with the caller's stream that stays within the
+            four hardware queues HIP gives a process).
     """
 
-    def __init__(self, hbm_budget: float = 200e9, streams: int = 4, device=None):
+    def __init__(self, hbm_budget: float = 200e9, streams: int = 2, device=None):
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.budget = float(hbm_budget)
         self.used = 0
@@ -352,10 +354,13 @@ def pool(n: int = 32, seed: int = 5005, horizon: int = 4096, frac2d: float = 0.7
     rng = np.random.Generator(np.random.PCG64(seed))
     sizes2 = ((128, 64), (256, 128), (384, 192), (512, 256))
     sizes3 = (96, 128, 160, 192)
+    # exactly round(frac2d * n) D2Q9 environments, in a seeded order
+    kinds = np.array([True] * int(round(frac2d * n)) + [False] * (n - int(round(frac2d * n))))
+    rng.shuffle(kinds)
     out = []
     for k in range(n):
         A, f, phi = float(rng.uniform(0.2, 0.6)), float(rng.uniform(0.5, 2.0)), float(rng.uniform(0, np.pi / 2))
-        if rng.random() < frac2d:
+        if kinds[k]:
             nx, ny = sizes2[int(rng.integers(0, 4))]
             e = inputs.Eel(nx=nx, ny=ny, amplitude=A, frequency=f, phase_lag=phi, head=(0.25 * nx, 0.5 * ny),
                            seg_len=0.3 * nx / inputs.EEL_SEGMENTS)

@@ -267,17 +267,19 @@ def _polygon_worlds(nworld, steps=5, batched=True):
     return s, np.array(forces)
 
 
-@pytest.mark.parametrize("batched", [True, False])
-def test_worlds_equal_each_world_alone(batched):
+@pytest.mark.parametrize("nworld,batched", [(5, True), (5, False), (17, True)])
+def test_worlds_equal_each_world_alone(nworld, batched):
     """Several cut-cell worlds step in one multi-world call (batched) or on
     concurrent side streams; every world's state and forces equal those of
-    the same world stepped in a solver of its own."""
-    s, F = _polygon_worlds(5, batched=batched)
+    the same world stepped in a solver of its own. With 17 worlds (chunks of
+    8, 8 and 1) the middle chunk both sums the previous chunk's force rows
+    and has its own rows summed by the next chunk, at one step per call."""
+    s, F = _polygon_worlds(nworld, batched=batched)
     if batched:
         assert s._streams is None
     else:
         assert s._streams is not None and len(s._streams) == 2
-    for w in range(5):
+    for w in range(nworld):
         alone = LBM_Solver(NX, NY, solid_num=1, nworld=1)
         f = alone.flows[0]
         shape = np.array([[-0.1, -0.05], [0.12, -0.06], [0.08, 0.07], [-0.09, 0.05]], np.float32)
@@ -288,7 +290,7 @@ def test_worlds_equal_each_world_alone(batched):
         f.configure_solid(0, shape, position=(20.0 + 2 * w, 24.0), angle=0.2, scale=0.3)
         for k in range(5):
             f.solid_position = [[20.0 + 2 * w + 0.3 * k, 24.0 - 0.1 * k]]
-            if k == 2 and w == 4:
+            if k == 2 and w == nworld - 1:
                 f.bc_value = [[0.03, 0.0], [0, 0], [0, 0], [0, 0]]
             alone.step()
             got = [f.solid_forcex.numpy()[0], f.solid_forcey.numpy()[0], f.torque.numpy()[0]]
@@ -409,3 +411,36 @@ def test_step_polygons_worlds_of_different_sizes():
         assert np.array_equal(many[w].get_state(np.float32), alone[w].get_state(np.float32)), w
         assert np.array_equal(hists[w].cpu().numpy(), hist_alone[w].cpu().numpy()), w
         assert np.abs(hist_alone[w].cpu().numpy()[:, 0, [0, 1, 5]]).min() > 0
+
+
+def test_switching_cut_cell_after_steps_at_rest():
+    """cut_cell switched on after mask-rule steps with the body at rest: the
+    next steps take the exact cut cells, equal to a solver that starts on
+    them from the same state."""
+    shape = np.array([[-0.1, -0.05], [0.12, -0.06], [0.08, 0.07], [-0.09, 0.05]], np.float32)
+
+    def solver(cut):
+        s = LBM_Solver(NX, NY, solid_num=1, cut_cell=cut)
+        f = s.flows[0]
+        f.vis_shear = 0.015
+        f.bc_type = list(BC_TYPE)
+        f.bc_value = [[0.04, 0.0], [0, 0], [0, 0], [0, 0]]
+        f.configure_solid(0, shape, position=(22.0, 24.0), angle=0.2, scale=0.3)
+        return s
+
+    a = solver(False)
+    a.flows[0].lattice.set_state(random_state(9, (NY, NX), seed=15).astype(np.float32))
+    for _ in range(3):
+        a.step()
+    mid = a.flows[0].lattice.get_state(np.float32)
+    a.flows[0].cut_cell = True
+    b = solver(True)
+    b.flows[0].lattice.set_state(mid)
+    for _ in range(2):
+        a.step()
+        b.step()
+    torch.cuda.synchronize()
+    assert np.array_equal(a.flows[0].lattice.get_state(np.float32), b.flows[0].lattice.get_state(np.float32))
+    assert np.array_equal(a.flows[0].solid_forcex.numpy(), b.flows[0].solid_forcex.numpy())
+    a.flows[0].cut_cell = False  # and back to the mask rule
+    a.step()
