@@ -97,6 +97,20 @@ const char* hl_last_error(void);
 int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history,
                 void* stream);
 
+/* hl_step without any allocation, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). No counterpart in the reference. Same arguments, steps
+ * and results as hl_step; its near-body list and force rows live in a
+ * file-scope device array of 8 MiB per GPU (held until the process ends)
+ * instead of a per-stream workspace. A lattice whose near-body work needs
+ * more (a mask box of more than about 1.2 million cells) is rejected with
+ * HL_E_NOMEM. Every call on a GPU shares that GPU's array, so calls whose
+ * device work overlaps on one GPU (several streams, or host threads) race;
+ * the array keeps the last call's contents (a call builds what it reads
+ * first, so nothing needs clearing). */
+int32_t hl_step_noalloc(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history,
+                        void* stream);
+
 /* Per-solid momentum-exchange force and torque of state `cur`, FP64,
  * deterministic. out: device [nsolid][6] = Fx, Fy, Fz, Tx, Ty, Tz.
  * Replaces the atomics of lbm_func.py:392-401 / lbm_func_3d.py:497-511 and the

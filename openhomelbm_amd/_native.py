@@ -22,7 +22,8 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
-           "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state")
+           "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
+           "hl_step_noalloc")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -79,6 +80,7 @@ def load():
     L.hl_last_error.restype = C.c_char_p
     env_p = C.POINTER(HLEnv)
     L.hl_step.argtypes = [env_p, I32, I32, I32, P, P]
+    L.hl_step_noalloc.argtypes = [env_p, I32, I32, I32, P, P]
     L.hl_forces.argtypes = [env_p, I32, P, P]
     L.hl_init_uniform.argtypes = [env_p, I32, C.c_float, C.c_float, C.c_float, P]
     L.hl_voxelize.argtypes = [env_p, P, P, I32, P, P]
@@ -115,7 +117,8 @@ def load():
                  "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
-                 "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state"):
+                 "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
+                 "hl_step_noalloc"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

@@ -118,16 +118,19 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
 // from several host threads on one stream are not supported). Created on first use, kept until the process ends. Call a step or
 // reduction once outside any stream capture so the allocation does not happen
 // while capturing.
+// The near-body list a workspace holds and what it was built for (hl_step
+// with a nonzero mask_epoch); any other use of the workspace clears it.
+struct ListCache {
+    bool valid = false;
+    uint64_t epoch = 0;
+    const void* mask = nullptr;
+    int32_t key[10] = {0};
+};
 struct StreamCtx {
     std::mutex mutex;  // held for the whole call that uses this context
     void* ws = nullptr;
     size_t bytes = 0;
-    // the near-body list in ws and what it was built for (hl_step with a
-    // nonzero mask_epoch); any other use of ws clears it
-    bool list_valid = false;
-    uint64_t list_epoch = 0;
-    const void* list_mask = nullptr;
-    int32_t list_key[10] = {0};
+    ListCache list;
 };
 std::mutex g_ctx_mutex;
 std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
@@ -155,7 +158,7 @@ int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
             c->ws = nullptr;
             c->bytes = 0;
         }
-        c->list_valid = false;
+        c->list.valid = false;
         e = hipMalloc(&c->ws, bytes);
         if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
         c->bytes = bytes;
@@ -243,41 +246,24 @@ template <typename T> void uniform_moments(int lattice, T ux, T uy, T uz, double
 
 }  // namespace
 
-namespace hl {
-int32_t report(int32_t code, const char* msg) { return fail(code, "%s", msg); }
-int32_t check_env(const hl_env* e) { return validate(e); }
-EnvArgs env_args(const hl_env* e, int cur) { return make_args(e, cur); }
-}  // namespace hl
+namespace {
 
-extern "C" {
-
-int32_t hl_version(void) { return 100; }
-
-int32_t hl_env_bytes(void) { return (int32_t)sizeof(hl_env); }
-
-const char* hl_last_error(void) { return g_error.c_str(); }
-
-int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history, void* stream)
+// hl_step after its argument checks, with the reduction workspace from
+// get_ws(bytes, &ptr) and the near-body list cache that goes with it.
+template <typename GetWs>
+int32_t step_core(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history, hipStream_t s,
+                  ListCache& cache, GetWs get_ws)
 {
-    int32_t rc = validate(env);
-    if (rc) return rc;
-    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
-    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
-    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
-    hipStream_t s = static_cast<hipStream_t>(stream);
+    int32_t rc = HL_OK;
     // the near-body pass needs the cut-cell list when a mask is present and
     // either the tiled kernel runs or forces are requested
     const bool tiled = kernel != HL_KERNEL_GATHER && hl::tiling_supported(env->lattice, 32, 8, 1) &&
                        env->dtype != HL_F64;
     const bool want_forces = force_history && env->nsolid > 0;
-    StreamCtx* ctx = nullptr;
-    rc = stream_ctx(s, &ctx);
-    if (rc) return rc;
-    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
     if (env->mask && nsteps > 0 && (tiled || want_forces)) {
         hl::EnvArgs a0 = make_args(env, cur);
-        rc = workspace(ctx, s, hl::forces_workspace_bytes(a0, env->lattice), &ws);
+        rc = get_ws(hl::forces_workspace_bytes(a0, env->lattice), &ws);
         if (rc) return rc;
         // the list depends on the mask, its box and the lattice; a caller that
         // versions its mask (mask_epoch) gets it rebuilt only when that changes.
@@ -286,16 +272,16 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
                                  env->mask_box[2], env->mask_box[3], env->mask_box[4], env->mask_box[5]};
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusActive;
-        const bool reuse = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone && ctx->list_valid &&
-                           ctx->list_epoch == env->mask_epoch && ctx->list_mask == env->mask &&
-                           std::memcmp(ctx->list_key, key, sizeof(key)) == 0;
+        const bool reuse = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone && cache.valid &&
+                           cache.epoch == env->mask_epoch && cache.mask == env->mask &&
+                           std::memcmp(cache.key, key, sizeof(key)) == 0;
         if (!reuse) {
             hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
             if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
-            ctx->list_valid = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone;
-            ctx->list_epoch = env->mask_epoch;
-            ctx->list_mask = env->mask;
-            std::memcpy(ctx->list_key, key, sizeof(key));
+            cache.valid = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone;
+            cache.epoch = env->mask_epoch;
+            cache.mask = env->mask;
+            std::memcpy(cache.key, key, sizeof(key));
         }
     }
     // force rows a tiled launch left for the next launch to sum
@@ -389,6 +375,90 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
     return HL_OK;
 }
 
+}  // namespace
+
+namespace {
+
+// Scratch of the allocation-free entry point: one array per GPU (every GPU
+// that runs the library holds it until the process ends). In host code the
+// name is only a stand-in for the symbol: its device address comes from
+// hipGetSymbolAddress, once per device with that device current.
+constexpr size_t kNoAllocScratchBytes = (size_t)8 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+struct NoAllocDevice {
+    std::mutex mutex;
+    void* ptr = nullptr;  // device address of this GPU's scratch
+    ListCache list;
+};
+NoAllocDevice g_noalloc[kMaxDevices];
+
+}  // namespace
+
+namespace hl {
+int32_t report(int32_t code, const char* msg) { return fail(code, "%s", msg); }
+int32_t check_env(const hl_env* e) { return validate(e); }
+EnvArgs env_args(const hl_env* e, int cur) { return make_args(e, cur); }
+}  // namespace hl
+
+extern "C" {
+
+int32_t hl_version(void) { return 100; }
+
+int32_t hl_env_bytes(void) { return (int32_t)sizeof(hl_env); }
+
+const char* hl_last_error(void) { return g_error.c_str(); }
+
+int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    return step_core(env, cur, nsteps, kernel, force_history, s, ctx->list,
+                     [&](size_t bytes, void** ws) { return workspace(ctx, s, bytes, ws); });
+}
+
+int32_t hl_step_noalloc(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history,
+                        void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(HL_E_UNSUPPORTED, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+    NoAllocDevice& d = g_noalloc[dev];
+    std::lock_guard<std::mutex> lock(d.mutex);
+    if (!d.ptr) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+        d.ptr = p;
+    }
+    return step_core(env, cur, nsteps, kernel, force_history, static_cast<hipStream_t>(stream), d.list,
+                     [&](size_t bytes, void** ws) -> int32_t {
+                         if (bytes > kNoAllocScratchBytes)
+                             return fail(HL_E_NOMEM,
+                                         "hl_step_noalloc: the near-body work of this lattice needs %zu bytes of "
+                                         "scratch, more than the %zu bytes the allocation-free entry point holds "
+                                         "(a smaller mask box, or hl_step)",
+                                         bytes, kNoAllocScratchBytes);
+                         *ws = d.ptr;
+                         return HL_OK;
+                     });
+}
+
 int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const float* verts,
                          const float* verts_last, const int32_t* counts, int32_t maxv, const float* radius2,
                          double* force_history, void* stream)
@@ -408,7 +478,7 @@ int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const f
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
-    ctx->list_valid = false;  // ws is reused below
+    ctx->list.valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::polygon_workspace_bytes(), &ws);
     if (rc) return rc;
     hl::EnvArgs a = make_args(env, cur);
@@ -456,7 +526,7 @@ int32_t hl_step_polygons_worlds(const hl_env* envs, int32_t nworld, int32_t cur,
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
-    ctx->list_valid = false;  // ws is reused below
+    ctx->list.valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::polygon_worlds_workspace_bytes(total_width), &ws);
     if (rc) return rc;
     hipError_t e = hl::step_polygons_worlds(envs[0].dtype, args.data(), nworld, nsteps, verts, last.data(), counts,
@@ -478,7 +548,7 @@ int32_t hl_forces(const hl_env* env, int32_t cur, double* out, void* stream)
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
-    ctx->list_valid = false;  // ws is reused below
+    ctx->list.valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::forces_workspace_bytes(a, env->lattice), &ws);
     if (rc) return rc;
     hipError_t e = hl::forces(env->lattice, env->dtype, a, out, ws, s);
@@ -612,7 +682,7 @@ int32_t hl_mass(const hl_env* env, int32_t cur, double* out, void* stream)
     if (rc) return rc;
     std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
     void* ws = nullptr;
-    ctx->list_valid = false;  // ws is reused below
+    ctx->list.valid = false;  // ws is reused below
     rc = workspace(ctx, s, hl::mass_workspace_bytes(a, env->lattice), &ws);
     if (rc) return rc;
     hipError_t e = hl::mass(env->lattice, env->dtype, a, out, ws, s);
