@@ -384,18 +384,7 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
                 for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
             }
         }
-        if (partial) {
-            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
-            while (uni) {
-                const int s = __builtin_ctzll(uni);
-                uni &= uni - 1;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double v = wave_sum(sid == s ? (double)F[q] : 0.0);
-                    if (lane == 0) acc[s * 6 + q] += v;
-                }
-            }
-        }
+        if (partial) wave_solid_sums(sid, F, kRow6, lane, acc);
         __syncthreads();  // pops are rewritten by the next pair
     }
     if (!partial) return;

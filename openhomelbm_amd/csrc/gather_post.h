@@ -117,7 +117,8 @@ __device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[
 // the reconstruction from the source cell's moments, clamped at 0, or on a
 // cut link the unclamped wall-rule population from the cell's moments mc.
 template <typename T, int DT>
-__device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, int i, const T* mc, bool masked, int mval_c)
+__device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, int i, const T* mc, bool masked, int mval_c,
+                                        const float* poses = nullptr)
 {
     using S = Storage<DT>;
     using ST = typename S::type;
@@ -139,7 +140,8 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
     T pop_i;
     if (cut) {
         T up[3], r[3];
-        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+        wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy,
+                            T(z) - T(0.5) * dz, up, r);
         T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
         T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
         T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
@@ -197,12 +199,19 @@ __device__ __forceinline__ uint32_t d3q27_group_bits(int k)
     return bits;
 }
 
+// Group k's sum over pop (27 entries, e.g. in LDS): all 27 read at once, the
+// members added in ascending order and +0 added for the others. From +0 a
+// sum of IEEE additions never reaches -0, so adding +0 leaves it unchanged:
+// the result is bitwise that of adding the members alone.
 template <typename T> __device__ __forceinline__ T d3q27_group_sum(const T* pop, int k)
 {
     const uint32_t bits = d3q27_group_bits(k);
+    T p[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) p[i] = pop[i];
     T acc = 0;
-    for (int i = 0; i < 27; ++i)
-        if ((bits >> i) & 1u) acc = acc + pop[i];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) acc = acc + (((bits >> i) & 1u) ? p[i] : T(0));
     return acc;
 }
 
@@ -281,7 +290,8 @@ __device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z
 // Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
 // torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
 template <typename T, int DT, int Q>
-__device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x, int y, int z, int i, int sid, T F[6])
+__device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x, int y, int z, int i, int sid, T F[6],
+                                           const float* poses = nullptr)
 {
     if (Q == 9) {
         T up[2], r[2];
@@ -301,7 +311,8 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
     } else {
         const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
         T up[3], r[3];
-        wall_velocity_3d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, up, r);
+        wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy,
+                            T(z) - T(0.5) * dz, up, r);
         T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
         T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
         T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];

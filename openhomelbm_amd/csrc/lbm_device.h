@@ -351,6 +351,48 @@ __device__ __forceinline__ double lane_rows_sum(const double* rows, int nrows, i
     return v;
 }
 
+// N wavefront-wide FP64 sums at once: wave_sum's butterfly with the N
+// exchanges of each stage issued together, so their latencies overlap (each
+// sum is bitwise that of wave_sum).
+template <int N> __device__ __forceinline__ void wave_sums(double* v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        double t[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) t[q] = __shfl_xor(v[q], off, 64);
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] += t[q];
+    }
+}
+
+// Per-solid FP64 sums of the lanes' link forces into the solid's force row:
+// F[q] goes to acc[6 sid + Q[q]] (lane 0 adds). One pass per solid present
+// in the wave (the lowest lane's solid first), each a wave_sums over the
+// lanes of that solid; every solid gets one addition per call, so the sums
+// are those of one wave_sum per solid and component. sid < 0: no link.
+template <int N, typename T>
+__device__ __forceinline__ void wave_solid_sums(int sid, const T* F, const int (&Q)[N], int lane, double* acc)
+{
+    int s = sid;
+    for (;;) {
+        const uint64_t act = __ballot(s >= 0);
+        if (!act) break;
+        const int sv = __builtin_amdgcn_readlane(s, (int)__builtin_ctzll(act));
+        double v[N];
+#pragma unroll
+        for (int q = 0; q < N; ++q) v[q] = s == sv ? (double)F[q] : 0.0;
+        wave_sums<N>(v);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < N; ++q) acc[sv * 6 + Q[q]] += v[q];
+        }
+        if (s == sv) s = -1;
+    }
+}
+constexpr int kRow6[6] = {0, 1, 2, 3, 4, 5};   // F, T of a 3D link (2D: Fx, Fy, 0, 0, 0, Tz)
+constexpr int kRow2D[3] = {0, 1, 5};           // Fx, Fy, Tz of a 2D link
+
 __device__ __forceinline__ uint64_t wave_or(uint64_t v)
 {
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);

@@ -343,21 +343,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
                     }
                 }
             }
-            if (partial) {
-                uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
-                while (uni) {
-                    const int s = __builtin_ctzll(uni);
-                    uni &= uni - 1;
-                    const bool m = sid == s;
-                    const double fx = wave_sum(m ? (double)F[0] : 0.0), fy = wave_sum(m ? (double)F[1] : 0.0);
-                    const double tz = wave_sum(m ? (double)F[2] : 0.0);
-                    if (lane == 0) {
-                        acc[s * 6 + 0] += fx;
-                        acc[s * 6 + 1] += fy;
-                        acc[s * 6 + 5] += tz;
-                    }
-                }
-            }
+            if (partial) wave_solid_sums(sid, F, kRow2D, lane, acc);
         }
         if (near) {
             T q0 = d2q9_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], 0);
@@ -484,21 +470,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
 #pragma unroll
             for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
         }
-        if (partial) {
-            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
-            while (uni) {
-                const int s = __builtin_ctzll(uni);
-                uni &= uni - 1;
-                const bool m = sid == s;
-                const double fx = wave_sum(m ? (double)F[0] : 0.0), fy = wave_sum(m ? (double)F[1] : 0.0);
-                const double tz = wave_sum(m ? (double)F[2] : 0.0);
-                if (lane == 0) {
-                    acc[s * 6 + 0] += fx;
-                    acc[s * 6 + 1] += fy;
-                    acc[s * 6 + 5] += tz;
-                }
-            }
-        }
+        if (partial) wave_solid_sums(sid, F, kRow2D, lane, acc);
         // the lead lanes' reads of pops end before the next pass writes them
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();

@@ -91,16 +91,7 @@ __global__ void __launch_bounds__(256) k_gather_d2q9_fx(EnvArgs a, FixArgs fx)
                     }
                 }
             }
-            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
-            while (uni) {
-                const int s = __builtin_ctzll(uni);
-                uni &= uni - 1;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double v = wave_sum(sid == s ? (double)F[q] : 0.0);
-                    if (lane == 0) acc[s * 6 + q] += v;
-                }
-            }
+            wave_solid_sums(sid, F, kRow6, lane, acc);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         __builtin_amdgcn_wave_barrier();

@@ -297,14 +297,24 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
     float(*pops)[2][32] = reinterpret_cast<float(*)[2][32]>(lds);
     float(*sums)[2][16] = reinterpret_cast<float(*)[2][16]>(lds + W * 64);
     double* accs = reinterpret_cast<double*>(lds + W * 96);
+    // the pose table, copied to LDS with the first loads: the wall rule of a
+    // cut link reads it without another global-memory round trip
+    float* poses = lds + W * 96 + 2 * W * HL_MAX_SOLIDS * 6;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, h = lane >> 5, i = lane & 31;
     const int nw = fx.nfix * W, gw = blk * W + wave;
     const int width = fx.partial ? fx.width : 0;
     double* acc = accs + wave * width;
     for (int k = lane; k < width; k += 64) acc[k] = 0.0;
+    for (int k = tid; k < a.nsolid * 24; k += NT) poses[k] = a.solids[k];
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const Region rg = fx.rg;
+    // the count and this lane's first list entry in one round trip: the entry
+    // is read inside the list's capacity (the region's cell count) and used
+    // only when it is below the count
+    const int first = gw * 2 + h;
     const int ncut = *fx.count;
+    const int tt_first = fx.list[min((int64_t)first, rg.n() - 1)];
+    __syncthreads();  // the pose table is in LDS
     for (int base = gw * 2; base < ncut; base += nw * 2) {
         const int e = base + h;
         const bool cell = e < ncut;
@@ -312,7 +322,7 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
         int64_t c = 0;
         float F[6] = {0, 0, 0, 0, 0, 0};
         if (cell) {
-            const int tt = fx.list[e];
+            const int tt = e == first ? tt_first : fx.list[e];
             x = rg.x0 + tt % rg.w;
             y = rg.y0 + (tt / rg.w) % rg.h;
             z = rg.z0 + tt / (rg.w * rg.h);
@@ -325,7 +335,7 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
         if (cell && i < 27) {
             float mc[10];
             load_cell<float, DT>(a, cell_at(a, px, py, pz), 10, mc);
-            pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c);
+            pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c, poses);
             // force of this lane's link (interior cells only: a face cell repeats its projection)
             if (width && i > 0 && px == x && py == y && pz == z) {
                 const int mv = a.mask[c - D3CX[i] - (int64_t)nx * (D3CY[i] + (int64_t)ny * D3CZ[i])];
@@ -333,7 +343,7 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
                     const int s2 = mval_c ? mval_c - 1 : mv - 1;
                     if (s2 < a.nsolid) {
                         sid = s2;
-                        link_force<float, DT, 27>(a, mc, x, y, z, i, sid, F);
+                        link_force<float, DT, 27>(a, mc, x, y, z, i, sid, F, poses);
                     }
                 }
             }
@@ -352,18 +362,7 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
 #pragma unroll
             for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + d, m[f], f, a.scale[f]);
         }
-        if (width) {
-            uint64_t uni = wave_or(sid >= 0 ? 1ull << sid : 0ull);
-            while (uni) {
-                const int sv = __builtin_ctzll(uni);
-                uni &= uni - 1;
-#pragma unroll
-                for (int q = 0; q < 6; ++q) {
-                    const double v = wave_sum(sid == sv ? (double)F[q] : 0.0);
-                    if (lane == 0) acc[sv * 6 + q] += v;
-                }
-            }
-        }
+        if (width) wave_solid_sums(sid, F, kRow6, lane, acc);
         wave_sync();  // pops and sums are rewritten by the next pass
     }
     if (!width) return;
@@ -400,7 +399,8 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     using IO = BufIO<DT>;
     constexpr uint32_t E = IO::size;
     __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
-    static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) <= sizeof(sp), "near-body work fits the LDS");
+    static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) + HL_MAX_SOLIDS * 24 * 4 <= sizeof(sp),
+                  "near-body work fits the LDS");
     const int L = (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
@@ -551,6 +551,15 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         __syncthreads();
     };
 
+    // the near-body cells of this step, spread over the first fx.nfix
+    // workgroups ahead of their tiled pass (tiled cells skip them). They run
+    // before the tiled prologue's loads are issued: vector-memory loads
+    // complete in order, so behind those loads each of their dependent round
+    // trips would wait for whole planes
+    if (fx.list) {
+        if (L < fx.nfix) fixup_block<DT, NT>(a, fx, &sp[0][0][0], L);
+        __syncthreads();  // the LDS is free again
+    }
     // prologue: planes zb-1 (up pops), zb (mid/up pops), zb+1 (into buffer 0)
     {
         float m0[10], m1[10], m2[10];
@@ -559,14 +568,6 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         load_plane(zb + 1, m2);
         if (zb + 2 <= ze) load_plane(zb + 2, pfa);
         if (FORCE) load_force(zb, fa);
-        // the near-body cells of this step, spread over the first fx.nfix
-        // workgroups ahead of their tiled pass (tiled cells skip them); the
-        // loads above land meanwhile
-        if (fx.list) {
-            if (L < fx.nfix) fixup_block<DT, NT>(a, fx, &sp[0][0][0], L);
-            if (fx.prev) sum_previous<NT>(fx, L);
-            __syncthreads();  // the LDS is free again
-        }
         if (src_lds) {
             float f[27];
             recon27(m0, f);
@@ -601,6 +602,8 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
             if (z + 2 < ze) step(z + 2, pfb, pfa, up_a, fb, fa, true);
         }
     }
+    // the previous step's force rows (a few waves of the grid), after the march
+    if (fx.list && fx.prev) sum_previous<NT>(fx, L);
 }
 
 // Tile shape of the D3Q27 kernel (set by hl_set_tiling / the autotuner).
