@@ -438,6 +438,9 @@ def main():
     batched = None
     if args.batched_steps > 0 and args.workload == "d3q27_sphere":
         batched = bench_batched(args.batched_steps, 20, min(args.batched_steps, 100))
+        t0d = time.perf_counter()
+        batched["fp64_drag"] = inputs.batched_drag_check(64, 3003, steps=args.batched_steps)
+        batched["fp64_drag"]["seconds"] = round(time.perf_counter() - t0d, 1)
         log(f"[bench] batched {batched}")
 
     serve = None

@@ -269,6 +269,54 @@ def build_eel_batch(n: int = 64, seed: int = 3003, steps: int = 2000):
     return b, lats, eels
 
 
+def eel_alone(e: Eel, index: int, seed: int = 3003, steps: int = 2000, dtype: str = "f64"):
+    """Environment `index` of an eel batch stepped on its own (one Lattice,
+    the gather kernel, masks re-voxelized every step from the same tables):
+    the reference computation of the batched run, in FP64 by default.
+    Returns the device force history [steps, 8, 6] (FP64)."""
+    import torch
+
+    from . import _native as N
+    from .lattice import Lattice
+
+    L = Lattice(9, e.nx, e.ny, dtype=dtype, nsolid=EEL_SEGMENTS)
+    L.nu = e.nu
+    L.bc_type = [0, 2, 1, 2, 1, 1]
+    L.bc_value = [[EEL_U, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+    L.set_state(eel_state(e, index, seed))
+    caps, poses = e.tables(0, steps)
+    box = e.box(0, steps)
+    caps_d, poses_d = torch.from_numpy(caps).to(L.device), torch.from_numpy(poses).to(L.device)
+    hist = torch.zeros((steps, EEL_SEGMENTS, 6), dtype=torch.float64, device=L.device)
+    for t in range(steps):
+        L.voxelize_device(caps_d[t], box)
+        L.solids[:EEL_SEGMENTS].copy_(poses_d[t])
+        L.step(1, N.HL_KERNEL_GATHER, force_history=hist[t:t + 1])
+    return hist
+
+
+def batched_drag_check(n: int = 64, seed: int = 3003, steps: int = 400):
+    """Mean drag of every eel of the batched FP32 run against the same eel
+    stepped alone in FP64 over the same steps: per-eel relative errors of the
+    mean total drag (sum of Fx over the chain), their maximum and the error of
+    the batch-wide mean."""
+    import torch
+
+    b, lats, eels = build_eel_batch(n, seed, steps=steps)
+    h = b.history_buffer(steps)
+    b.step(steps, force_history=h)
+    d32 = h[:, :, :EEL_SEGMENTS, 0].sum(-1).mean(0).cpu().numpy()  # [n]
+    b.close()
+    del b, lats
+    d64 = np.array([float(eel_alone(e, i, seed, steps)[:, :, 0].sum(-1).mean()) for i, e in enumerate(eels)])
+    torch.cuda.synchronize()
+    rel = np.abs(d32 - d64) / np.abs(d64)
+    return {"envs": n, "steps": steps, "mean_drag_rel_err_max": float(rel.max()),
+            "mean_drag_rel_err_median": float(np.median(rel)),
+            "batch_mean_drag_rel_err": float(abs(d32.mean() - d64.mean()) / abs(d64.mean())),
+            "reference": "each eel alone, FP64 gather kernel, same kinematics tables"}
+
+
 @dataclass
 class Eel3D:
     """Articulated capsule chain swimming in the x-y plane of a D3Q27 lattice

@@ -186,3 +186,14 @@ def test_history_layout_and_static_bodies():
     torch.cuda.synchronize()
     assert h.shape == (1, 1, MAX_SOLIDS, 6)
     torch.testing.assert_close(h[0, 0, :EEL_SEGMENTS], ref, rtol=1e-12, atol=1e-15)
+
+
+def test_batched_drag_within_one_percent_of_fp64():
+    """Mean drag of batched FP32 eels against each eel alone in FP64 (the
+    gather kernel, same kinematics): within the 1% bar (a reduced set: 6 eels
+    of the batch workload, 300 steps)."""
+    from openhomelbm_amd import inputs
+    r = inputs.batched_drag_check(n=6, seed=3003, steps=300)
+    print(r)
+    assert r["mean_drag_rel_err_max"] < 0.01
+    assert r["batch_mean_drag_rel_err"] < 0.01
