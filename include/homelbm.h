@@ -53,6 +53,7 @@ extern "C" {
 #define HL_E_NOMEM (-4)
 
 #define HL_MAX_SOLIDS 64
+#define HL_MAX_CAPSULES 4096
 
 typedef struct hl_env {
     int32_t lattice;      /* HL_D2Q9 or HL_D3Q27 */
@@ -192,6 +193,29 @@ int32_t hl_voxelize_mesh(const hl_env* env, uint8_t* mask, const float* tris, co
 int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const float* verts,
                          const float* verts_last, const int32_t* counts, int32_t maxv, const float* radius2,
                          double* force_history, void* stream);
+
+/* D3Q27 steps whose cut links are the crossings of capsule surfaces instead
+ * of the mask: link i of a cell is cut when the ray from the cell along -c_i
+ * meets a capsule surface at t in [0, 1] (the nearest such crossing over
+ * every capsule, the first capsule winning ties), and a cut link takes the
+ * wall rule at the hit point x - c_i t, its velocity and lever arm from the
+ * pose table at that point. Replaces the ray query of get_cutcell_multi_3d
+ * (envs/lbm3d/lbm_func_3d.py:254-316) and its use in stream_and_collide_3d
+ * (lbm_func_3d.py:424-511) for bodies that are capsules and spheres (a
+ * capsule with a == b), whose surfaces cross links in closed form. Every
+ * interior cell takes the update, inside a body or not (the reference flags
+ * no 3D cell solid). Device input caps: [ncap][8] = ax, ay, az, bx, by, bz,
+ * radius, solid_id in lattice coordinates at this pose (hl_voxelize's table;
+ * capsules with solid_id outside [0, nsolid) are ignored), at most
+ * HL_MAX_CAPSULES. Host input box: a half-open cell box holding every capsule
+ * (hl_voxelize's box); links farther than one cell from it are never cut.
+ * env->mask is ignored; kernel and force_history as in hl_step (forces of
+ * the state each step reads; the torque about the pose table's position). The
+ * near-body cells are listed from the capsules at the start of every call.
+ * FP32 / FP16 / BF16 take the tiled kernel, FP64 the gather kernel and a
+ * near-body pass. */
+int32_t hl_step_capsules(const hl_env* env, const float* caps, int32_t ncap, const int32_t box[6], int32_t cur,
+                         int32_t nsteps, int32_t kernel, double* force_history, void* stream);
 
 /* hl_step_polygons for nworld independent worlds of one dtype, all at buffer
  * `cur`: the world loop of LBM_Solver.step (envs/lbm/lbm_solver.py:97-141)

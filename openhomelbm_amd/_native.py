@@ -23,7 +23,7 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
            "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-           "hl_step_noalloc")
+           "hl_step_noalloc", "hl_step_capsules")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -108,6 +108,7 @@ def load():
     L.hl_body_capsules.argtypes = [I32, I32, P, I32, P, I32, P, P, P, P, P, P, P]
     L.hl_voxelize_mesh.argtypes = [env_p, P, P, P, I32, P, I32, P, P]
     L.hl_step_polygons.argtypes = [env_p, I32, I32, P, P, P, I32, P, P, P]
+    L.hl_step_capsules.argtypes = [env_p, P, I32, P, I32, I32, I32, P, P]
     L.hl_step_polygons_worlds.argtypes = [env_p, I32, I32, I32, P, P, P, P, P, P, P]
     L.hl_kernel_timer.argtypes = [I32, I32]
     L.hl_kernel_timer_read.argtypes = [C.POINTER(C.c_double), PI32]
@@ -118,7 +119,7 @@ def load():
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
                  "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-                 "hl_step_noalloc"):
+                 "hl_step_noalloc", "hl_step_capsules"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

@@ -190,6 +190,9 @@ static void axis_range(int b0, int b1, int n, int& lo, int& hi)
     if (hi == n - 1) hi = n;
 }
 
+// Bodies that cut links: a mask, or capsules (which take precedence).
+static bool has_bodies(const EnvArgs& a) { return a.mask != nullptr || a.caps != nullptr; }
+
 static Region fixup_region(const EnvArgs& a, int lattice)
 {
     Region r{0, 0, 0, 0, 0, 0};
@@ -238,6 +241,19 @@ __device__ __forceinline__ bool region_cell_cut(const EnvArgs& a, const Region& 
     const int z = rg.z0 + t / (rg.w * rg.h);
     const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
     const int pz = Q == 9 ? 0 : min(max(z, 1), nz - 2);
+    if (Q == 27 && a.caps) {  // a crossing of any capsule surface
+        // in FP32 or FP64, so the list holds the cells of either precision
+        bool has = false;
+        for (int i = 1; i < 27 && !has; ++i) {
+            float t32;
+            double t64;
+            has = capsule_cut<float>(a.caps, a.ncap, a.nsolid, float(px), float(py), float(pz), D3CX[i], D3CY[i],
+                                     D3CZ[i], t32) >= 0 ||
+                  capsule_cut<double>(a.caps, a.ncap, a.nsolid, double(px), double(py), double(pz), D3CX[i],
+                                      D3CY[i], D3CZ[i], t64) >= 0;
+        }
+        return has;
+    }
     if (!in_box_halo(a, px, py, pz)) return false;
     const int64_t c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
     const int mv_c = a.mask[c] != 0;
@@ -340,16 +356,22 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
             py = min(max(y, 1), ny - 2);
             pz = Q == 9 ? 0 : min(max(z, 1), nz - 2);
             c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
-            mval_c = a.mask[c];
+            mval_c = a.caps ? 0 : a.mask[c];
         }
         if (act) {
             T mc[10];
             load_cell<T, DT>(a, cell_at(a, px, py, pz), Q == 9 ? 6 : 10, mc);
+            WallPoint<T> w{-1, {T(0), T(0), T(0)}};
             const T pop = Q == 9 ? d2q9_pull<T, DT>(a, px, py, i, mc, true, mval_c)
-                                 : d3q27_pull<T, DT>(a, px, py, pz, i, mc, true, mval_c);
+                                 : d3q27_pull<T, DT>(a, px, py, pz, i, mc, true, mval_c, nullptr, &w);
             pops[h][i] = pop;
             // force of this lane's link (interior cells only: a face cell repeats its projection)
-            if (partial && i > 0 && px == x && py == y && pz == z) {
+            if (Q == 27 && a.caps) {
+                if (partial && px == x && py == y && pz == z && w.sid >= 0) {
+                    sid = w.sid;
+                    link_force_at<T>(a, mc, i, sid, w.h[0], w.h[1], w.h[2], F, nullptr);
+                }
+            } else if (partial && i > 0 && px == x && py == y && pz == z) {
                 const int cx = Q == 9 ? D2CX[i] : D3CX[i], cy = Q == 9 ? D2CY[i] : D3CY[i], cz = Q == 9 ? 0 : D3CZ[i];
                 const int mv = a.mask[c - cx - (int64_t)nx * (cy + (int64_t)ny * cz)];
                 if ((mv != 0) != (mval_c != 0)) {
@@ -442,7 +464,7 @@ size_t forces_workspace_bytes(const EnvArgs& a, int lattice)
 void cut_bits_view(int lattice, EnvArgs& a, void* ws)
 {
     const Region rg = fixup_region(a, lattice);
-    if (a.mask == nullptr || rg.n() == 0 || ws == nullptr) {
+    if (!has_bodies(a) || rg.n() == 0 || ws == nullptr) {
         a.cut_bits = nullptr;
         a.cut_cols = nullptr;
         return;
@@ -458,7 +480,7 @@ void cut_bits_view(int lattice, EnvArgs& a, void* ws)
 hipError_t cut_list(int lattice, const EnvArgs& a, void* ws, hipStream_t s)
 {
     const Region rg = fixup_region(a, lattice);
-    if (a.mask == nullptr || rg.n() == 0) return hipSuccess;
+    if (!has_bodies(a) || rg.n() == 0) return hipSuccess;
     if (rg.n() > (int64_t)1 << 30) return hipErrorInvalidValue;
     const FixupLayout L = fixup_layout(rg);
     char* base = static_cast<char*>(ws);
@@ -490,7 +512,7 @@ hipError_t fixup(int lattice, int dtype, const EnvArgs& a, bool store, double* o
     const int width = a.nsolid * 6;
     const bool want_forces = out != nullptr && width > 0;
     const Region rg = fixup_region(a, lattice);
-    if (a.mask == nullptr || rg.n() == 0) {
+    if (!has_bodies(a) || rg.n() == 0) {
         if (want_forces) {
             hipLaunchKernelGGL(k_zero, (width + 63) / 64, 64, 0, s, out, width);
             return hipGetLastError();
@@ -523,7 +545,7 @@ FixArgs fixup_args(int lattice, const EnvArgs& a, void* ws, bool forces, int par
     FixArgs f;
     std::memset(&f, 0, sizeof(f));
     const Region rg = fixup_region(a, lattice);
-    if (a.mask == nullptr || rg.n() == 0 || ws == nullptr) return f;
+    if (!has_bodies(a) || rg.n() == 0 || ws == nullptr) return f;
     const FixupLayout L = fixup_layout(rg);
     char* base = static_cast<char*>(ws);
     f.rg = rg;

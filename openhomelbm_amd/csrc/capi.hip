@@ -248,20 +248,49 @@ template <typename T> void uniform_moments(int lattice, T ux, T uy, T uz, double
 
 namespace {
 
+// Capsule bodies whose surface crossings cut links (hl_step_capsules).
+struct CapsuleBodies {
+    const float* caps;
+    int32_t ncap;
+    int32_t box[6];
+};
+
+// The step arguments of env, with capsule bodies in place of the mask.
+hl::EnvArgs body_args(const hl_env* env, int cur, const CapsuleBodies* cb)
+{
+    hl::EnvArgs a = make_args(env, cur);
+    if (cb) {
+        a.mask = nullptr;
+        a.caps = cb->caps;
+        a.ncap = cb->ncap;
+        for (int k = 0; k < 6; ++k) a.box[k] = cb->box[k];
+    }
+    return a;
+}
+
 // hl_step after its argument checks, with the reduction workspace from
-// get_ws(bytes, &ptr) and the near-body list cache that goes with it.
+// get_ws(bytes, &ptr) and the near-body list cache that goes with it. With
+// cb, the capsules' crossings cut links instead of the mask.
 template <typename GetWs>
 int32_t step_core(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history, hipStream_t s,
-                  ListCache& cache, GetWs get_ws)
+                  ListCache& cache, GetWs get_ws, const CapsuleBodies* cb = nullptr)
 {
     int32_t rc = HL_OK;
     // the near-body pass needs the cut-cell list when a mask is present and
-    // either the tiled kernel runs or forces are requested
+    // either the tiled kernel runs or forces are requested; capsule bodies
+    // always need it (no other kernel applies their wall rule)
     const bool tiled = kernel != HL_KERNEL_GATHER && hl::tiling_supported(env->lattice, 32, 8, 1) &&
                        env->dtype != HL_F64;
     const bool want_forces = force_history && env->nsolid > 0;
     void* ws = nullptr;
-    if (env->mask && nsteps > 0 && (tiled || want_forces)) {
+    if (cb && nsteps > 0) {
+        hl::EnvArgs a0 = body_args(env, cur, cb);
+        rc = get_ws(hl::forces_workspace_bytes(a0, env->lattice), &ws);
+        if (rc) return rc;
+        cache.valid = false;  // the capsules live in device memory: their list is rebuilt per call
+        hipError_t e = hl::cut_list(env->lattice, a0, ws, s);
+        if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
+    } else if (env->mask && nsteps > 0 && (tiled || want_forces)) {
         hl::EnvArgs a0 = make_args(env, cur);
         rc = get_ws(hl::forces_workspace_bytes(a0, env->lattice), &ws);
         if (rc) return rc;
@@ -297,7 +326,7 @@ int32_t step_core(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel
         return e;
     };
     for (int k = 0; k < nsteps; ++k) {
-        hl::EnvArgs a = make_args(env, cur);
+        hl::EnvArgs a = body_args(env, cur, cb);
         double* fout = want_forces ? force_history + (int64_t)k * env->nsolid * 6 : nullptr;
         hipError_t e = hipErrorNotSupported;
         bool fixed = false;
@@ -361,12 +390,17 @@ int32_t step_core(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel
         if (e == hipErrorNotSupported) {
             e = flush();
             if (e == hipSuccess) {
+                // capsule bodies: the gather kernel streams every cell and the
+                // near-body pass below rewrites the listed ones
+                hl::EnvArgs ag = a;
+                ag.caps = nullptr;
                 const int tk = timer_begin(s);
-                e = hl::step_gather(env->lattice, env->dtype, a, s);
+                e = hl::step_gather(env->lattice, env->dtype, ag, s);
                 timer_end(e == hipSuccess ? tk : -1, s);
             }
         }
-        if (e == hipSuccess && !fixed && fout) e = hl::fixup(env->lattice, env->dtype, a, false, fout, ws, s);
+        if (e == hipSuccess && !fixed && (fout || cb))
+            e = hl::fixup(env->lattice, env->dtype, a, cb != nullptr, fout, ws, s);
         if (e != hipSuccess) return hip_fail(e, "step kernel launch");
         cur ^= 1;
     }
@@ -457,6 +491,36 @@ int32_t hl_step_noalloc(const hl_env* env, int32_t cur, int32_t nsteps, int32_t 
                          *ws = d.ptr;
                          return HL_OK;
                      });
+}
+
+int32_t hl_step_capsules(const hl_env* env, const float* caps, int32_t ncap, const int32_t box[6], int32_t cur,
+                         int32_t nsteps, int32_t kernel, double* force_history, void* stream)
+{
+    int32_t rc = validate(env);
+    if (rc) return rc;
+    if (env->lattice != HL_D3Q27) return fail(HL_E_UNSUPPORTED, "capsule cut links are D3Q27 only");
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
+    if (ncap < 0 || ncap > HL_MAX_CAPSULES) return fail(HL_E_INVALID, "ncap must be in [0, %d], got %d", HL_MAX_CAPSULES, ncap);
+    if (ncap > 0 && !caps) return fail(HL_E_INVALID, "caps is NULL");
+    if (!box) return fail(HL_E_INVALID, "box is NULL");
+    const int32_t n[3] = {env->nx, env->ny, env->nz};
+    for (int k = 0; k < 3; ++k)
+        if (box[2 * k] < 0 || box[2 * k] > box[2 * k + 1] || box[2 * k + 1] > n[k])
+            return fail(HL_E_INVALID, "box axis %d [%d, %d) is not inside [0, %d]", k, box[2 * k], box[2 * k + 1], n[k]);
+    CapsuleBodies cb;
+    cb.caps = caps;
+    cb.ncap = ncap;
+    for (int k = 0; k < 6; ++k) cb.box[k] = box[k];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    return step_core(
+        env, cur, nsteps, kernel, force_history, s, ctx->list,
+        [&](size_t bytes, void** ws) { return workspace(ctx, s, bytes, ws); }, &cb);
 }
 
 int32_t hl_step_polygons(const hl_env* env, int32_t cur, int32_t nsteps, const float* verts,

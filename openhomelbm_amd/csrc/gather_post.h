@@ -113,12 +113,22 @@ __device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[
     d2q9_collide<T>(a, pop, Fx, Fy, out);
 }
 
+// The cut state of one link: the solid (-1: not cut) and the wall point.
+template <typename T>
+struct WallPoint {
+    int sid;
+    T h[3];
+};
+
 // Population i pulled into interior cell (x, y, z) (lbm_func_3d.py:414-526):
 // the reconstruction from the source cell's moments, clamped at 0, or on a
 // cut link the unclamped wall-rule population from the cell's moments mc.
+// A link is cut by the capsules' crossings when a.caps is set (the mask is
+// then not read), else by the mask rule when masked; wall (optional)
+// receives the link's cut state.
 template <typename T, int DT>
 __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, int i, const T* mc, bool masked, int mval_c,
-                                        const float* poses = nullptr)
+                                        const float* poses = nullptr, WallPoint<T>* wall = nullptr)
 {
     using S = Storage<DT>;
     using ST = typename S::type;
@@ -133,15 +143,30 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
 #pragma unroll
     for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + so, f, a.inv_scale[f]);
     int cut = 0, sid = -1;
-    if (masked) {
+    // the wall point: the link midpoint under the mask rule, the crossing
+    // x - c_i t (lbm_func_3d.py:446) with capsule bodies
+    T hx = T(x) - T(0.5) * dx, hy = T(y) - T(0.5) * dy, hz = T(z) - T(0.5) * dz;
+    if (a.caps) {
+        T t = T(0);
+        if (i > 0) sid = capsule_cut<T>(a.caps, a.ncap, a.nsolid, T(x), T(y), T(z), D3CX[i], D3CY[i], D3CZ[i], t);
+        if (sid >= 0) {
+            cut = 1;
+            hx = T(x) - dx * t;
+            hy = T(y) - dy * t;
+            hz = T(z) - dz * t;
+        }
+    } else if (masked) {
         const int mval_s = a.mask[s];
         if ((mval_s != 0) != (mval_c != 0)) { cut = 1; sid = mval_c ? mval_c - 1 : mval_s - 1; }
+    }
+    if (wall) {
+        wall->sid = cut ? sid : -1;
+        wall->h[0] = hx; wall->h[1] = hy; wall->h[2] = hz;
     }
     T pop_i;
     if (cut) {
         T up[3], r[3];
-        wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy,
-                            T(z) - T(0.5) * dz, up, r);
+        wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, hx, hy, hz, up, r);
         T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
         T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
         T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
@@ -287,6 +312,36 @@ __device__ __forceinline__ void d3q27_post(const EnvArgs& a, int x, int y, int z
     d3q27_collide<T>(a, pop, Fx, Fy, Fz, out);
 }
 
+// Momentum exchange of cut link i of a D3Q27 cell with moments mc, the wall
+// point h on solid sid (lbm_func_3d.py:478-511: bndForce, and bndTorque with
+// dv = h - solid position).
+template <typename T>
+__device__ __forceinline__ void link_force_at(const EnvArgs& a, const T* mc, int i, int sid, T hx, T hy, T hz, T F[6],
+                                              const float* poses)
+{
+    const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
+    T up[3], r[3];
+    wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, hx, hy, hz, up, r);
+    T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
+    T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
+    T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
+    T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
+    T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
+    T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
+    T pop = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, d3w<T>(i));
+    const int inv = D3INV[i];
+    const T ex = T(D3CX[inv]), ey = T(D3CY[inv]), ez = T(D3CZ[inv]);
+    T cur_f = d3q27_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[7], mc[8], mc[5], mc[9], mc[6], ex, ey, ez,
+                             d3w<T>(inv));
+    T Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
+    T Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
+    T Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
+    F[0] = Fx; F[1] = Fy; F[2] = Fz;
+    F[3] = r[1] * Fz - r[2] * Fy;
+    F[4] = r[2] * Fx - r[0] * Fz;
+    F[5] = r[0] * Fy - r[1] * Fx;
+}
+
 // Momentum exchange of one cut link (lbm_func.py:383-401 in 2D with the
 // torque scaled by grid_length = 1/nx; lbm_func_3d.py:491-511 in 3D).
 template <typename T, int DT, int Q>
@@ -310,27 +365,7 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
         F[5] = (r[0] * Fy - r[1] * Fx) * grid_length;
     } else {
         const T dx = T(D3CX[i]), dy = T(D3CY[i]), dz = T(D3CZ[i]);
-        T up[3], r[3];
-        wall_velocity_3d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy,
-                            T(z) - T(0.5) * dz, up, r);
-        T Sxx_s = up[0] * up[0] + mc[4] - mc[1] * mc[1];
-        T Sxy_s = up[0] * up[1] + mc[7] - mc[1] * mc[2];
-        T Sxz_s = up[0] * up[2] + mc[8] - mc[1] * mc[3];
-        T Syy_s = up[1] * up[1] + mc[5] - mc[2] * mc[2];
-        T Syz_s = up[1] * up[2] + mc[9] - mc[2] * mc[3];
-        T Szz_s = up[2] * up[2] + mc[6] - mc[3] * mc[3];
-        T pop = d3q27_recon<T>(mc[0], up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s, dx, dy, dz, d3w<T>(i));
-        const int inv = D3INV[i];
-        const T ex = T(D3CX[inv]), ey = T(D3CY[inv]), ez = T(D3CZ[inv]);
-        T cur_f = d3q27_recon<T>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[7], mc[8], mc[5], mc[9], mc[6], ex, ey, ez,
-                                 d3w<T>(inv));
-        T Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
-        T Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
-        T Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
-        F[0] = Fx; F[1] = Fy; F[2] = Fz;
-        F[3] = r[1] * Fz - r[2] * Fy;
-        F[4] = r[2] * Fx - r[0] * Fz;
-        F[5] = r[0] * Fy - r[1] * Fx;
+        link_force_at<T>(a, mc, i, sid, T(x) - T(0.5) * dx, T(y) - T(0.5) * dy, T(z) - T(0.5) * dz, F, poses);
     }
 }
 

@@ -324,6 +324,112 @@ __device__ __forceinline__ void wall_velocity_3d(const float* solids, int nsolid
     r[0] = d0; r[1] = d1; r[2] = d2;
 }
 
+// Correctly rounded square roots (HIP's default for FP32 and FP64).
+__device__ __forceinline__ float exact_sqrt(float v) { return __builtin_sqrtf(v); }
+__device__ __forceinline__ double exact_sqrt(double v) { return __builtin_sqrt(v); }
+
+// The crossing of the link from cell (x, y, z) towards its source,
+// P(t) = X - c t with t in [0, 1], nearest to the cell, over the surfaces of
+// the capsules caps[k] = (ax, ay, az, bx, by, bz, r, solid) with
+// 0 <= solid < nsolid: the ray query of get_cutcell_multi_3d
+// (lbm_func_3d.py:254-316: origin X, direction -c, nearest hit with
+// 0 <= t <= 1, the first solid winning ties) with each capsule surface in
+// closed form. Every capsule surface counts on its own, as every triangle of
+// the reference's concatenated body meshes does: a ray from inside a capsule
+// hits where it leaves it.
+//
+// Along the line a capsule is the hull [lo, hi] of the intervals inside its
+// end spheres and inside the cylinder between them (cut to the segment's
+// parameter range); the crossing nearest to X is lo when lo >= 0, else hi.
+// Capsules whose bounding sphere lies more than 2 > |c| from X are skipped.
+// Operation order and rounding follow the C restatement of the same query,
+// with contraction off. Returns the solid (-1: no crossing), t in t_hit.
+template <typename T>
+__device__ __forceinline__ int capsule_cut(const float* caps, int ncap, int nsolid, T x, T y, T z, int cx, int cy,
+                                           int cz, T& t_hit)
+{
+    const T dx = T(-cx), dy = T(-cy), dz = T(-cz);
+    const T dd = T(cx * cx + cy * cy + cz * cz);
+    const T big = T(1e30);
+    T best = T(-1);
+    int sid = -1;
+    for (int k = 0; k < ncap; ++k) {
+        const float* C = caps + 8 * k;
+        const int s = (int)C[7];
+        if (s < 0 || s >= nsolid) continue;
+        const T ax = C[0], ay = C[1], az = C[2], bx = C[3], by = C[4], bz = C[5], rad = C[6];
+        const T bax = bx - ax, bay = by - ay, baz = bz - az;
+        const T baba = bax * bax + bay * bay + baz * baz;
+        const T mx = x - (ax + bx) * T(0.5), my = y - (ay + by) * T(0.5), mz = z - (az + bz) * T(0.5);
+        const T reach = exact_sqrt(baba) * T(0.5) + rad + T(2);
+        if (mx * mx + my * my + mz * mz > reach * reach) continue;
+        const T rr = rad * rad;
+        T lo = big, hi = -big;
+        const T ox = x - ax, oy = y - ay, oz = z - az;
+        const T rdoa = dx * ox + dy * oy + dz * oz;
+        const T oaoa = ox * ox + oy * oy + oz * oz;
+        {  // end sphere at a
+            const T disc = rdoa * rdoa - dd * (oaoa - rr);
+            if (disc >= T(0)) {
+                const T q = exact_sqrt(disc);
+                const T t0 = (-rdoa - q) / dd, t1 = (-rdoa + q) / dd;
+                if (t0 < lo) lo = t0;
+                if (t1 > hi) hi = t1;
+            }
+        }
+        if (baba > T(0)) {
+            {  // end sphere at b
+                const T px = x - bx, py = y - by, pz = z - bz;
+                const T b = dx * px + dy * py + dz * pz;
+                const T disc = b * b - dd * (px * px + py * py + pz * pz - rr);
+                if (disc >= T(0)) {
+                    const T q = exact_sqrt(disc);
+                    const T t0 = (-b - q) / dd, t1 = (-b + q) / dd;
+                    if (t0 < lo) lo = t0;
+                    if (t1 > hi) hi = t1;
+                }
+            }
+            // cylinder: |P - a|^2 baba - ((P - a).ba)^2 <= r^2 baba with
+            // 0 <= (P - a).ba <= baba; a ray parallel to the axis meets it
+            // only between the end spheres
+            const T bard = bax * dx + bay * dy + baz * dz;
+            const T baoa = bax * ox + bay * oy + baz * oz;
+            const T qa = baba * dd - bard * bard;
+            if (qa > T(0)) {
+                const T qb = baba * rdoa - baoa * bard;
+                const T qc = baba * (oaoa - rr) - baoa * baoa;
+                const T disc = qb * qb - qa * qc;
+                if (disc >= T(0)) {
+                    const T q = exact_sqrt(disc);
+                    T c0 = (-qb - q) / qa, c1 = (-qb + q) / qa;
+                    bool ok;
+                    if (bard != T(0)) {
+                        T s0 = -baoa / bard, s1 = (baba - baoa) / bard;
+                        if (bard < T(0)) { const T sw = s0; s0 = s1; s1 = sw; }
+                        if (s0 > c0) c0 = s0;
+                        if (s1 < c1) c1 = s1;
+                        ok = c0 <= c1;
+                    } else {
+                        ok = baoa >= T(0) && baoa <= baba;
+                    }
+                    if (ok) {
+                        if (c0 < lo) lo = c0;
+                        if (c1 > hi) hi = c1;
+                    }
+                }
+            }
+        }
+        if (lo > hi) continue;
+        const T t = lo >= T(0) ? lo : hi;
+        if (t >= T(0) && t <= T(1) && (best < T(0) || t < best)) {
+            best = t;
+            sid = s;
+        }
+    }
+    t_hit = best;
+    return sid;
+}
+
 // Wavefront-wide FP64 sum and 64-bit OR (butterfly over the 64 lanes).
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -443,6 +549,9 @@ struct EnvArgs {
     // holds bit k for z = cz0 + 64 j + k
     const uint64_t* cut_cols;
     int32_t cut_colwords;
+    // capsule bodies whose crossings cut links (null: the mask rule): [ncap][8]
+    const float* caps;
+    int32_t ncap;
 };
 
 // Offset of cell (x, y, z) within a moment field: x + rp (y + ny z). The

@@ -330,22 +330,17 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
             py = min(max(y, 1), ny - 2);
             pz = min(max(z, 1), nz - 2);
             c = (int64_t)px + (int64_t)nx * ((int64_t)py + (int64_t)ny * pz);
-            mval_c = a.mask[c];
+            mval_c = a.caps ? 0 : a.mask[c];
         }
         if (cell && i < 27) {
             float mc[10];
             load_cell<float, DT>(a, cell_at(a, px, py, pz), 10, mc);
-            pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c, poses);
+            WallPoint<float> w{-1, {0.f, 0.f, 0.f}};
+            pops[wave][h][i] = d3q27_pull<float, DT>(a, px, py, pz, i, mc, true, mval_c, poses, &w);
             // force of this lane's link (interior cells only: a face cell repeats its projection)
-            if (width && i > 0 && px == x && py == y && pz == z) {
-                const int mv = a.mask[c - D3CX[i] - (int64_t)nx * (D3CY[i] + (int64_t)ny * D3CZ[i])];
-                if ((mv != 0) != (mval_c != 0)) {
-                    const int s2 = mval_c ? mval_c - 1 : mv - 1;
-                    if (s2 < a.nsolid) {
-                        sid = s2;
-                        link_force<float, DT, 27>(a, mc, x, y, z, i, sid, F, poses);
-                    }
-                }
+            if (width && px == x && py == y && pz == z && w.sid >= 0 && w.sid < a.nsolid) {
+                sid = w.sid;
+                link_force_at<float>(a, mc, i, sid, w.h[0], w.h[1], w.h[2], F, poses);
             }
         }
         wave_sync();

@@ -263,6 +263,30 @@ class Lattice:
         N.check(self.lib.hl_step(C.byref(e), self.cur, int(nsteps), int(kernel), fh, _stream_ptr(stream)), "hl_step")
         self.cur ^= nsteps & 1
 
+    def step_capsules(self, nsteps: int, caps: torch.Tensor, box: Sequence[int], kernel: int = N.HL_KERNEL_AUTO,
+                      force_history: Optional[torch.Tensor] = None, stream=None, _tuned: bool = True) -> None:
+        """Advance nsteps (D3Q27) cutting links where they cross the surface of
+        a capsule instead of by the mask, with the wall rule at the crossing
+        (the ray query of get_cutcell_multi_3d, envs/lbm3d/lbm_func_3d.py:254-316,
+        used by stream_and_collide_3d, lbm_func_3d.py:424-511): caps a device
+        FP32 [n, 8] table (ax, ay, az, bx, by, bz, r, solid_id) at this pose,
+        box a half-open cell box holding every capsule (capsule_box gives
+        one). The mask is not read."""
+        if self.lattice != 27:
+            raise ValueError("capsule cut links are D3Q27 only")
+        if caps.dtype != torch.float32 or caps.device != self.device or not caps.is_contiguous() or caps.shape[-1] != 8:
+            raise ValueError("caps must be a contiguous FP32 device tensor [n, 8]")
+        if len(box) != 6:
+            raise ValueError("box must be (x0, x1, y0, y1, z0, z1)")
+        if _tuned:
+            autotune.apply(self)
+        fh = self._history_ptr(force_history, nsteps)
+        e = self._step_env()
+        b = (C.c_int32 * 6)(*[int(v) for v in box])
+        N.check(self.lib.hl_step_capsules(C.byref(e), caps.data_ptr(), int(caps.numel() // 8), b, self.cur,
+                                          int(nsteps), int(kernel), fh, _stream_ptr(stream)), "hl_step_capsules")
+        self.cur ^= nsteps & 1
+
     def step_polygons(self, nsteps: int, verts: torch.Tensor, verts_last: Optional[torch.Tensor],
                       counts: torch.Tensor, radius2: torch.Tensor, force_history: Optional[torch.Tensor] = None,
                       stream=None) -> None:

@@ -11,7 +11,11 @@ all deliberate:
   (w, x, y, z). When a pose or shape changes they are voxelized into an
   integer mask (meshes by +x ray parity) and the wall rule runs on mask links
   (mask(x) != mask(x-c_i)) instead of BVH ray queries (get_cutcell_multi_3d,
-  lbm_func_3d.py:254-316).
+  lbm_func_3d.py:254-316). With HomeFlow3D.cut_cell = True (or
+  LBM_Solver3D(..., cut_cell=True)), capsule and sphere bodies take the
+  reference's own rule instead: a link is cut where the ray from the cell
+  along -c_i meets a capsule surface at t in [0, 1] (closed form in place of
+  the BVH), with u_p and the lever arm at the crossing x - c_i t.
 * solid_force / solid_torque come from a deterministic FP64 reduction.
 * step() is asynchronous; reading a field synchronizes.
 """
@@ -103,6 +107,13 @@ class HomeFlow3D:
         self._meshes: Dict[int, Tuple[np.ndarray, np.ndarray]] = {}  # body-frame meshes (lattice units)
         self._last: Optional[np.ndarray] = None  # pose table of the previous step
         self.capsules = np.zeros((0, 8), np.float32)  # world-frame capsules of the last voxelization
+        self._caps_dev: Optional[torch.Tensor] = None  # the same table on the device (capsule cut links)
+        self._caps_box = (0, 0, 0, 0, 0, 0)
+        # cut_cell: capsule and sphere bodies cut links where the link crosses
+        # their surface, with the wall rule at the crossing (the reference's
+        # ray query, lbm_func_3d.py:254-316, in closed form); False: the mask
+        # rule at link midpoints. Mesh bodies always take the mask rule.
+        self.cut_cell = False
         self._dirty = True
         self._forces = torch.zeros((1, n, 6), dtype=torch.float64, device=L.device)
 
@@ -270,6 +281,8 @@ class HomeFlow3D:
             R, p = now[k, 3:12].reshape(3, 3), now[k, 0:3]
             caps.append(list(R @ a + p) + list(R @ b + p) + [r, k])
         self.capsules = np.array(caps, np.float32).reshape(-1, 8)
+        self._caps_dev = torch.from_numpy(self.capsules).to(L.device) if caps else None
+        self._caps_box = capsule_box(self.capsules, (self.nx, self.ny, self.nz)) if caps else (0, 0, 0, 0, 0, 0)
         tris, tsid = [], []
         for k, (v, f) in sorted(self._meshes.items()):
             R, p = now[k, 3:12].reshape(3, 3), now[k, 0:3]
@@ -287,10 +300,21 @@ class HomeFlow3D:
             L.voxelize_mesh(np.concatenate(tris), np.concatenate(tsid), stream=stream)
         self._dirty = False
 
+    def _uses_capsule_cuts(self) -> bool:
+        if not self.cut_cell or self.n_objects == 0:
+            return False
+        if self._meshes:
+            raise ValueError("cut_cell needs every body to be a capsule or a sphere (mesh bodies take the mask "
+                             "rule: set cut_cell = False)")
+        return self._caps_dev is not None
+
     def step(self, stream=None) -> None:
         self._prepare(stream)
-        self.lattice.step(1, N.HL_KERNEL_AUTO, force_history=self._forces if self.n_objects > 0 else None,
-                          stream=stream)
+        fh = self._forces if self.n_objects > 0 else None
+        if self._uses_capsule_cuts():
+            self.lattice.step_capsules(1, self._caps_dev, self._caps_box, force_history=fh, stream=stream)
+        else:
+            self.lattice.step(1, N.HL_KERNEL_AUTO, force_history=fh, stream=stream)
 
     def reset(self) -> None:
         """ResetSingleWorldFlow3D (lbm_func_3d.py:87-140): rho = 1, u = 0, S = 0
@@ -305,7 +329,7 @@ class LBM_Solver3D:
     """Worlds of one size stepped together (envs/lbm3d/lbm_solver_3d.py:19-420)."""
 
     def __init__(self, nx: int, ny: int, nz: int, solid_num: int = 1, nworld: int = 1,
-                 device: Optional[Any] = None, dtype: str = "f32") -> None:
+                 device: Optional[Any] = None, dtype: str = "f32", cut_cell: bool = False) -> None:
         if nworld < 1:
             raise ValueError("nworld must be >= 1")
         self.nx, self.ny, self.nz = nx, ny, nz
@@ -317,6 +341,7 @@ class LBM_Solver3D:
         self.flows = [HomeFlow3D(dtype=dtype, device=self.device) for _ in range(nworld)]
         for flow in self.flows:
             flow.Initialize(nx, ny, nz, n_objects=solid_num)
+            flow.cut_cell = cut_cell
         self.flows_wp = self.flows
         self.meshes = [None] * solid_num
         self.mesh_wps = [None] * solid_num

@@ -45,15 +45,19 @@ static const double third_f64 = 1.0 / 3.0;
 
 #define R float
 #define SFX f32
+#define RSQRT sqrtf
 #include "oracle_impl.h"
 #undef R
 #undef SFX
+#undef RSQRT
 
 #define R double
 #define SFX f64
+#define RSQRT sqrt
 #include "oracle_impl.h"
 #undef R
 #undef SFX
+#undef RSQRT
 
 void hlo_step_omp_f32(const hlo_env* e, const float* src, float* dst)
 {
@@ -79,6 +83,25 @@ void hlo_step_omp_f32(const hlo_env* e, const float* src, float* dst)
                     for (int f = 0; f < 10; ++f) dst[f * st + c] = out[f];
                 }
     }
+    bc_passes_f32(e, dst);
+}
+
+/* hlo_step_capsules_f32 without forces, parallel over cells with OpenMP
+ * (identical results). */
+void hlo_step_capsules_omp_f32(const hlo_env* e, const float* caps, int32_t ncap, const float* src, float* dst)
+{
+    const int64_t st = e->stride;
+    if (e->lattice != 27) return;
+    const int ny = e->ny, nz = e->nz;
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int z = 1; z < nz - 1; ++z)
+        for (int y = 1; y < ny - 1; ++y)
+            for (int x = 1; x < e->nx - 1; ++x) {
+                float out[10];
+                d3q27_cell_rule_f32(e, src, x, y, z, caps, ncap, NULL, out);
+                int64_t c = (int64_t)x + (int64_t)e->nx * ((int64_t)y + (int64_t)e->ny * z);
+                for (int f = 0; f < 10; ++f) dst[f * st + c] = out[f];
+            }
     bc_passes_f32(e, dst);
 }
 

@@ -96,6 +96,24 @@ typedef struct hlo_polys {
 void hlo_step_polygons_f32(const hlo_env* e, const hlo_polys* p, const float* src, float* dst, double* forces);
 void hlo_step_polygons_f64(const hlo_env* e, const hlo_polys* p, const double* src, double* dst, double* forces);
 
+/* Capsule cut links (3D). caps: [ncap][8] = ax, ay, az, bx, by, bz, radius,
+ * solid_id in lattice coordinates at the current pose (the voxelizer's
+ * table). hlo_capsule_cut: the crossing of link i from (x, y, z) towards
+ * x - c_i nearest to the cell, over every capsule surface (the ray query of
+ * get_cutcell_multi_3d, lbm_func_3d.py:254-316): returns the solid (-1: none)
+ * and t in [0, 1]. hlo_step_capsules: one D3Q27 step cutting those links,
+ * the wall point at the crossing; forces (nullable, [nsolid][6] FP64) of the
+ * state read. env.mask is not read. */
+int32_t hlo_capsule_cut_f32(const hlo_env* e, const float* caps, int32_t ncap, float x, float y, float z, int32_t i,
+                            float* t_hit);
+int32_t hlo_capsule_cut_f64(const hlo_env* e, const float* caps, int32_t ncap, double x, double y, double z, int32_t i,
+                            double* t_hit);
+void hlo_step_capsules_f32(const hlo_env* e, const float* caps, int32_t ncap, const float* src, float* dst,
+                           double* forces);
+void hlo_step_capsules_f64(const hlo_env* e, const float* caps, int32_t ncap, const double* src, double* dst,
+                           double* forces);
+void hlo_step_capsules_omp_f32(const hlo_env* e, const float* caps, int32_t ncap, const float* src, float* dst);
+
 /* FP64 sum of rho over all cells. */
 double hlo_mass_f32(const hlo_env* e, const float* src);
 

@@ -76,6 +76,17 @@ def lib():
         for name in ("hlo_step_polygons_f32", "hlo_step_polygons_f64"):
             getattr(L, name).argtypes = [C.POINTER(_Env), C.POINTER(_Polys), P, P, P]
             getattr(L, name).restype = None
+        L.hlo_capsule_cut_f32.argtypes = [C.POINTER(_Env), P, C.c_int32, C.c_float, C.c_float, C.c_float,
+                                          C.c_int32, C.POINTER(C.c_float)]
+        L.hlo_capsule_cut_f32.restype = C.c_int32
+        L.hlo_capsule_cut_f64.argtypes = [C.POINTER(_Env), P, C.c_int32, C.c_double, C.c_double, C.c_double,
+                                          C.c_int32, C.POINTER(C.c_double)]
+        L.hlo_capsule_cut_f64.restype = C.c_int32
+        for name in ("hlo_step_capsules_f32", "hlo_step_capsules_f64"):
+            getattr(L, name).argtypes = [C.POINTER(_Env), P, C.c_int32, P, P, P]
+            getattr(L, name).restype = None
+        L.hlo_step_capsules_omp_f32.argtypes = [C.POINTER(_Env), P, C.c_int32, P, P]
+        L.hlo_step_capsules_omp_f32.restype = None
         L.hlo_mass_f32.argtypes = [C.POINTER(_Env), P]
         L.hlo_mass_f32.restype = C.c_double
         _lib = L
@@ -211,6 +222,48 @@ def step_polygons(env: Env, src: np.ndarray, verts: np.ndarray, verts_last: np.n
     out = np.zeros((max(e.nsolid, 1), 6), dtype=np.float64)
     fn = lib().hlo_step_polygons_f32 if precision == "f32" else lib().hlo_step_polygons_f64
     fn(C.byref(e), C.byref(pol), src.ctypes.data, dst.ctypes.data, out.ctypes.data)
+    return dst, out[: e.nsolid]
+
+
+def _caps(capsules) -> np.ndarray:
+    return np.ascontiguousarray(capsules, dtype=np.float32).reshape(-1, 8)
+
+
+def capsule_cut(env: Env, capsules: np.ndarray, cell: Sequence[float], i: int, precision: str = "f32"):
+    """(solid, t) of the crossing of link i from cell towards cell - c_i
+    nearest to the cell over every capsule surface (solid -1: none)."""
+    caps = _caps(capsules)
+    e = env.c_env()
+    if precision == "f32":
+        t = C.c_float(0)
+        sid = lib().hlo_capsule_cut_f32(C.byref(e), caps.ctypes.data, caps.shape[0], *[float(v) for v in cell],
+                                        int(i), C.byref(t))
+    else:
+        t = C.c_double(0)
+        sid = lib().hlo_capsule_cut_f64(C.byref(e), caps.ctypes.data, caps.shape[0], *[float(v) for v in cell],
+                                        int(i), C.byref(t))
+    return int(sid), float(t.value)
+
+
+def step_capsules(env: Env, src: np.ndarray, capsules: np.ndarray, precision: str = "f32", omp: bool = False):
+    """One D3Q27 step whose cut links are the capsules' crossings (the ray
+    query of get_cutcell_multi_3d with closed-form capsule surfaces) and whose
+    wall point is the crossing (lbm_func_3d.py:424-511). Returns (state,
+    forces [nsolid, 6] of the state read; None with omp). env.mask is
+    ignored."""
+    dtype = np.float32 if precision == "f32" else np.float64
+    src = _check(env, src, dtype)
+    dst = np.empty_like(src)
+    caps = _caps(capsules)
+    e = env.c_env()
+    if omp:
+        assert precision == "f32"
+        lib().hlo_step_capsules_omp_f32(C.byref(e), caps.ctypes.data, caps.shape[0], src.ctypes.data,
+                                        dst.ctypes.data)
+        return dst, None
+    out = np.zeros((max(e.nsolid, 1), 6), dtype=np.float64)
+    fn = lib().hlo_step_capsules_f32 if precision == "f32" else lib().hlo_step_capsules_f64
+    fn(C.byref(e), caps.ctypes.data, caps.shape[0], src.ctypes.data, dst.ctypes.data, out.ctypes.data)
     return dst, out[: e.nsolid]
 
 

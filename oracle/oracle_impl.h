@@ -10,6 +10,8 @@
 static void FN(wall_velocity_2d)(const hlo_env* e, int sid, R hx, R hy, R up[2], R r[2]);
 static void FN(wall_velocity_3d)(const hlo_env* e, int sid, R hx, R hy, R hz, R up[3], R r[3]);
 static void FN(d3q27_feq_S)(R rho, R ux, R uy, R uz, R S[6]);
+static int FN(capsule_cut)(const hlo_env* e, const float* caps, int ncap, R x, R y, R z, int cx, int cy, int cz, R* t_hit);
+static void FN(link_force_3d)(const hlo_env* e, int sid, int i, R hx, R hy, R hz, const R m[10], double o[6]);
 
 /* ------------------------------------------------------------------ D2Q9 */
 
@@ -239,7 +241,8 @@ static void FN(d3q27_feq_S)(R rho, R ux, R uy, R uz, R S[6])
  * midpoint; the cut branch is not clamped (lbm_func_3d.py:487-489), the plain
  * branch is (525-526); moments are accumulated by sign group in ascending i
  * (528-562). */
-static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, R out[10])
+static void FN(d3q27_cell_rule)(const hlo_env* e, const R* src, int x, int y, int z, const float* caps, int ncap,
+                                double* forces, R out[10])
 {
     const int64_t st = e->stride;
     const int nx = e->nx, ny = e->ny;
@@ -255,7 +258,19 @@ static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, 
         int x1 = x - D3_CX[i], y1 = y - D3_CY[i], z1 = z - D3_CZ[i];
         int64_t s = (int64_t)x1 + (int64_t)nx * ((int64_t)y1 + (int64_t)ny * z1);
         int cut = 0, sid = -1;
-        if (e->mask) {
+        /* the wall point: the link midpoint under the mask rule, the hit
+         * point x - c_i t (lbm_func_3d.py:446) with capsules */
+        R hx = (R)x - (R)0.5 * dx, hy = (R)y - (R)0.5 * dy, hz = (R)z - (R)0.5 * dz;
+        if (caps) {
+            R t = 0;
+            if (i > 0) sid = FN(capsule_cut)(e, caps, ncap, (R)x, (R)y, (R)z, D3_CX[i], D3_CY[i], D3_CZ[i], &t);
+            if (sid >= 0) {
+                cut = 1;
+                hx = (R)x - dx * t;
+                hy = (R)y - dy * t;
+                hz = (R)z - dz * t;
+            }
+        } else if (e->mask) {
             int occ_s = e->mask[s] != 0;
             if (occ_s != occ_c) {
                 cut = 1;
@@ -265,7 +280,7 @@ static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, 
         R pop_i;
         if (cut) {
             R up[3];
-            FN(wall_velocity_3d)(e, sid, (R)x - (R)0.5 * dx, (R)y - (R)0.5 * dy, (R)z - (R)0.5 * dz, up, NULL);
+            FN(wall_velocity_3d)(e, sid, hx, hy, hz, up, NULL);
             R Sxx_s = up[0] * up[0] + Sxx_c - ux_c * ux_c;
             R Sxy_s = up[0] * up[1] + Sxy_c - ux_c * uy_c;
             R Sxz_s = up[0] * up[2] + Sxz_c - ux_c * uz_c;
@@ -274,6 +289,10 @@ static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, 
             R Szz_s = up[2] * up[2] + Szz_c - uz_c * uz_c;
             pop_i = FN(d3q27_recon_s)(rho_c, up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s,
                                       dx, dy, dz, wi);
+            if (forces && sid < e->nsolid) {
+                const R mc[10] = {rho_c, ux_c, uy_c, uz_c, Sxx_c, Syy_c, Szz_c, Sxy_c, Sxz_c, Syz_c};
+                FN(link_force_3d)(e, sid, i, hx, hy, hz, mc, forces + 6 * sid);
+            }
         } else {
             pop_i = FN(d3q27_recon_s)(src[s], src[st + s], src[2 * st + s], src[3 * st + s],
                                       src[4 * st + s], src[7 * st + s], src[8 * st + s],
@@ -328,6 +347,11 @@ static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, 
     out[9] = piyz_n * invRho;
 }
 
+static void FN(d3q27_cell)(const hlo_env* e, const R* src, int x, int y, int z, R out[10])
+{
+    FN(d3q27_cell_rule)(e, src, x, y, z, NULL, 0, NULL, out);
+}
+
 /* --------------------------------------------------------- wall velocity */
 
 /* Velocity of the body material point at h (time step 1): the point's
@@ -378,6 +402,172 @@ static void FN(wall_velocity_3d)(const hlo_env* e, int sid, R hx, R hy, R hz, R 
     R l2 = (R)ML[6] * q0 + (R)ML[7] * q1 + (R)ML[8] * q2 + (R)L[2];
     up[0] = c0 - l0; up[1] = c1 - l1; up[2] = c2 - l2;
     if (r) { r[0] = d0; r[1] = d1; r[2] = d2; }
+}
+
+/* Momentum exchange of cut link i of a cell with moments m (rho, u, Sxx,
+ * Syy, Szz, Sxy, Sxz, Syz), the wall point h on solid sid, added to o
+ * (lbm_func_3d.py:478-511: bndForce and bndTorque, dv = h - solid position). */
+static void FN(link_force_3d)(const hlo_env* e, int sid, int i, R hx, R hy, R hz, const R m[10], double o[6])
+{
+    R rho = m[0], ux = m[1], uy = m[2], uz = m[3];
+    R Sxx = m[4], Syy = m[5], Szz = m[6], Sxy = m[7], Sxz = m[8], Syz = m[9];
+    R dx = (R)D3_CX[i], dy = (R)D3_CY[i], dz = (R)D3_CZ[i];
+    R up[3], r[3];
+    FN(wall_velocity_3d)(e, sid, hx, hy, hz, up, r);
+    R Sxx_s = up[0] * up[0] + Sxx - ux * ux;
+    R Sxy_s = up[0] * up[1] + Sxy - ux * uy;
+    R Sxz_s = up[0] * up[2] + Sxz - ux * uz;
+    R Syy_s = up[1] * up[1] + Syy - uy * uy;
+    R Syz_s = up[1] * up[2] + Syz - uy * uz;
+    R Szz_s = up[2] * up[2] + Szz - uz * uz;
+    R pop = FN(d3q27_recon_s)(rho, up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s,
+                              dx, dy, dz, FN(d3_w)[i]);
+    int inv = D3_INV[i];
+    R ex = (R)D3_CX[inv], ey = (R)D3_CY[inv], ez = (R)D3_CZ[inv];
+    R cur_f = FN(d3q27_recon_s)(rho, ux, uy, uz, Sxx, Sxy, Sxz, Syy, Syz, Szz, ex, ey, ez, FN(d3_w)[inv]);
+    R Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
+    R Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
+    R Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
+    R Tx = r[1] * Fz - r[2] * Fy;
+    R Ty = r[2] * Fx - r[0] * Fz;
+    R Tz = r[0] * Fy - r[1] * Fx;
+    o[0] += (double)Fx; o[1] += (double)Fy; o[2] += (double)Fz;
+    o[3] += (double)Tx; o[4] += (double)Ty; o[5] += (double)Tz;
+}
+
+/* ------------------------------------------------- capsule cut links (3D) */
+
+/* The crossing of the link from cell X towards its source, P(t) = X - c t
+ * with t in [0, 1], nearest to X, over the surfaces of capsules
+ * (ax, ay, az, bx, by, bz, r, solid) with 0 <= solid < nsolid. This is the
+ * ray query of get_cutcell_multi_3d (lbm_func_3d.py:254-316: origin X,
+ * direction -c, the nearest hit with 0 <= t <= 1 over every solid, the first
+ * solid winning ties) with each capsule's surface in closed form instead of
+ * a triangle mesh. As with a mesh, every surface counts on its own: a ray
+ * from inside a capsule hits where it leaves it, and the surfaces of
+ * overlapping capsules all count (the reference concatenates a body's geom
+ * meshes, mjcf_parser.py:133).
+ *
+ * Along the line, a capsule is the interval where P(t) lies in the end
+ * sphere at a, the end sphere at b or the cylinder between them (the roots of
+ * each quadratic, the cylinder's cut to the segment's parameter range); the
+ * capsule being convex, the union is the hull [lo, hi]. The crossing nearest
+ * to X is lo when lo >= 0 (X outside), otherwise hi (X inside). A capsule
+ * whose bounding sphere (centre (a+b)/2, radius |b-a|/2 + r) lies more than
+ * 2 > |c| from X is skipped (no crossing can reach it). Returns the solid id
+ * (-1: no crossing) and t in *t_hit. */
+static int FN(capsule_cut)(const hlo_env* e, const float* caps, int ncap, R x, R y, R z, int cx, int cy, int cz, R* t_hit)
+{
+    const R dx = (R)-cx, dy = (R)-cy, dz = (R)-cz;
+    const R dd = (R)(cx * cx + cy * cy + cz * cz);
+    const R big = (R)1e30;
+    R best = -1;
+    int sid = -1;
+    for (int k = 0; k < ncap; ++k) {
+        const float* C = caps + 8 * k;
+        const int s = (int)C[7];
+        if (s < 0 || s >= e->nsolid) continue;
+        const R ax = C[0], ay = C[1], az = C[2], bx = C[3], by = C[4], bz = C[5], rad = C[6];
+        const R bax = bx - ax, bay = by - ay, baz = bz - az;
+        const R baba = bax * bax + bay * bay + baz * baz;
+        const R mx = x - (ax + bx) * (R)0.5, my = y - (ay + by) * (R)0.5, mz = z - (az + bz) * (R)0.5;
+        const R reach = RSQRT(baba) * (R)0.5 + rad + (R)2;
+        if (mx * mx + my * my + mz * mz > reach * reach) continue;
+        const R rr = rad * rad;
+        R lo = big, hi = -big;
+        /* end sphere at a */
+        const R ox = x - ax, oy = y - ay, oz = z - az;
+        const R rdoa = dx * ox + dy * oy + dz * oz;
+        const R oaoa = ox * ox + oy * oy + oz * oz;
+        {
+            const R disc = rdoa * rdoa - dd * (oaoa - rr);
+            if (disc >= (R)0) {
+                const R q = RSQRT(disc);
+                const R t0 = (-rdoa - q) / dd, t1 = (-rdoa + q) / dd;
+                if (t0 < lo) lo = t0;
+                if (t1 > hi) hi = t1;
+            }
+        }
+        if (baba > (R)0) {
+            /* end sphere at b */
+            const R px = x - bx, py = y - by, pz = z - bz;
+            const R b = dx * px + dy * py + dz * pz;
+            const R disc = b * b - dd * (px * px + py * py + pz * pz - rr);
+            if (disc >= (R)0) {
+                const R q = RSQRT(disc);
+                const R t0 = (-b - q) / dd, t1 = (-b + q) / dd;
+                if (t0 < lo) lo = t0;
+                if (t1 > hi) hi = t1;
+            }
+            /* cylinder: |P - a|^2 baba - ((P - a).ba)^2 <= r^2 baba with
+             * 0 <= (P - a).ba <= baba; a ray parallel to the axis meets the
+             * cylinder only between the two end spheres */
+            const R bard = bax * dx + bay * dy + baz * dz;
+            const R baoa = bax * ox + bay * oy + baz * oz;
+            const R qa = baba * dd - bard * bard;
+            if (qa > (R)0) {
+                const R qb = baba * rdoa - baoa * bard;
+                const R qc = baba * (oaoa - rr) - baoa * baoa;
+                const R disc = qb * qb - qa * qc;
+                if (disc >= (R)0) {
+                    const R q = RSQRT(disc);
+                    R c0 = (-qb - q) / qa, c1 = (-qb + q) / qa;
+                    int ok = 1;
+                    if (bard != (R)0) {
+                        R s0 = -baoa / bard, s1 = (baba - baoa) / bard;
+                        if (bard < (R)0) { const R sw = s0; s0 = s1; s1 = sw; }
+                        if (s0 > c0) c0 = s0;
+                        if (s1 < c1) c1 = s1;
+                        ok = c0 <= c1;
+                    } else {
+                        ok = baoa >= (R)0 && baoa <= baba;
+                    }
+                    if (ok) {
+                        if (c0 < lo) lo = c0;
+                        if (c1 > hi) hi = c1;
+                    }
+                }
+            }
+        }
+        if (lo > hi) continue;
+        const R t = lo >= (R)0 ? lo : hi;
+        if (t >= (R)0 && t <= (R)1 && (best < (R)0 || t < best)) {
+            best = t;
+            sid = s;
+        }
+    }
+    *t_hit = best;
+    return sid;
+}
+
+int FN(hlo_capsule_cut)(const hlo_env* e, const float* caps, int32_t ncap, R x, R y, R z, int32_t i, R* t_hit)
+{
+    *t_hit = -1;
+    if (e->lattice != 27 || i <= 0 || i >= 27) return -1;
+    return FN(capsule_cut)(e, caps, ncap, x, y, z, D3_CX[i], D3_CY[i], D3_CZ[i], t_hit);
+}
+
+/* One D3Q27 step with the capsules' cut links: a link is cut when
+ * capsule_cut finds a crossing, and its wall point is the crossing
+ * x - c_i t (lbm_func_3d.py:424-511); every interior cell, inside a body or
+ * not, takes the update (the reference flags no cell solid in 3D). Forces
+ * (nullable, [nsolid][6] FP64) of the state read, summed in cell order
+ * (z, y, x) and ascending direction. env.mask is not read. */
+void FN(hlo_step_capsules)(const hlo_env* e, const float* caps, int32_t ncap, const R* src, R* dst, double* forces)
+{
+    const int64_t st = e->stride;
+    if (forces)
+        for (int k = 0; k < 6 * e->nsolid; ++k) forces[k] = 0.0;
+    if (e->lattice != 27) return;
+    for (int z = 1; z < e->nz - 1; ++z)
+        for (int y = 1; y < e->ny - 1; ++y)
+            for (int x = 1; x < e->nx - 1; ++x) {
+                R out[10];
+                FN(d3q27_cell_rule)(e, src, x, y, z, caps, ncap, forces, out);
+                int64_t c = (int64_t)x + (int64_t)e->nx * ((int64_t)y + (int64_t)e->ny * z);
+                for (int f = 0; f < 10; ++f) dst[f * st + c] = out[f];
+            }
+    FN(bc_passes)(e, dst);
 }
 
 /* ------------------------------------------------------------ public API */
@@ -450,32 +640,11 @@ void FN(hlo_forces)(const hlo_env* e, const R* src, double* out)
                         R Tz = (r[0] * Fy - r[1] * Fx) * grid_length;
                         o[0] += (double)Fx; o[1] += (double)Fy; o[5] += (double)Tz;
                     } else {
-                        R rho = src[c], ux = src[st + c], uy = src[2 * st + c], uz = src[3 * st + c];
-                        R Sxx = src[4 * st + c], Syy = src[5 * st + c], Szz = src[6 * st + c];
-                        R Sxy = src[7 * st + c], Sxz = src[8 * st + c], Syz = src[9 * st + c];
-                        R dx = (R)cx, dy = (R)cy, dz = (R)cz;
-                        R up[3], r[3];
-                        FN(wall_velocity_3d)(e, sid, (R)x - (R)0.5 * dx, (R)y - (R)0.5 * dy, (R)z - (R)0.5 * dz, up, r);
-                        R Sxx_s = up[0] * up[0] + Sxx - ux * ux;
-                        R Sxy_s = up[0] * up[1] + Sxy - ux * uy;
-                        R Sxz_s = up[0] * up[2] + Sxz - ux * uz;
-                        R Syy_s = up[1] * up[1] + Syy - uy * uy;
-                        R Syz_s = up[1] * up[2] + Syz - uy * uz;
-                        R Szz_s = up[2] * up[2] + Szz - uz * uz;
-                        R pop = FN(d3q27_recon_s)(rho, up[0], up[1], up[2], Sxx_s, Sxy_s, Sxz_s, Syy_s, Syz_s, Szz_s,
-                                                  dx, dy, dz, FN(d3_w)[i]);
-                        int inv = D3_INV[i];
-                        R ex = (R)D3_CX[inv], ey = (R)D3_CY[inv], ez = (R)D3_CZ[inv];
-                        R cur_f = FN(d3q27_recon_s)(rho, ux, uy, uz, Sxx, Sxy, Sxz, Syy, Syz, Szz, ex, ey, ez,
-                                                    FN(d3_w)[inv]);
-                        R Fx = cur_f * (ex - up[0]) - pop * (dx - up[0]);
-                        R Fy = cur_f * (ey - up[1]) - pop * (dy - up[1]);
-                        R Fz = cur_f * (ez - up[2]) - pop * (dz - up[2]);
-                        R Tx = r[1] * Fz - r[2] * Fy;
-                        R Ty = r[2] * Fx - r[0] * Fz;
-                        R Tz = r[0] * Fy - r[1] * Fx;
-                        o[0] += (double)Fx; o[1] += (double)Fy; o[2] += (double)Fz;
-                        o[3] += (double)Tx; o[4] += (double)Ty; o[5] += (double)Tz;
+                        const R m[10] = {src[c], src[st + c], src[2 * st + c], src[3 * st + c], src[4 * st + c],
+                                         src[5 * st + c], src[6 * st + c], src[7 * st + c], src[8 * st + c],
+                                         src[9 * st + c]};
+                        FN(link_force_3d)(e, sid, i, (R)x - (R)0.5 * (R)cx, (R)y - (R)0.5 * (R)cy,
+                                          (R)z - (R)0.5 * (R)cz, m, o);
                     }
                 }
             }
