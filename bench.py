@@ -45,6 +45,9 @@ def parse():
                    help="steps of the two-way coupled sphere reported beside the D3Q27 line (0 = skip)")
     p.add_argument("--precision-steps", type=int, default=5000,
                    help="steps of the FP64/FP32/FP16/BF16 eel sweep reported beside the D3Q27 line (0 = skip)")
+    p.add_argument("--sphere-steps", type=int, default=4000,
+                   help="steps of the sphere drag under the capsule-crossing and mask rules reported beside the "
+                        "D3Q27 line (0 = skip)")
     p.add_argument("--dtype", default="f32", choices=["f32", "f16", "bf16", "f64"])
     p.add_argument("--kernel", default="auto", choices=["auto", "gather", "tiled"])
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="bound on the CPU-baseline sample")
@@ -133,6 +136,27 @@ def cpu_baseline(w, seconds: float):
             break
     return {"value": round(w.ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
             "sample": f"{steps} FP32 steps of the full {w.name} lattice, OpenMP oracle, {el:.1f} s"}
+
+
+def bench_capsule_speed(lat, w, K: int, W: int = 20):
+    """Per-step time of the sphere with its capsule-crossing cut links
+    (hl_step_capsules, forces recorded) on the lattice of the main line."""
+    import torch
+
+    from openhomelbm_amd.lattice import capsule_box
+
+    caps = torch.from_numpy(w.capsules).to(lat.device)
+    box = capsule_box(w.capsules, (w.nx, w.ny, w.nz))
+    hist = torch.zeros((max(K, W), lat.nsolid, 6), dtype=torch.float64, device=lat.device)
+    lat.step_capsules(W, caps, box, force_history=hist)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    lat.step_capsules(K, caps, box, force_history=hist)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / K
+    return {"steps": K, "ms_per_step": round(ms, 5), "mlups": round(w.ncell / (ms * 1e-3) / 1e6, 1)}
 
 
 def bench_karman_speed(K: int = 2000, W: int = 50):
@@ -435,6 +459,19 @@ def main():
     log(f"[bench] device copy {copy_gbs:.0f} GB/s (torch, 2 GiB), state streaming copy {stream_gbs:.0f} GB/s "
         f"(read + write)")
 
+    sphere_b = None
+    if args.sphere_steps > 0 and args.workload == "d3q27_sphere" and args.dtype == "f32":
+        from openhomelbm_amd import sphere
+
+        t0s = time.perf_counter()
+        sphere_b = sphere.compare_boundaries(steps=args.sphere_steps, window=max(args.sphere_steps // 4, 1), log=log)
+        sphere_b["capsules_speed"] = bench_capsule_speed(lat, w, K)
+        from openhomelbm_amd import precision
+
+        sphere_b["eel"] = precision.compare_boundaries(steps=args.sphere_steps, log=log)
+        sphere_b["seconds"] = round(time.perf_counter() - t0s, 1)
+        log(f"[bench] sphere boundaries {sphere_b}")
+
     batched = None
     if args.batched_steps > 0 and args.workload == "d3q27_sphere":
         batched = bench_batched(args.batched_steps, 20, min(args.batched_steps, 100))
@@ -499,7 +536,7 @@ def main():
                    "grid": [w.nx, w.ny, w.nz], "nu": w.nu, "u_in": w.u_in[0], "kernel": args.kernel,
                    "step": "stream/collide/boundary kernel + FP64 per-solid force reduction",
                    "bytes_per_cell_update": b_cell, "batched_d2q9": batched, "serving": serve, "karman": karm,
-                   "precision": prec, "coupled": coupled},
+                   "precision": prec, "coupled": coupled, "sphere_boundaries": sphere_b},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": PROFILE if traffic else None,

@@ -18,9 +18,16 @@ from . import inputs
 from .lattice import Lattice
 
 
-def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float = 500.0, log=None):
+def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float = 500.0, log=None,
+        boundary: str = "mask", history: bool = False):
     """Advance the eel workload `steps` steps in `dtype`; returns (final u as
-    an FP64 device tensor [3, nz, ny, nx], drag history [steps] FP64)."""
+    an FP64 device tensor [3, nz, ny, nx], drag history [steps] FP64, and
+    with history=True the full force history [steps, nseg, 6]). boundary
+    "mask": the capsules voxelized every step, the mask rule; "capsules":
+    links cut where they cross a capsule surface, the wall point at the
+    crossing (Lattice.step_capsules)."""
+    if boundary not in ("mask", "capsules"):
+        raise ValueError("boundary must be 'mask' or 'capsules'")
     nseg = eel.nseg
     L = Lattice(27, eel.nx, eel.ny, eel.nz, dtype=dtype, nsolid=nseg)
     L.nu = eel.nu(re)
@@ -43,9 +50,12 @@ def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float
     poses_d = torch.from_numpy(poses).to(L.device)
     hist = torch.zeros((steps, nseg, 6), dtype=torch.float64, device=L.device)
     for t in range(steps):
-        L.voxelize_device(caps_d[t], box)
         L.solids[:nseg].copy_(poses_d[t])
-        L.step(1, force_history=hist[t:t + 1])
+        if boundary == "capsules":
+            L.step_capsules(1, caps_d[t], eel.box(t, 1), force_history=hist[t:t + 1])
+        else:
+            L.voxelize_device(caps_d[t], box)
+            L.step(1, force_history=hist[t:t + 1])
         if log and (t + 1) % 1000 == 0:
             log(f"[precision] {dtype}: step {t + 1}/{steps}")
     u = L.fields()[1:4].to(torch.float64)
@@ -54,7 +64,33 @@ def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float
     drag = hist[:, :, 0].sum(1)
     if not bool(torch.isfinite(u).all()):
         raise RuntimeError(f"{dtype} run diverged")
+    if history:
+        return u, drag, hist.cpu().numpy()
     return u, drag
+
+
+def compare_boundaries(steps: int = 4000, eel=None, log=None) -> Dict:
+    """The swimming eel in FP32 with capsule-crossing cut links and with the
+    mask rule, from the same inputs: mean drag (sum of Fx over the chain),
+    amplitude of the lateral force (sum of Fy) and of the yaw torque (sum of
+    Tz), over the second half, and the mask rule's relative differences."""
+    eel = eel or inputs.eel3d()
+    out: Dict[str, dict] = {}
+    us = {}
+    for b in ("capsules", "mask"):
+        u, drag, h = run(eel, "f32", steps, log=log, boundary=b, history=True)
+        us[b] = u
+        half = h[steps // 2:].sum(axis=1)
+        out[b] = {"mean_drag": float(half[:, 0].mean()),
+                  "lateral_amplitude": float(0.5 * (half[:, 1].max() - half[:, 1].min())),
+                  "yaw_torque_amplitude": float(0.5 * (half[:, 5].max() - half[:, 5].min()))}
+    ref = out["capsules"]
+    rel = {k: abs(out["mask"][k] - ref[k]) / abs(ref[k]) for k in ref}
+    u_rel = (torch.linalg.vector_norm(us["mask"] - us["capsules"]) / torch.linalg.vector_norm(us["capsules"])).item()
+    del us
+    torch.cuda.empty_cache()
+    return {"workload": f"d3q27_eel_{eel.nx}x{eel.ny}x{eel.nz}", "steps": steps, "capsules": ref,
+            "mask": out["mask"], "mask_rel_diff": rel, "u_rel_l2_mask_vs_capsules": u_rel}
 
 
 def sweep(steps: int = 5000, dtypes: Iterable[str] = ("f64", "f32", "f16", "bf16"), eel=None, log=None) -> Dict:
