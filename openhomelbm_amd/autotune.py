@@ -97,11 +97,13 @@ def candidates() -> Iterable[Tuple[int, int, int]]:
 
 
 def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[int, int, int]]] = None,
-         save: bool = True, path: Optional[str] = None, log=None) -> dict:
+         save: bool = True, path: Optional[str] = None, log=None, rounds: int = 2, settle: int = 200) -> dict:
     """Time the tiled kernel on `lat` for every shape; keep the fastest.
 
-    The lattice's state is restored afterwards. Returns the size-class entry
-    {"tile": [tx, ty, zc], "us_per_step": t, "timings": {...}}."""
+    `settle` steps run first (the clocks ramp up over the first launches);
+    every shape is then timed `rounds` times, interleaved, and keeps its best
+    time. The lattice's state is restored afterwards. Returns the size-class
+    entry {"tile": [tx, ty, zc], "us_per_step": t, "timings": {...}}."""
     import torch
 
     if lat.lattice != 27:
@@ -109,19 +111,26 @@ def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[
     saved = lat.buf.clone()
     cur = lat.cur
     timings = {}
+    shapes = list(shapes or candidates())
     try:
-        for t in (shapes or candidates()):
-            N.set_tiling(27, *t)
-            lat.step(warmup, N.HL_KERNEL_TILED, _tuned=False)
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            e0.record()
-            lat.step(steps, N.HL_KERNEL_TILED, _tuned=False)
-            e1.record()
-            torch.cuda.synchronize()
-            timings["x".join(map(str, t))] = e0.elapsed_time(e1) * 1e3 / steps
-            if log:
-                log(f"[autotune] tile {t}: {timings['x'.join(map(str, t))]:.1f} us/step")
+        if settle > 0:
+            N.set_tiling(27, *shapes[0])
+            lat.step(settle, N.HL_KERNEL_TILED, _tuned=False)
+        for _ in range(max(rounds, 1)):
+            for t in shapes:
+                N.set_tiling(27, *t)
+                lat.step(warmup, N.HL_KERNEL_TILED, _tuned=False)
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                e0.record()
+                lat.step(steps, N.HL_KERNEL_TILED, _tuned=False)
+                e1.record()
+                torch.cuda.synchronize()
+                key = "x".join(map(str, t))
+                us = e0.elapsed_time(e1) * 1e3 / steps
+                timings[key] = min(us, timings.get(key, us))
+                if log:
+                    log(f"[autotune] tile {t}: {us:.1f} us/step")
     finally:
         lat.buf.copy_(saved)
         lat.cur = cur
@@ -145,9 +154,10 @@ def main(argv=None) -> None:
 
     p = argparse.ArgumentParser()
     p.add_argument("--out", default=None, help="table path (default: the shipped table)")
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=50)
     a = p.parse_args(argv)
-    # one lattice per size class: 96^3 (4^9), 128^3 (4^10), the 256x128x128 sphere (4^11)
+    # one lattice per size class: 96^3 (4^9), 128^3 (4^10), the 256x128x128
+    # sphere (4^11)
     for shape in ((96, 96, 96), (128, 128, 128), (256, 128, 128)):
         lat = Lattice(27, *shape)
         lat.nu = 0.01
