@@ -21,12 +21,15 @@ Warp arrays; whole-array assignment (flow.bc_value = ..., flow.vis_shear =
 * Forces and torques are reduced in FP64 in a fixed order and are
   deterministic; solid_forcex / solid_forcey / torque hold them as FP32 like
   the reference's atomics.
-* create_solid_from_mujoco needs MuJoCo, which is not available here; it
-  raises with that message.
+* create_solid_from_mujoco takes an MJCF file (or openhomelbm_amd.mjcf.Model)
+  instead of a MuJoCo model and data: MuJoCo does not run here, so the
+  outline comes from the file's reference configuration (every joint at 0).
 * step() is asynchronous like the reference's graph launch; reading a field
   synchronizes.
 """
 from __future__ import annotations
+
+import os
 
 import math
 from typing import Any, Optional, Sequence
@@ -538,8 +541,31 @@ class LBM_Solver:
 
     def create_solid_from_mujoco(self, solid_id, model, data, body_or_geom_name, lbm_position=None, lbm_scale=0.3,
                                  n_samples=20, is_body=True, **kw):
-        raise NotImplementedError("create_solid_from_mujoco needs MuJoCo, which is not available on this platform; "
-                                  "build the outline with HomeFlow.configure_solid (polygon) or add_capsule")
+        """Project one body or geom of an MJCF model into every world
+        (lbm_solver.py:143-208, extract_lbm_polygon_from_mujoco,
+        envs/lbm/mujoco_to_lbm.py:343-395): its x-y outline, convex hull and
+        centring, placed at lbm_position (the grid centre by default) with its
+        rotation about z and lbm_scale. `model` is an MJCF file path or an
+        openhomelbm_amd.mjcf.Model (the file's reference configuration: every
+        joint at 0, as after mj_forward on qpos0); `data` is unused. A MuJoCo
+        MjModel is not accepted (MuJoCo does not run on this platform)."""
+        from . import mjcf
+
+        if isinstance(model, (str, os.PathLike)):
+            model = mjcf.load_model(os.fspath(model))
+        if not isinstance(model, mjcf.Model):
+            raise NotImplementedError("create_solid_from_mujoco takes an MJCF path or openhomelbm_amd.mjcf.Model; "
+                                      "MuJoCo models need MuJoCo, which is not available on this platform")
+        info = mjcf.polygon_from_model(model, body_or_geom_name, n_samples=n_samples, use_convex_hull=True,
+                                       normalize=True, is_body=is_body)
+        if lbm_position is None:
+            lbm_position = (self.nx * 0.5, self.ny * 0.5)
+        for flow in self.flows:
+            flow.configure_solid(solid_id, info["vertices"], lbm_position, angle=info["angle"], scale=lbm_scale, **kw)
+        self.mujoco_mappings[solid_id] = {"mujoco_origin": np.array(info["position"], np.float64).copy(),
+                                          "lbm_origin": np.array(lbm_position, np.float32),
+                                          "scale": lbm_scale * self.nx}
+        return info
 
     def finalize_mappings(self, solid_ids: Sequence[int]) -> None:
         """Coordinate-mapping tables for the coupling (lbm_solver.py:211-240)."""

@@ -423,6 +423,36 @@ class LBM_Solver3D:
         return self.create_solid_capsule(solid_id, (0, 0, 0), (0, 0, 0), radius, lbm_position, lbm_scale,
                                          (1, 0, 0, 0), mujoco_origin)
 
+    def create_solids_from_mjcf(self, mjcf_path: str, root_position: Optional[Sequence[float]] = None,
+                                lbm_scale: float = 0.15, first_solid: int = 0,
+                                mesh_search_paths: Optional[Sequence[str]] = None) -> Dict[str, int]:
+        """One mesh solid per MJCF body with geometry, in file order from
+        solid first_solid (the body meshes the reference's fluid environment
+        takes from parse_mjcf, envs/lbm3d/lbm_fluid_env_3d.py:130-131): each
+        body's combined geom mesh in its body frame, placed at root_position +
+        lbm_scale * nx * (its world position) with its world orientation, and
+        mapped from MuJoCo to lattice coordinates as create_solid_from_mesh
+        maps it. Returns {body name: solid id}."""
+        from . import mjcf
+
+        data = mjcf.parse_mjcf(mjcf_path, list(mesh_search_paths) if mesh_search_paths else None)
+        poses = mjcf.get_body_world_poses(data)
+        if first_solid + len(data["bodies"]) > self.solid_num:
+            raise ValueError(f"{len(data['bodies'])} bodies from solid {first_solid} need solid_num >= "
+                             f"{first_solid + len(data['bodies'])}, have {self.solid_num}")
+        if root_position is None:
+            root_position = (self.nx * 0.5, self.ny * 0.5, self.nz * 0.5)
+        s = lbm_scale * self.nx
+        ids = {}
+        for k, b in enumerate(data["bodies"]):
+            p, R = poses[b["name"]]
+            sid = first_solid + k
+            self.create_solid_from_mesh(sid, b["combined_mesh"], lbm_position=tuple(np.asarray(root_position) + s * p),
+                                        lbm_scale=lbm_scale, init_quaternion=tuple(mjcf.matrix_to_quat(R)),
+                                        mujoco_origin=p)
+            ids[b["name"]] = sid
+        return ids
+
     def update_solids_batch(self, world_idx: int, solid_ids: Sequence[int], positions: np.ndarray,
                             quaternions: np.ndarray) -> None:
         """Poses of several solids of one world (lbm_solver_3d.py:270-306):
