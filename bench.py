@@ -309,10 +309,11 @@ def bench_coupled(K: int, W: int = 20):
             "step": "capsule mask + pose table + lattice step with forces + load conversion + free-body step"}
 
 
-def bench_batched(K: int, W: int, per_env_steps: int, graph_steps: int = 50):
+def bench_batched(K: int, W: int, per_env_steps: int, graph_steps: int = 50, run_steps: int = 2000):
     """64 eel environments (D2Q9, uneven sizes, masks re-voxelized every step):
     batched launches vs the one-launch-per-environment loop of the same kernels,
-    plus a captured multi-step graph. Per-step device latency from events."""
+    plus a captured multi-step graph and run_steps steps in one launch of the
+    persistent engine. Per-step device latency from events."""
     import torch
 
     from openhomelbm_amd import inputs
@@ -354,7 +355,20 @@ def bench_batched(K: int, W: int, per_env_steps: int, graph_steps: int = 50):
     g1.record()
     torch.cuda.synchronize()
     ms_graph = g0.elapsed_time(g1) / (reps * graph_steps)
-    drag = gh[-1, :, :8, 0].sum().item()
+    # run_steps steps of every environment in one persistent launch
+    rh = b.history_buffer(run_steps)
+    b.run(8)
+    torch.cuda.synchronize()
+    r0, r1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    r0.record()
+    b.run(run_steps, force_history=rh)
+    r1.record()
+    torch.cuda.synchronize()
+    ms_run = r0.elapsed_time(r1) / run_steps
+    run_timed_out = b.run_timed_out()
+    if run_timed_out:
+        raise RuntimeError("a grid-barrier wait of the persistent launch gave up")
+    drag = gh[-1, :, :8, 0].sum().item() + rh[-1, :, :8, 0].sum().item()
     if not np.isfinite(drag) or not all(np.isfinite(L.fields()[0].float().sum().item()) for L in lats[:4]):
         raise RuntimeError("non-finite state after the batched run")
     mlups = lambda ms: cells / (ms * 1e-3) / 1e6
@@ -362,6 +376,7 @@ def bench_batched(K: int, W: int, per_env_steps: int, graph_steps: int = 50):
     out = {"workload": "d2q9_eels64", "envs": 64, "cells": cells, "steps": K,
            "mlups": round(mlups(best), 1), "mlups_eager": round(mlups(ms_batched), 1),
            "mlups_graph": round(mlups(ms_graph), 1), "mlups_per_env_loop": round(mlups(ms_per_env), 1),
+           "mlups_persistent": round(mlups(ms_run), 1), "persistent_steps_per_launch": run_steps,
            "speedup_vs_per_env_loop": round(ms_per_env / best, 2),
            "step_ms_p50": round(float(np.percentile(lat_ms, 50)), 4),
            "step_ms_p99": round(float(np.percentile(lat_ms, 99)), 4),

@@ -23,7 +23,7 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
            "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-           "hl_step_noalloc", "hl_step_capsules")
+           "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -99,6 +99,8 @@ def load():
     L.hl_batch_capture.argtypes = [P, P, I32, P]
     L.hl_batch_replay.argtypes = [P, P]
     L.hl_batch_size.argtypes = [P, PI32, PI32, PI32]
+    L.hl_batch_run.argtypes = [P, P, I32, P, I32, P]
+    L.hl_batch_run_status.argtypes = [P, PI32]
     F32 = C.c_float
     L.hl_render_plane.argtypes = [env_p, I32, I32, I32, I32, F32, F32, P, P]
     L.hl_render_projection.argtypes = [env_p, I32, I32, I32, P, P]
@@ -119,7 +121,7 @@ def load():
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
                  "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-                 "hl_step_noalloc", "hl_step_capsules"):
+                 "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

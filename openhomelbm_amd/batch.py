@@ -186,6 +186,23 @@ class Batch:
                                        fh, 1 if per_env else 0, _stream_ptr(stream)), "hl_batch_step")
         self._advance_mirror(a, nsteps)
 
+    def run(self, nsteps: int = 1, active=None, force_history: Optional[torch.Tensor] = None,
+            per_env: bool = False, stream=None) -> None:
+        """Advance the (active) environments nsteps in one launch of the
+        persistent engine (per_env: one launch per environment); states equal
+        step()'s bitwise."""
+        act_dev, a = self._active(active)
+        fh = self._check_history(force_history, nsteps)
+        N.check(self.lib.hl_batch_run(self._h, act_dev.data_ptr() if act_dev is not None else None, int(nsteps),
+                                      fh, 1 if per_env else 0, _stream_ptr(stream)), "hl_batch_run")
+        self._advance_mirror(a, nsteps)
+
+    def run_timed_out(self) -> bool:
+        """Whether the last run() gave up a grid-barrier wait (synchronizes)."""
+        v = C.c_int32(0)
+        N.check(self.lib.hl_batch_run_status(self._h, C.byref(v)), "hl_batch_run_status")
+        return bool(v.value)
+
     def capture(self, nsteps: int, active=None, force_history: Optional[torch.Tensor] = None) -> None:
         """Capture nsteps batched steps into a graph replayed by replay()."""
         act_dev, a = self._active(active)

@@ -10,7 +10,7 @@
 // Per step, for all active environments:
 //   k_bvox      rebuild the body mask in the body box from the capsule table
 //               at the environment's time t
-//   k_bstep     the D2Q9 update of one 64x4 tile staged with its halo in LDS
+//   k_bstep     the D2Q9 update of one 64x8 tile staged with its halo in LDS
 //               (reference operation order, mask wall rule); tiles that
 //               meet the body also reduce the per-solid momentum-exchange
 //               force of the state they read into an FP64 partial
@@ -19,6 +19,16 @@
 // The same kernels can be launched one environment at a time (the
 // per-environment baseline); every cell's arithmetic and every sum's order
 // are the same, so the two schedules give bitwise equal results.
+//
+// The persistent engine (k_brun, hl_batch_run) advances every environment K
+// steps in one cooperative launch: the workgroups deal out the step tiles of
+// every environment, each tile doing k_bstep's update with k_bvox fused into
+// its staging (each staged cell's mask value is computed from the capsule
+// table at the environment's time and the tile's own cells are written to
+// the mask), then a grid barrier ends the step; the first pass of step k+1
+// sums step k's force rows into the history. The cell arithmetic, the tile
+// slots and the summation order are those of the three-kernel schedule, so
+// states and forces agree with it bit for bit.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -42,6 +52,7 @@ constexpr int kLX = kTX + 3, kLY = kTY + 3;  // staged region: 2 cells below, 1 
 constexpr int kStage = (kLX * kLY + kThreads - 1) / kThreads;
 constexpr int kVoxW = 16;         // voxelizer item: 16 x 16 cells
 constexpr int kPartialWidth = kBatchMaxSolids * 3;  // Fx, Fy, Tz per solid
+constexpr int kRunCaps = 32;  // capsules per environment the persistent engine stages in LDS
 
 struct BatchEnv {
     EnvArgs base;         // arguments for parity 0 (moments[0] -> moments[1])
@@ -69,6 +80,17 @@ struct BatchArgs {
 
 __device__ __forceinline__ bool env_active(const BatchArgs& b, int e) { return b.active == nullptr || b.active[e]; }
 
+// A block-uniform value read from LDS, moved to a scalar register (so
+// addresses built from it are a scalar base plus a 32-bit lane offset).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+template <typename T>
+__device__ __forceinline__ T* uni(T* p)
+{
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a), hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
 // Per-block copy of the environment's arguments in LDS, with the source,
 // destination and poses of its current step.
 __device__ __forceinline__ void load_env(const BatchArgs& b, int e, EnvArgs& ea)
@@ -89,23 +111,15 @@ __device__ __forceinline__ void load_env(const BatchArgs& b, int e, EnvArgs& ea)
     __syncthreads();
 }
 
-// Capsule voxelizer over one item (the same evaluation as the single-lattice
-// voxelizer): mask = solid id + 1 of the lowest-index capsule whose segment
-// lies within its radius of the cell centre, else 0.
-__global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0)
+// Mask value of cell (x, y) (the same evaluation as the single-lattice
+// voxelizer): solid id + 1 of the lowest-index capsule whose segment lies
+// within its radius of the cell centre, else 0.
+template <typename P>
+__device__ __forceinline__ uint8_t capsule_voxel(P caps, int ncap, int x, int y)
 {
-    const BatchItem it = b.vox_items[item0 + blockIdx.x];
-    if (!env_active(b, it.env)) return;
-    const BatchEnv& be = b.envs[it.env];
-    if (!be.caps || be.T <= 0) return;
-    const float* caps = be.caps + (int64_t)(b.time[it.env] % be.T) * be.ncap * 8;
-    const int k = threadIdx.x;
-    if (k >= it.w * it.h) return;
-    const int x = it.x0 + k % it.w, y = it.y0 + k / it.w;
     const float px = (float)x, py = (float)y, pz = 0.0f;
-    uint8_t v = 0;
-    for (int j = 0; j < be.ncap; ++j) {
-        const float* C = caps + 8 * j;
+    for (int j = 0; j < ncap; ++j) {
+        const auto C = caps + 8 * j;
         float abx = C[3] - C[0], aby = C[4] - C[1], abz = C[5] - C[2];
         float apx = px - C[0], apy = py - C[1], apz = pz - C[2];
         float len2 = abx * abx + aby * aby + abz * abz;
@@ -116,9 +130,24 @@ __global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0)
         }
         float dx = apx - tt * abx, dy = apy - tt * aby, dz = apz - tt * abz;
         float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 <= C[6] * C[6]) { v = (uint8_t)((int)C[7] + 1); break; }
+        if (d2 <= C[6] * C[6]) return (uint8_t)((int)C[7] + 1);
     }
-    const_cast<uint8_t*>(be.base.mask)[(int64_t)x + (int64_t)be.base.nx * y] = v;
+    return 0;
+}
+
+// Capsule voxelizer over one item.
+__global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0)
+{
+    const BatchItem it = b.vox_items[item0 + blockIdx.x];
+    if (!env_active(b, it.env)) return;
+    const BatchEnv& be = b.envs[it.env];
+    if (!be.caps || be.T <= 0) return;
+    const float* caps = be.caps + (int64_t)(b.time[it.env] % be.T) * be.ncap * 8;
+    const int k = threadIdx.x;
+    if (k >= it.w * it.h) return;
+    const int x = it.x0 + k % it.w, y = it.y0 + k / it.w;
+    const_cast<uint8_t*>(be.base.mask)[(int64_t)x + (int64_t)be.base.nx * y] =
+        capsule_voxel(as_global(caps), be.ncap, x, y);
 }
 
 __device__ __forceinline__ double bwave_sum(double v)
@@ -128,7 +157,13 @@ __device__ __forceinline__ double bwave_sum(double v)
     return v;
 }
 
-__global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
+// One step of tile `it` of environment be at buffer parity par and time t,
+// its force row (if any) into part. VOX: the staged cells are voxelized from
+// the capsule table at t on the spot (and the tile's own cells of the mask
+// written) instead of read from the mask. The block's threads all call it
+// for the same tile.
+template <bool VOX>
+__device__ __forceinline__ void bstep_tile(const BatchEnv& be, const BatchItem it, int par, int t, double* part)
 {
     __shared__ EnvArgs ea;
     __shared__ float tile[6][kLY][kLX];   // moments (near the body only)
@@ -136,26 +171,32 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
     __shared__ uint8_t mtile[kLY][kLX + 1];
     __shared__ float pose[kBatchMaxSolids * 8];
     __shared__ double acc[kWaves][kPartialWidth];
-    const BatchItem it = b.step_items[item0 + blockIdx.x];
-    const int e = it.env;
-    if (!env_active(b, e)) return;
+    __shared__ float vcaps[kRunCaps * 8];  // VOX: the capsule table at t
     const int tid = threadIdx.x;
     const bool near = it.slot >= 0;
-    const BatchEnv& be = b.envs[e];
-    const int par = b.parity[e] & 1;
-    const int nx = be.base.nx, ny = be.base.ny, nsolid = be.base.nsolid;
-    const int64_t st = be.base.stride;
-    const float* src_p = static_cast<const float*>(par ? be.base.dst : be.base.src);
+    const int nx = uni(be.base.nx), ny = uni(be.base.ny), nsolid = uni(be.base.nsolid);
+    // an environment's fields hold fewer than 2^31 values: 32-bit offsets
+    const int st = uni((int)be.base.stride);
+    const float* src_p = uni(static_cast<const float*>(par ? be.base.dst : be.base.src));
     const auto src = as_global(src_p);
-    const auto mask = as_global(be.base.mask);
+    const auto mask = as_global(uni(const_cast<uint8_t*>(be.base.mask)));
     // the staged region [x0-2, x0+kTX] x [y0-2, y0+kTY] reaches the body box
     // (the mask is 0 outside it): stage the mask and evaluate the wall rule
     const bool masked = mask != nullptr && it.x0 - 2 <= be.base.box[1] && it.x0 + kTX >= be.base.box[0] - 1 &&
                         it.y0 - 2 <= be.base.box[3] && it.y0 + kTY >= be.base.box[2] - 1;
-    const float* poses = be.poses && be.T > 0 ? be.poses + (int64_t)(b.time[e] % be.T) * nsolid * 8 : be.base.solids;
-    // stage the tile with a halo of two cells below and one above in x and y
-    // (so every face cell's interior projection finds its sources here); all
-    // loads are issued before the first wait
+    const int T = uni(be.T), ncap = uni(be.ncap);
+    const float* bposes = uni(be.poses);
+    const float* caps = uni(be.caps);
+    const float* poses = bposes && T > 0 ? bposes + (t % T) * nsolid * 8 : uni(be.base.solids);
+    const bool vox = VOX && masked && caps && T > 0;
+    const int bx0 = uni(be.base.box[0]), bx1 = uni(be.base.box[1]), by0 = uni(be.base.box[2]), by1 = uni(be.base.box[3]);
+    // issue every load of the tile before the first wait: the capsule table
+    // and poses at t, then the tile with a halo of two cells below and one
+    // above in x and y (so every face cell's interior projection finds its
+    // sources here) and its mask
+    float cv = 0.0f, pv = 0.0f;
+    if (vox && tid < ncap * 8) cv = as_global(caps + (t % T) * ncap * 8)[tid];
+    if (masked && poses && tid < nsolid * 8) pv = as_global(poses)[tid];
     float v[kStage][6];
     int mv[kStage];
     bool in[kStage];
@@ -165,13 +206,26 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
         const int lx = k % kLX, ly = k / kLX;
         const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
         in[r] = k < kLX * kLY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const int64_t c = (int64_t)(in[r] ? gx : 0) + (int64_t)nx * (in[r] ? gy : 0);
+        const int c = (in[r] ? gx : 0) + nx * (in[r] ? gy : 0);
 #pragma unroll
         for (int f = 0; f < 6; ++f) v[r][f] = in[r] ? src[f * st + c] : 0.0f;
-        mv[r] = (masked && in[r]) ? mask[c] : 0;
+        mv[r] = (!vox && masked && in[r]) ? mask[c] : 0;
     }
-    float pv = 0.0f;
-    if (masked && poses && tid < nsolid * 8) pv = poses[tid];
+    if (vox) {
+        if (tid < ncap * 8) vcaps[tid] = cv;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kStage; ++r) {
+            const int k = tid + r * kThreads;
+            const int lx = k % kLX, ly = k / kLX;
+            const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
+            if (in[r] && gx >= bx0 && gx < bx1 && gy >= by0 && gy < by1) {
+                mv[r] = capsule_voxel(vcaps, ncap, gx, gy);
+                // the tile's own cells of the mask
+                if (lx >= 2 && lx < kTX + 2 && ly >= 2 && ly < kTY + 2) mask[gx + nx * gy] = (uint8_t)mv[r];
+            }
+        }
+    }
     static_assert(sizeof(EnvArgs) % 4 == 0 && sizeof(EnvArgs) / 4 <= kThreads, "EnvArgs copy");
     if (tid < sizeof(EnvArgs) / 4)
         reinterpret_cast<uint32_t*>(&ea)[tid] = reinterpret_cast<const uint32_t*>(&be.base)[tid];
@@ -208,7 +262,7 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6;
     const int x = it.x0 + lane;
-    const auto dst = as_global(static_cast<float*>(ea.dst));
+    const auto dst = as_global(uni(static_cast<float*>(const_cast<void*>(par ? be.base.src : be.base.dst))));
     uint32_t cutbits[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
@@ -242,7 +296,7 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
         d2q9_collide<float>(ea, pop, 0.0f, 0.0f, m);
         if (px != x || py != y) bc_chain<float, 2>(m, x, y, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
         else cutbits[r] = cb;  // forces come from interior cells only
-        const int64_t c = (int64_t)x + (int64_t)nx * y;
+        const int c = x + nx * y;
 #pragma unroll
         for (int f = 0; f < 6; ++f) dst[f * st + c] = m[f];
     }
@@ -296,10 +350,17 @@ __global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
     }
     __syncthreads();
     if (tid < kPartialWidth) {
-        double t = 0.0;
-        for (int w = 0; w < kWaves; ++w) t += acc[w][tid];
-        as_global(b.partial)[(int64_t)it.slot * kPartialWidth + tid] = t;
+        double sum = 0.0;
+        for (int w = 0; w < kWaves; ++w) sum += acc[w][tid];
+        as_global(part)[(int64_t)it.slot * kPartialWidth + tid] = sum;
     }
+}
+
+__global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
+{
+    const BatchItem it = b.step_items[item0 + blockIdx.x];
+    if (!env_active(b, it.env)) return;
+    bstep_tile<false>(b.envs[it.env], it, b.parity[it.env] & 1, b.time[it.env], b.partial);
 }
 
 // Per environment (one block): the fixed-order sum of its partial slots into
@@ -341,6 +402,122 @@ __global__ void __launch_bounds__(64) k_badvance(BatchArgs b, int env0, double* 
     }
 }
 
+// ------------------------------------------------- persistent engine
+
+// grid barrier: iterations of a 256-cycle sleep before a wait gives up and
+// flags the launch (about half a second)
+constexpr unsigned kSpinLimit = 1u << 21;
+
+// Every workgroup arrives once per barrier; target = arrivals expected so
+// far. Stores before the barrier are released to the device (every XCD's
+// L2) and loads after it acquire them. A wait that outlasts kSpinLimit
+// sleeps sets *flag and lets the launch finish (wrong results, no hang).
+__device__ __forceinline__ void grid_barrier(unsigned* bar, unsigned target, int* flag)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned n = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(4);
+            if (++n > kSpinLimit) {
+                __hip_atomic_fetch_or(flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+// Step k's force rows (bank part) of environments [e0, e0 + ne) summed in
+// slot order into history[k][env][16][6] (Fx, Fy, 0, 0, 0, Tz), one thread
+// per output: the order of k_badvance, so the sums agree bitwise.
+__device__ __forceinline__ void run_sums(const BatchArgs& b, const double* part, int k, double* history,
+                                         int nenv_total, int e0, int ne)
+{
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (int o = gt; o < ne * kBatchMaxSolids * 6; o += nt) {
+        const int e = e0 + o / (kBatchMaxSolids * 6), r = o % (kBatchMaxSolids * 6);
+        if (!env_active(b, e)) continue;
+        const int so = r / 6, comp = r % 6;
+        double v = 0.0;
+        if (comp == 0 || comp == 1 || comp == 5) {
+            const BatchEnv& be = b.envs[e];
+            const double* p = part + (int64_t)be.slot_begin * kPartialWidth + so * 3 + (comp == 5 ? 2 : comp);
+            const int n = be.slot_count;
+            int j = 0;
+            for (; j + 8 <= n; j += 8) {  // loads issued together, added in slot order
+                double tv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) tv[u] = p[(int64_t)(j + u) * kPartialWidth];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v += tv[u];
+            }
+            for (; j < n; ++j) v += p[(int64_t)j * kPartialWidth];
+        }
+        history[((int64_t)k * nenv_total + e) * kBatchMaxSolids * 6 + r] = v;
+    }
+}
+
+// K steps of the step tiles [i0, i0 + ni) (environments [e0, e0 + ne)) in
+// one launch: the tiles dealt to the workgroups in order, each tile the
+// three-kernel schedule's step with the voxelizer fused into its staging; a
+// grid barrier after every step; step k's force rows go to bank k & 1 and
+// are summed by step k + 1 (the last step's after the loop); then parity and
+// time advance by K.
+__global__ void __launch_bounds__(kThreads) k_brun(BatchArgs b, int i0, int ni, int K, double* history,
+                                                   int nenv_total, int e0, int ne, int nslots, unsigned* bar,
+                                                   int* flag)
+{
+    static_assert(sizeof(BatchEnv) % 4 == 0 && sizeof(BatchEnv) / 4 < kThreads, "BatchEnv copy");
+    __shared__ BatchEnv hdr;
+    __shared__ int hst[3];  // parity, time, active
+    for (int k = 0; k < K; ++k) {
+        if (k > 0 && history)
+            run_sums(b, b.partial + (size_t)((k - 1) & 1) * nslots * kPartialWidth, k - 1, history, nenv_total, e0, ne);
+        for (int i = i0 + blockIdx.x; i < i0 + ni; i += gridDim.x) {
+            // the tile's environment copied to LDS in one round of loads (its
+            // fields then read from LDS, not one dependent load at a time)
+            const BatchItem it = b.step_items[i];
+            const int e = it.env;
+            __syncthreads();  // the previous tile's LDS reads are done
+            if (threadIdx.x < sizeof(BatchEnv) / 4) {
+                reinterpret_cast<uint32_t*>(&hdr)[threadIdx.x] =
+                    as_global(reinterpret_cast<const uint32_t*>(b.envs + e))[threadIdx.x];
+            } else if (threadIdx.x >= kThreads - 64) {  // the last wave: loads issued together
+                const int p = as_global(b.parity)[e], tm = as_global(b.time)[e];
+                const int a = b.active ? (int)as_global(b.active)[e] : 1;
+                if (threadIdx.x == kThreads - 1) {
+                    hst[0] = p;
+                    hst[1] = tm;
+                    hst[2] = a;
+                }
+            }
+            __syncthreads();
+            if (!hst[2]) continue;
+            bstep_tile<true>(hdr, it, (hst[0] + k) & 1, hst[1] + k,
+                             b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
+        }
+        grid_barrier(bar, (unsigned)(k + 1) * gridDim.x, flag);
+    }
+    if (K > 0 && history)
+        run_sums(b, b.partial + (size_t)((K - 1) & 1) * nslots * kPartialWidth, K - 1, history, nenv_total, e0, ne);
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int e = e0 + gt; e < e0 + ne; e += gridDim.x * blockDim.x)
+        if (env_active(b, e)) {
+            b.parity[e] = (b.parity[e] + K) & 1;
+            b.time[e] += K;
+        }
+}
+
+__global__ void k_brun_reset(unsigned* bar, int* flag)
+{
+    *bar = 0u;
+    *flag = 0;
+}
+
 }  // namespace hl
 
 // ----------------------------------------------------------------- C ABI
@@ -350,6 +527,8 @@ struct hl_batch {
     std::vector<hl::BatchEnv> envs;  // host mirror of the device table
     std::vector<hl::BatchItem> step_items, vox_items;
     int nslots = 0;
+    unsigned* d_bar = nullptr;  // grid barrier counter of the persistent engine
+    int* d_flag = nullptr;      // set when a barrier wait gave up
     hl::BatchEnv* d_envs = nullptr;
     hl::BatchItem *d_step = nullptr, *d_vox = nullptr;
     size_t cap_envs = 0, cap_step = 0, cap_vox = 0, cap_partial = 0;
@@ -434,7 +613,9 @@ int32_t sync_device(hl_batch* b)
         (e = upload(&b->d_step, &b->cap_step, b->step_items.data(), b->step_items.size())) != hipSuccess ||
         (e = upload(&b->d_vox, &b->cap_vox, b->vox_items.data(), b->vox_items.size())) != hipSuccess)
         return hip_report(e, "batch work-list upload");
-    const size_t np = (size_t)std::max(b->nslots, 1) * hl::kPartialWidth;
+    // force rows of the tiles: two banks (the persistent engine sums one
+    // step's while it writes the next)
+    const size_t np = (size_t)std::max(2 * b->nslots, 1) * hl::kPartialWidth;
     if (!b->d_partial || b->cap_partial < np) {
         if (b->d_partial && (e = hipFree(b->d_partial)) != hipSuccess) return hip_report(e, "hipFree");
         b->d_partial = nullptr;
@@ -547,6 +728,8 @@ void hl_batch_destroy(hl_batch* b)
     hipFree(b->d_parity);
     hipFree(b->d_time);
     hipFree(b->d_partial);
+    hipFree(b->d_bar);
+    hipFree(b->d_flag);
     delete b;
 }
 
@@ -644,6 +827,71 @@ int32_t hl_batch_replay(hl_batch* b, void* stream)
     if (b->dirty) return report(HL_E_INVALID, "bodies changed since the capture; capture again");
     hipError_t e = hipGraphLaunch(b->graph_exec, static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_report(e, "hipGraphLaunch");
+    return HL_OK;
+}
+
+int32_t hl_batch_run(hl_batch* b, const uint8_t* active, int32_t nsteps, double* force_history, int32_t per_env,
+                     void* stream)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    if (nsteps < 0) return report(HL_E_INVALID, "nsteps must be >= 0");
+    if (nsteps == 0) return HL_OK;
+    for (int env = 0; env < b->n; ++env)
+        if (b->envs[env].caps && b->envs[env].ncap > hl::kRunCaps) {
+            std::string m = "environment " + std::to_string(env) + ": the persistent engine takes at most " +
+                            std::to_string(hl::kRunCaps) + " capsules (hl_batch_step takes any number)";
+            return report(HL_E_UNSUPPORTED, m.c_str());
+        }
+    int32_t rc = sync_device(b);
+    if (rc) return rc;
+    hipError_t e;
+    if (!b->d_bar && (e = hipMalloc(&b->d_bar, sizeof(unsigned))) != hipSuccess) return hip_report(e, "hipMalloc");
+    if (!b->d_flag && (e = hipMalloc(&b->d_flag, sizeof(int))) != hipSuccess) return hip_report(e, "hipMalloc");
+    int dev = 0, cus = 0, occ = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess ||
+        (e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess ||
+        (e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, hl::k_brun, hl::kThreads, 0)) != hipSuccess)
+        return hip_report(e, "persistent engine occupancy");
+    if (occ < 1) return report(HL_E_UNSUPPORTED, "the persistent batch kernel does not fit on a CU");
+    // at most two workgroups per CU: the occupancy calculator's four per CU
+    // (8 waves of 8 a SIMD) were not all resident at once on MI355X, and the
+    // barrier waits gave up; two per CU were
+    occ = std::min(occ, 2);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const hl::BatchArgs a = batch_args(b, active);
+    auto launch = [&](int i0, int ni, int e0, int ne) -> int32_t {
+        if (ni <= 0) return HL_OK;
+        // every workgroup resident (a cooperative launch); no more than the tiles
+        const int grid = std::max(1, std::min(occ * cus, ni));
+        hipLaunchKernelGGL(hl::k_brun_reset, dim3(1), dim3(1), 0, s, b->d_bar, b->d_flag);
+        int i0v = i0, niv = ni, K = nsteps, ntot = b->n, e0v = e0, nev = ne, nsl = b->nslots;
+        double* hist = force_history;
+        unsigned* bar = b->d_bar;
+        int* flag = b->d_flag;
+        hl::BatchArgs av = a;
+        void* args[] = {&av, &i0v, &niv, &K, &hist, &ntot, &e0v, &nev, &nsl, &bar, &flag};
+        hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(hl::k_brun), dim3(grid),
+                                                   dim3(hl::kThreads), args, 0, s);
+        if (le != hipSuccess) return hip_report(le, "persistent batch launch");
+        return HL_OK;
+    };
+    if (!per_env) return launch(0, (int)b->step_items.size(), 0, b->n);
+    for (int env = 0; env < b->n; ++env) {
+        rc = launch(b->envs[env].step_begin, b->envs[env].step_count, env, 1);
+        if (rc) return rc;
+    }
+    return HL_OK;
+}
+
+int32_t hl_batch_run_status(hl_batch* b, int32_t* timed_out)
+{
+    if (!b || !timed_out) return report(HL_E_INVALID, "batch and timed_out are required");
+    *timed_out = 0;
+    if (!b->d_flag) return HL_OK;
+    int v = 0;
+    hipError_t e = hipMemcpy(&v, b->d_flag, sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_report(e, "batch run status");
+    *timed_out = v;
     return HL_OK;
 }
 

@@ -17,12 +17,13 @@ namespace hl {
 // cell's moments ms; clamped at 0 on both branches as in the reference.
 template <typename T>
 __device__ __forceinline__ T d2q9_link_pop(const EnvArgs& a, int x, int y, int i, const T* mc, int cut, int sid,
-                                           const T* ms)
+                                           const T* ms, const float* poses = nullptr)
 {
     T pop;
     if (cut) {
         T up[2], r[2];
-        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        wall_velocity_2d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]),
+                            T(y) - T(0.5) * T(D2CY[i]), up, r);
         T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
         T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
         T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
@@ -350,7 +351,8 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
 {
     if (Q == 9) {
         T up[2], r[2];
-        wall_velocity_2d<T>(a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]), T(y) - T(0.5) * T(D2CY[i]), up, r);
+        wall_velocity_2d<T>(poses ? poses : a.solids, a.nsolid, sid, T(x) - T(0.5) * T(D2CX[i]),
+                            T(y) - T(0.5) * T(D2CY[i]), up, r);
         T Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
         T Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
         T Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
