@@ -316,6 +316,74 @@ int32_t hl_body_capsules(int32_t nworld, int32_t nbody, const float* pos, int32_
                          void* stream);
 
 /* ------------------------------------------------------------------------
+ * Articulated chains: planar kinematic trees (a free or fixed root, hinges
+ * about z, slides in the plane) stepped on the device in FP64, one workgroup
+ * per world. Stands in for the MuJoCo-Warp step of the reference's coupled
+ * loop (envs/lbm/lbm_fluid_env.py:441-533: poses out, fluid loads in as
+ * xfrc_applied and, accumulated, qfrc_applied; mjw.step) on chains such as
+ * envs/lbm/eel/eel_2d.xml (free root, five z hinges, motors with gears, joint
+ * damping and armature, RK4 at 0.01). Per world: joint-space inertia from the
+ * centre-of-mass Jacobians plus armature, bias forces at zero acceleration
+ * with gravity, passive damping and stiffness, motor forces gear *
+ * clamp(ctrl), the loads xfrc [nbody][6] (force at the centre of mass,
+ * torque) mapped through J^T at every stage plus applied_factor times that
+ * mapping at the step's start (the reference's qfrc_applied); MuJoCo's RK4
+ * (loads and controls held over the step) or semi-implicit Euler with
+ * implicit damping. A free joint is three dofs of its body: slides along
+ * world x and y, then a hinge about the body origin; its qpos0 is the body's
+ * position and angle. Bodies in tree order, each body's dofs consecutive. */
+#define HL_CHAIN_MAX_BODIES 16
+#define HL_CHAIN_MAX_DOFS 16
+#define HL_CHAIN_MAX_ACTS 16
+#define HL_CHAIN_SLIDE 0
+#define HL_CHAIN_HINGE 1
+#define HL_CHAIN_RK4 0
+#define HL_CHAIN_EULER 1
+
+typedef struct hl_chain_model {
+    int32_t nbody, ndof, nact, integrator;
+    int32_t parent[HL_CHAIN_MAX_BODIES];     /* -1: the world */
+    int32_t free_body[HL_CHAIN_MAX_BODIES];  /* 1: the body offset is absorbed in its dofs */
+    int32_t dof_body[HL_CHAIN_MAX_DOFS];
+    int32_t dof_type[HL_CHAIN_MAX_DOFS];     /* HL_CHAIN_SLIDE / HL_CHAIN_HINGE */
+    int32_t act_dof[HL_CHAIN_MAX_ACTS];
+    int32_t ctrl_limited[HL_CHAIN_MAX_ACTS];
+    double pos[HL_CHAIN_MAX_BODIES][2];      /* body offset in the parent frame */
+    double angle[HL_CHAIN_MAX_BODIES];       /* body rotation about z relative to the parent */
+    double com[HL_CHAIN_MAX_BODIES][2];      /* centre of mass, body frame */
+    double mass[HL_CHAIN_MAX_BODIES];
+    double inertia[HL_CHAIN_MAX_BODIES];     /* zz moment about the centre of mass */
+    double posz[HL_CHAIN_MAX_BODIES];        /* world z of the body origin */
+    double comz[HL_CHAIN_MAX_BODIES];        /* world z of the centre of mass */
+    double dof_axis[HL_CHAIN_MAX_DOFS][2];   /* slide axis in the frame before the joint */
+    double dof_anchor[HL_CHAIN_MAX_DOFS][2]; /* hinge anchor, body frame */
+    double dof_sign[HL_CHAIN_MAX_DOFS];      /* hinge axis +z (1) or -z (-1) */
+    double damping[HL_CHAIN_MAX_DOFS], stiffness[HL_CHAIN_MAX_DOFS], springref[HL_CHAIN_MAX_DOFS];
+    double armature[HL_CHAIN_MAX_DOFS];
+    double gear[HL_CHAIN_MAX_ACTS], ctrl_lo[HL_CHAIN_MAX_ACTS], ctrl_hi[HL_CHAIN_MAX_ACTS];
+    double gravity[2];
+    double dt;
+    double applied_factor; /* 1: the reference's loop (xfrc_applied and its accumulation both act) */
+} hl_chain_model;
+
+typedef struct hl_chain hl_chain;
+
+/* Validate a host model and keep a device copy. */
+int32_t hl_chain_create(const hl_chain_model* model, hl_chain** out);
+int32_t hl_chain_destroy(hl_chain* c);
+
+/* nsteps engine steps of nworld worlds. qpos, qvel: device FP64
+ * [nworld][ndof], advanced in place. ctrl: device FP32 [nworld][nact] or
+ * NULL (zero); xfrc: device FP32 [nworld][nbody][6] (fx, fy, fz, tx, ty, tz;
+ * the planar parts act) or NULL, both held over the call. Outputs after the
+ * last step (each may be NULL): qfrc_applied FP64 [nworld][ndof] (the last
+ * step's), and the body poses as MuJoCo's data keeps them, FP32: xpos and
+ * xipos [nworld][nbody][3] (body origin, centre of mass), xquat
+ * [nworld][nbody][4] (w, x, y, z). */
+int32_t hl_chain_step(hl_chain* c, int32_t nworld, double* qpos, double* qvel, const float* ctrl, const float* xfrc,
+                      int32_t nsteps, double* qfrc_applied, float* xpos, float* xipos, float* xquat, void* stream);
+
+/* ------------------------------------------------------------------------
  * Batches: many resident D2Q9 FP32 lattices, of uneven sizes, advanced
  * together with one launch per kernel kind per step (work-lists of
  * (environment, tile) items). Replaces the nworld dimension of LBM_Solver

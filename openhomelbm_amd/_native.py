@@ -23,11 +23,37 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
            "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-           "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status")
+           "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status",
+           "hl_chain_create", "hl_chain_destroy", "hl_chain_step")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
 HL_PROJECT_FIRST_SPEED, HL_PROJECT_MAX_SPEED, HL_PROJECT_MAX_VORTICITY = 0, 1, 2
+
+
+HL_CHAIN_MAX_BODIES = HL_CHAIN_MAX_DOFS = HL_CHAIN_MAX_ACTS = 16
+HL_CHAIN_SLIDE, HL_CHAIN_HINGE = 0, 1
+HL_CHAIN_RK4, HL_CHAIN_EULER = 0, 1
+
+
+class HLChainModel(C.Structure):
+    """Mirror of struct hl_chain_model."""
+
+    _NB, _NV, _NA = HL_CHAIN_MAX_BODIES, HL_CHAIN_MAX_DOFS, HL_CHAIN_MAX_ACTS
+    _fields_ = [
+        ("nbody", C.c_int32), ("ndof", C.c_int32), ("nact", C.c_int32), ("integrator", C.c_int32),
+        ("parent", C.c_int32 * _NB), ("free_body", C.c_int32 * _NB),
+        ("dof_body", C.c_int32 * _NV), ("dof_type", C.c_int32 * _NV),
+        ("act_dof", C.c_int32 * _NA), ("ctrl_limited", C.c_int32 * _NA),
+        ("pos", (C.c_double * 2) * _NB), ("angle", C.c_double * _NB), ("com", (C.c_double * 2) * _NB),
+        ("mass", C.c_double * _NB), ("inertia", C.c_double * _NB), ("posz", C.c_double * _NB),
+        ("comz", C.c_double * _NB),
+        ("dof_axis", (C.c_double * 2) * _NV), ("dof_anchor", (C.c_double * 2) * _NV), ("dof_sign", C.c_double * _NV),
+        ("damping", C.c_double * _NV), ("stiffness", C.c_double * _NV), ("springref", C.c_double * _NV),
+        ("armature", C.c_double * _NV),
+        ("gear", C.c_double * _NA), ("ctrl_lo", C.c_double * _NA), ("ctrl_hi", C.c_double * _NA),
+        ("gravity", C.c_double * 2), ("dt", C.c_double), ("applied_factor", C.c_double),
+    ]
 
 
 class HLEnv(C.Structure):
@@ -115,13 +141,17 @@ def load():
     L.hl_kernel_timer.argtypes = [I32, I32]
     L.hl_kernel_timer_read.argtypes = [C.POINTER(C.c_double), PI32]
     L.hl_copy_state.argtypes = [env_p, I32, P]
+    L.hl_chain_create.argtypes = [C.POINTER(HLChainModel), C.POINTER(C.c_void_p)]
+    L.hl_chain_destroy.argtypes = [P]
+    L.hl_chain_step.argtypes = [P, I32, P, P, P, P, I32, P, P, P, P, P]
     for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
                  "hl_set_tiling", "hl_get_tiling", "hl_batch_create", "hl_batch_set_bodies", "hl_batch_set_state",
                  "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
                  "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
-                 "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status"):
+                 "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status",
+                 "hl_chain_create", "hl_chain_destroy", "hl_chain_step"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

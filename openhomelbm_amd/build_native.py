@@ -13,7 +13,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIB_DIR, "libhomelbm.so")
-SOURCES = ["capi.hip", "step_gather.hip", "step_tiled.hip", "aux_kernels.hip", "batch.hip", "render.hip", "coupling.hip", "mesh.hip", "polygon.hip"]
+SOURCES = ["capi.hip", "step_gather.hip", "step_tiled.hip", "aux_kernels.hip", "batch.hip", "render.hip", "coupling.hip", "mesh.hip", "polygon.hip", "chain.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

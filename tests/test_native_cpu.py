@@ -154,3 +154,38 @@ def test_moment_layouts_are_validated(lattice, nz, rpitch, stride, force_stride,
         assert rc == code and msg in err, err
     else:
         assert rc == -1 and "cur must be" in err, err
+
+
+def test_chain_layout_and_invalid_models_rejected_before_any_gpu_call():
+    """hl_chain_create validates the model on the host before its device copy."""
+    import numpy as np
+
+    from openhomelbm_amd.chain import chain_from_mjcf
+
+    lib = N.load()
+    m = chain_from_mjcf(os.path.join(ROOT, "tests", "golden", "eel_2d.xml"))
+    good = m.to_struct()
+    assert C.sizeof(N.HLChainModel) == 3120  # 100 int32 + 340 double, no padding
+
+    def rejected(edit, msg):
+        s = m.to_struct()
+        edit(s)
+        h = C.c_void_p()
+        assert lib.hl_chain_create(C.byref(s), C.byref(h)) == -1 and not h.value
+        assert msg in lib.hl_last_error().decode()
+
+    rejected(lambda s: setattr(s, "nbody", 0), "1..16 bodies")
+    rejected(lambda s: setattr(s, "ndof", 17), "1..16 bodies")
+    rejected(lambda s: setattr(s, "integrator", 5), "integrator")
+    rejected(lambda s: setattr(s, "dt", 0.0), "dt must be positive")
+    rejected(lambda s: s.parent.__setitem__(2, 3), "tree order")
+    rejected(lambda s: s.dof_body.__setitem__(4, 0), "grouped by body")
+    rejected(lambda s: s.dof_sign.__setitem__(4, 0.5), "hinge sign")
+    rejected(lambda s: s.act_dof.__setitem__(0, 9), "actuator dof")
+    assert good.nbody == 6 and good.ndof == 8 and np.isclose(good.gear[4], 0.2)
+    with pytest.raises(NotImplementedError):
+        import tempfile
+        with tempfile.NamedTemporaryFile("w", suffix=".xml", delete=False) as f:
+            f.write('<mujoco><worldbody><body><joint type="hinge" axis="1 0 0"/><geom size="0.1"/></body>'
+                    '</worldbody></mujoco>')
+        chain_from_mjcf(f.name)
