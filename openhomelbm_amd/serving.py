@@ -4,16 +4,21 @@ requests from many resident swimmer environments, with continuous batching.
 * Every admitted environment keeps its lattice resident on the GPU. An
   environment is admitted when its HBM footprint fits the budget; otherwise it
   waits until released environments free enough memory.
-* The loop advances in ticks of one time step. Each tick takes every
-  environment whose oldest request still has steps left:
-  * all such D2Q9 environments advance in one batched launch sequence
-    (Batch.step with an active mask);
-  * each such D3Q27 environment advances on one of a few streams, so their
-    kernels overlap on the device.
+* The loop advances in ticks. Each tick takes every environment whose
+  oldest request still has steps left and advances all of them by m steps:
+  the fewest steps any of those requests still needs (at most
+  max_tick_steps, and within each environment's window of kinematics
+  tables):
+  * all such D2Q9 environments advance m steps in one call of the batched
+    engine (Batch.step with an active mask and an m-step device force
+    history);
+  * each such D3Q27 environment advances m steps on one of two side
+    streams, so their kernels overlap on the device.
   A request that arrives while others are running joins at the next tick.
 * A request completes when its k-th step has run. Its result is the FP64
-  force/torque history of its k steps, [k, nsolid, 6]. Latency runs from
-  arrival to when the host sees the completion event.
+  force/torque history of its k steps, [k, nsolid, 6], gathered on the
+  device (one copy per environment and tick) and read back once. Latency
+  runs from arrival to when the host sees the completion event.
 * Results do not depend on what else is in the batch. The batched D2Q9
   kernels give every environment the same arithmetic as stepping it alone, and
   each D3Q27 environment runs its own kernels, so a request's result is
@@ -76,7 +81,8 @@ class Request:
     done: bool = False
     finish: float = 0.0
     forces: Optional[np.ndarray] = None
-    _hist: List[torch.Tensor] = field(default_factory=list, repr=False)
+    _dev: Optional[torch.Tensor] = field(default=None, repr=False)  # [steps, nsolid, 6] on the device
+    _off: int = 0
 
     @property
     def latency(self) -> float:
@@ -84,7 +90,7 @@ class Request:
 
 
 class _Resident:
-    def __init__(self, spec: EnvSpec, device):
+    def __init__(self, spec: EnvSpec, device, max_tick: int = 16):
         self.spec = spec
         e = spec.eel
         n = spec.nsolid
@@ -105,7 +111,7 @@ class _Resident:
         if spec.lattice == 27:
             L.mask = torch.zeros(L.ncell, dtype=torch.uint8, device=L.device)
             L.mask_changed()
-            self.hist = torch.zeros((1, n, 6), dtype=torch.float64, device=L.device)
+            self.hist = torch.zeros((max_tick, 1, n, 6), dtype=torch.float64, device=L.device)
         self.queue: Deque[Request] = collections.deque()
 
     def refresh(self, t0: int) -> None:
@@ -129,9 +135,13 @@ class Server:
         streams: side streams over which D3Q27 environments are spread per
             tick (default 2: with the caller's stream that stays within the
             four hardware queues HIP gives a process).
+        max_tick_steps: the most steps one tick advances.
     """
 
-    def __init__(self, hbm_budget: float = 200e9, streams: int = 2, device=None):
+    def __init__(self, hbm_budget: float = 200e9, streams: int = 2, device=None, max_tick_steps: int = 16):
+        self.max_tick = int(max_tick_steps)
+        if self.max_tick < 1:
+            raise ValueError("max_tick_steps must be >= 1")
         self.device = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
         self.budget = float(hbm_budget)
         self.used = 0
@@ -161,7 +171,7 @@ class Server:
         admitted = False
         while self.pending and self.used + self.pending[0][1].hbm_bytes() <= self.budget:
             eid, spec = self.pending.popleft()
-            self.envs[eid] = _Resident(spec, self.device)
+            self.envs[eid] = _Resident(spec, self.device, self.max_tick)
             self.used += spec.hbm_bytes()
             admitted = True
         if admitted:
@@ -200,7 +210,7 @@ class Server:
             b.time[k] = r.time
         b._push_state()
         self._batch = b
-        self._bhist = b.history_buffer(1)
+        self._bhist = b.history_buffer(self.max_tick)
 
     # --------------------------------------------------------- requests
     def submit(self, eid: int, steps: int, arrival: Optional[float] = None) -> Request:
@@ -208,8 +218,10 @@ class Server:
             raise ValueError("steps must be >= 1")
         if eid not in self.envs and all(p[0] != eid for p in self.pending):
             raise KeyError(eid)
+        spec = self.envs[eid].spec if eid in self.envs else next(p[1] for p in self.pending if p[0] == eid)
         req = Request(env=eid, steps=steps, arrival=time.perf_counter() if arrival is None else arrival,
                       remaining=steps)
+        req._dev = torch.empty((steps, spec.nsolid, 6), dtype=torch.float64, device=self.device)
         self.requests.append(req)
         if eid in self.envs:
             self.envs[eid].queue.append(req)
@@ -228,58 +240,63 @@ class Server:
 
     # ------------------------------------------------------------- ticks
     def tick(self) -> int:
-        """Advance every environment that has work by one step; returns how many."""
+        """Advance every environment that has work by the tick's m steps;
+        returns how many environments advanced."""
         ready = self._ready()
         if not ready:
             return 0
         cur = torch.cuda.current_stream(self.device)
+        for i in ready:
+            r = self.envs[i]
+            if r.time >= r.t_end:  # next window of the kinematics tables
+                r.refresh(r.time)
+                if r.spec.lattice == 9:
+                    self._batch.set_kinematics(self._batch_ids.index(i), r.caps_np, r.poses_np, r.box)
+        m = min([self.max_tick] + [self.envs[i].queue[0].remaining for i in ready] +
+                [self.envs[i].t_end - self.envs[i].time for i in ready])
         done_now: List[Request] = []
-        hists = {}
-        # D2Q9: one batched launch sequence for all ready environments
+        # D2Q9: m steps of every ready environment in one batched call
         ready2 = [i for i in ready if self.envs[i].spec.lattice == 9]
         if ready2:
-            for i in ready2:
-                r = self.envs[i]
-                if r.time >= r.t_end:  # next window of the kinematics tables
-                    r.refresh(r.time)
-                    self._batch.set_kinematics(self._batch_ids.index(i), r.caps_np, r.poses_np, r.box)
             active = np.zeros(len(self._batch_ids), bool)
             for i in ready2:
                 active[self._batch_ids.index(i)] = True
-            self._batch.step(1, active=active, force_history=self._bhist, stream=cur)
+            self._batch.step(m, active=active, force_history=self._bhist, stream=cur)
             for i in ready2:
+                r = self.envs[i]
+                req = r.queue[0]
                 k = self._batch_ids.index(i)
-                hists[i] = self._bhist[0, k, : self.envs[i].spec.nsolid].clone()
-                self.cells_stepped += self.envs[i].spec.cells
-        # D3Q27: per environment, spread over streams
+                req._dev[req._off:req._off + m].copy_(self._bhist[:m, k, : r.spec.nsolid])
+                self.cells_stepped += r.spec.cells * m
+        # D3Q27: per environment, m steps, spread over the side streams
         ready3 = [i for i in ready if self.envs[i].spec.lattice == 27]
         if ready3:
             fork = torch.cuda.Event()
             fork.record(cur)
             for n, i in enumerate(ready3):
                 r = self.envs[i]
-                if r.time >= r.t_end:
-                    r.refresh(r.time)
+                req = r.queue[0]
                 s = self._streams[n % len(self._streams)]
                 s.wait_event(fork)
                 with torch.cuda.stream(s):
-                    j = r.time % r.spec.horizon
-                    r.lat.voxelize_device(r.caps[j], r.box, stream=s)
-                    r.lat.solids[: r.spec.nsolid].copy_(r.poses[j])
-                    r.lat.step(1, force_history=r.hist, stream=s)
-                    hists[i] = r.hist[0].clone()
-                hists[i].record_stream(cur)  # read on the main stream later
-                self.cells_stepped += r.spec.cells
+                    for q in range(m):
+                        j = (r.time + q) % r.spec.horizon
+                        r.lat.voxelize_device(r.caps[j], r.box, stream=s)
+                        r.lat.solids[: r.spec.nsolid].copy_(r.poses[j])
+                        r.lat.step(1, force_history=r.hist[q], stream=s)
+                    req._dev[req._off:req._off + m].copy_(r.hist[:m, 0])
+                req._dev.record_stream(s)
+                self.cells_stepped += r.spec.cells * m
             for s in self._streams[: len(ready3)]:
                 j = torch.cuda.Event()
                 j.record(s)
                 cur.wait_event(j)
         for i in ready:
             r = self.envs[i]
-            r.time += 1
+            r.time += m
             req = r.queue[0]
-            req._hist.append(hists[i])
-            req.remaining -= 1
+            req._off += m
+            req.remaining -= m
             if req.remaining == 0:
                 r.queue.popleft()
                 done_now.append(req)
@@ -293,8 +310,8 @@ class Server:
     def _finish(self, reqs):
         t = time.perf_counter()
         for req in reqs:
-            req.forces = torch.stack(req._hist).cpu().numpy()
-            req._hist = []
+            req.forces = req._dev.cpu().numpy()
+            req._dev = None
             req.finish = t
             req.done = True
 
@@ -389,7 +406,7 @@ def poisson_arrivals(rate: float, duration: float, env_ids: Sequence[int], seed:
         out.append((t, int(rng.choice(env_ids)), int(rng.choice(ks))))
 
 
-def sweep(rates: Sequence[float] = (250.0, 500.0, 1000.0, 2000.0), duration: float = 1.0, n_envs: int = 32,
+def sweep(rates: Sequence[float] = (250.0, 500.0, 1000.0, 2000.0, 4000.0), duration: float = 1.0, n_envs: int = 32,
           hbm_budget: float = 200e9, seed: int = 5005, log=None) -> dict:
     """Serve Poisson request streams at each rate over one resident pool;
     report throughput and p50/p99 request latency per rate."""
