@@ -22,7 +22,12 @@
  * Field order:
  *   D2Q9  : rho, ux, uy, Sxx, Syy, Sxy
  *   D3Q27 : rho, ux, uy, uz, Sxx, Syy, Szz, Sxy, Sxz, Syz
- * 16-bit storage holds (rho - 1)*scale[0] and value*scale[f] for f > 0.
+ * 16-bit storage holds (rho - 1 - offset[0])*scale[0] and
+ * (value - offset[f])*scale[f] for f > 0 (offsets of zero: rho - 1 and the raw
+ * values). FP16 rounds to nearest even; BF16 rounds stochastically (up with
+ * the probability of the discarded fraction, the random bits a hash of the
+ * element index and the value: deterministic, unbiased, so updates smaller
+ * than half a BF16 ulp are not lost).
  */
 #ifndef HOMELBM_H
 #define HOMELBM_H
@@ -71,6 +76,7 @@ typedef struct hl_env {
     int32_t bc_type[6];   /* 0 velocity, 1 zero gradient, 2 free slip */
     float bc_value[6][3]; /* face velocity for type 0 */
     float scale[10];      /* 16-bit storage scale per field */
+    float offset[10];     /* 16-bit storage offset per field (see above) */
     uint64_t mask_epoch;  /* 0, or a caller-kept version of mask + mask_box: hl_step calls on one
                              stream with the same nonzero epoch, mask and box reuse the near-body
                              list the first of them built (a new epoch whenever the mask changes) */

@@ -73,6 +73,7 @@ class HLEnv(C.Structure):
         ("bc_type", C.c_int32 * 6),
         ("bc_value", (C.c_float * 3) * 6),
         ("scale", C.c_float * 10),
+        ("offset", C.c_float * 10),
         ("mask_epoch", C.c_uint64),
         ("rpitch", C.c_int64),
         ("force_stride", C.c_int64),

@@ -113,11 +113,7 @@ class LatticeField:
         comps = []
         for f in self.fields:
             t = raw[f]
-            if L.dtype in ("f16", "bf16"):
-                t = t.float() / L.scale[f]
-                if f == 0:
-                    t = t + 1.0
-            comps.append(self._xyz(t))
+            comps.append(self._xyz(L.decode(f, t)))
         return comps[0] if len(comps) == 1 else torch.stack(comps, dim=-1)
 
     def numpy(self) -> np.ndarray:
@@ -135,9 +131,7 @@ class LatticeField:
         for k, f in enumerate(self.fields):
             c = v[..., k] if len(self.fields) > 1 else v
             c = c.permute(*reversed(range(c.dim())))
-            if L.dtype in ("f16", "bf16"):
-                c = (c - (1.0 if f == 0 else 0.0)) * L.scale[f]
-            raw[f].copy_(c.to(device=raw.device, dtype=raw.dtype))
+            raw[f].copy_(L.encode(f, c.to(device=raw.device)))
 
     def fill_(self, value) -> None:
         shape = tuple(reversed(self.lattice.shape)) + ((len(self.fields),) if len(self.fields) > 1 else ())

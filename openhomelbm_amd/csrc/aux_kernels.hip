@@ -34,8 +34,9 @@ __global__ void k_fill(void* buf, int64_t stride, int64_t n, int nf, EnvArgs a, 
     for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t so = cell_of(a, c);
         for (int f = 0; f < nf; ++f) {
-            if (DT == HL_F64) Storage<DT>::template store<double>(p, f * stride + so, m[f], f, a.scale[f]);
-            else Storage<DT>::template store<float>(p, f * stride + so, (float)m[f], f, a.scale[f]);
+            const uint32_t key = (uint32_t)f * (uint32_t)n + (uint32_t)c;
+            if (DT == HL_F64) Storage<DT>::template store<double>(p, f * stride + so, m[f], f, a.scale[f], a.offset[f], key);
+            else Storage<DT>::template store<float>(p, f * stride + so, (float)m[f], f, a.scale[f], a.offset[f], key);
         }
     }
 }
@@ -396,14 +397,18 @@ __global__ void __launch_bounds__(64) k_fixup(EnvArgs a, Region rg, const int* l
                 d2q9_collide<T>(a, pops[h], Fx, Fy, m);
                 if (px != x || py != y) bc_chain<T, 2>(m, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
 #pragma unroll
-                for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+                for (int f = 0; f < 6; ++f)
+                    Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f], a.offset[f],
+                                                   HL_ROUNDING_KEY(a, f, x, y, z));
             } else {
                 T Fx = 0, Fy = 0, Fz = 0, m[10];
                 if (a.force) { Fx = T(a.force[c]); Fy = T(a.force[a.fs + c]); Fz = T(a.force[2 * a.fs + c]); }
                 d3q27_collide_sums<T>(a, sums[h], Fx, Fy, Fz, m);
                 if (px != x || py != y || pz != z) bc_chain<T, 3>(m, x, y, z, nx, ny, nz, a.bc_type, a.bc_value);
 #pragma unroll
-                for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+                for (int f = 0; f < 10; ++f)
+                    Storage<DT>::template store<T>(dst, f * a.stride + d, m[f], f, a.scale[f], a.offset[f],
+                                                   HL_ROUNDING_KEY(a, f, x, y, z));
             }
         }
         if (partial) wave_solid_sums(sid, F, kRow6, lane, acc);
@@ -586,7 +591,7 @@ __global__ void __launch_bounds__(kReduceBlock) k_mass(EnvArgs a, int64_t n, dou
     const ST* src = static_cast<const ST*>(a.src);
     double v = 0.0;
     for (int64_t c = blockIdx.x * (int64_t)kReduceBlock + threadIdx.x; c < n; c += (int64_t)gridDim.x * kReduceBlock)
-        v += Storage<DT>::template load<double>(src, cell_of(a, c), 0, a.inv_scale[0]);
+        v += Storage<DT>::template load<double>(src, cell_of(a, c), 0, a.inv_scale[0], a.offset[0]);
     v = wave_sum(v);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();

@@ -76,9 +76,11 @@ int32_t validate(const hl_env* e, bool need_moments = true)
             return fail(HL_E_INVALID, "mask_box (%d,%d,%d,%d,%d,%d) outside the grid", b[0], b[1], b[2], b[3], b[4], b[5]);
     }
     if (e->dtype == HL_F16 || e->dtype == HL_BF16)
-        for (int f = 0; f < nfields(e->lattice); ++f)
+        for (int f = 0; f < nfields(e->lattice); ++f) {
             if (!(e->scale[f] > 0.0f) || !std::isfinite(e->scale[f]))
                 return fail(HL_E_INVALID, "16-bit storage needs scale[%d] > 0", f);
+            if (!std::isfinite(e->offset[f])) return fail(HL_E_INVALID, "16-bit storage needs a finite offset[%d]", f);
+        }
     return HL_OK;
 }
 
@@ -106,6 +108,7 @@ hl::EnvArgs make_args(const hl_env* e, int cur)
         const bool sixteen = e->dtype == HL_F16 || e->dtype == HL_BF16;
         a.scale[f] = sixteen ? e->scale[f] : 1.0f;
         a.inv_scale[f] = sixteen ? 1.0f / e->scale[f] : 1.0f;
+        a.offset[f] = sixteen ? (f == 0 ? 1.0f : 0.0f) + e->offset[f] : 0.0f;
     }
     a.region[0] = 0; a.region[1] = e->nx; a.region[2] = 0; a.region[3] = e->ny; a.region[4] = 0; a.region[5] = e->nz;
     return a;

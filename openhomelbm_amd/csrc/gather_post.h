@@ -52,7 +52,7 @@ __device__ __forceinline__ T d2q9_pull(const EnvArgs& a, int x, int y, int i, co
     T m[6];
     if (!cut) {
 #pragma unroll
-        for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f]);
+        for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * st + s, f, a.inv_scale[f], a.offset[f]);
     }
     return d2q9_link_pop<T>(a, x, y, i, mc, cut, sid, m);
 }
@@ -94,7 +94,7 @@ __device__ __forceinline__ void load_cell(const EnvArgs& a, int64_t so, int nf, 
 {
     using S = Storage<DT>;
     const typename S::type* src = static_cast<const typename S::type*>(a.src);
-    for (int f = 0; f < nf; ++f) mc[f] = S::template load<T>(src, f * a.stride + so, f, a.inv_scale[f]);
+    for (int f = 0; f < nf; ++f) mc[f] = S::template load<T>(src, f * a.stride + so, f, a.inv_scale[f], a.offset[f]);
 }
 
 template <typename T, int DT>
@@ -103,7 +103,7 @@ __device__ __forceinline__ void d2q9_post(const EnvArgs& a, int x, int y, T out[
     const int64_t c = (int64_t)x + (int64_t)a.nx * y;
     T mc[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) mc[f] = Storage<DT>::template load<T>(static_cast<const typename Storage<DT>::type*>(a.src), f * a.stride + c, f, a.inv_scale[f]);
+    for (int f = 0; f < 6; ++f) mc[f] = Storage<DT>::template load<T>(static_cast<const typename Storage<DT>::type*>(a.src), f * a.stride + c, f, a.inv_scale[f], a.offset[f]);
     const bool masked = a.mask != nullptr && in_box_halo(a, x, y, 0);
     const int mval_c = masked ? a.mask[c] : 0;
     T pop[9];
@@ -142,7 +142,7 @@ __device__ __forceinline__ T d3q27_pull(const EnvArgs& a, int x, int y, int z, i
     // (the source cell is inside the lattice either way)
     T m[10];
 #pragma unroll
-    for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + so, f, a.inv_scale[f]);
+    for (int f = 0; f < 10; ++f) m[f] = S::template load<T>(src, f * st + so, f, a.inv_scale[f], a.offset[f]);
     int cut = 0, sid = -1;
     // the wall point: the link midpoint under the mask rule, the crossing
     // x - c_i t (lbm_func_3d.py:446) with capsule bodies
@@ -384,7 +384,8 @@ __device__ __forceinline__ void gather_d2q9_at(const EnvArgs& a, int x, int y)
     ST* dst = static_cast<ST*>(a.dst);
     const int64_t c = (int64_t)x + (int64_t)a.nx * y;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+    for (int f = 0; f < 6; ++f)
+        Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f], a.offset[f], HL_ROUNDING_KEY(a, f, x, y, 0));
 }
 
 // Cell (region x0 + bx, y0 + by) of a D2Q9 gather step.

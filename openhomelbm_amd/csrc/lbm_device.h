@@ -44,55 +44,79 @@ template <typename T> __host__ __device__ constexpr T cs2_3d() { return T(1.0 / 
 // ------------------------------------------------------------ storage
 
 // Moment codecs. 32/64-bit fields are stored as is. 16-bit fields hold
-// (rho - 1) * scale for the density and value * scale for every other field;
-// decode multiplies by 1/scale (precomputed by the host).
+// (value - offset) * scale, offset and scale per field (the density's offset
+// includes its 1); decode multiplies by 1/scale and adds the offset (both
+// precomputed by the host). FP16 rounds to nearest even. BF16 rounds
+// stochastically: up with the probability of the discarded 16-bit fraction,
+// the random bits a hash of the element index and the value, so the rounding
+// is deterministic and unbiased and increments below half a BF16 ulp survive
+// on average instead of being lost every step.
 template <int DT> struct Storage;
 template <> struct Storage<HL_F32> {
     using type = float;
-    template <typename T> __device__ static T load(const float* p, int64_t i, int, float) { return T(p[i]); }
-    template <typename T> __device__ static void store(float* p, int64_t i, T v, int, float) { p[i] = float(v); }
+    template <typename T> __device__ static T load(const float* p, int64_t i, int, float, float) { return T(p[i]); }
+    template <typename T> __device__ static void store(float* p, int64_t i, T v, int, float, float, uint32_t)
+    {
+        p[i] = float(v);
+    }
 };
 template <> struct Storage<HL_F64> {
     using type = double;
-    template <typename T> __device__ static T load(const double* p, int64_t i, int, float) { return T(p[i]); }
-    template <typename T> __device__ static void store(double* p, int64_t i, T v, int, float) { p[i] = double(v); }
+    template <typename T> __device__ static T load(const double* p, int64_t i, int, float, float) { return T(p[i]); }
+    template <typename T> __device__ static void store(double* p, int64_t i, T v, int, float, float, uint32_t)
+    {
+        p[i] = double(v);
+    }
 };
 template <> struct Storage<HL_F16> {
     using type = __half;
-    template <typename T> __device__ static T load(const __half* p, int64_t i, int f, float inv_scale)
+    template <typename T> __device__ static T load(const __half* p, int64_t i, int, float inv_scale, float offset)
     {
-        float v = __half2float(p[i]) * inv_scale;
-        return T(f == 0 ? v + 1.0f : v);
+        return T(__half2float(p[i]) * inv_scale + offset);
     }
-    template <typename T> __device__ static void store(__half* p, int64_t i, T v, int f, float scale)
+    template <typename T>
+    __device__ static void store(__half* p, int64_t i, T v, int, float scale, float offset, uint32_t)
     {
-        float w = f == 0 ? float(v) - 1.0f : float(v);
-        p[i] = __float2half_rn(w * scale);
+        p[i] = __float2half_rn((float(v) - offset) * scale);
     }
 };
-// bfloat16 as raw 16-bit words: round-to-nearest-even on encode.
-__device__ __forceinline__ uint16_t f32_to_bf16(float v)
+// bfloat16 as raw 16-bit words. key: the logical element index
+// f * ncell + x + nx (y + ny z) (rounding_key), the same in every kernel,
+// layout and tiling. The hash is a 32-bit integer mix (xor-shift
+// and odd multipliers) of key ^ value bits; its low 16 bits are the random
+// fraction added before truncation.
+__device__ __forceinline__ uint16_t f32_to_bf16_sr(float v, uint32_t key)
 {
     uint32_t u = __float_as_uint(v);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);
+    uint32_t h = key ^ u;
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    u += h & 0xffffu;  // carries into the kept bits with probability fraction / 2^16
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 template <> struct Storage<HL_BF16> {
     using type = uint16_t;
-    template <typename T> __device__ static T load(const uint16_t* p, int64_t i, int f, float inv_scale)
+    template <typename T> __device__ static T load(const uint16_t* p, int64_t i, int, float inv_scale, float offset)
     {
-        float v = bf16_to_f32(p[i]) * inv_scale;
-        return T(f == 0 ? v + 1.0f : v);
+        return T(bf16_to_f32(p[i]) * inv_scale + offset);
     }
-    template <typename T> __device__ static void store(uint16_t* p, int64_t i, T v, int f, float scale)
+    template <typename T>
+    __device__ static void store(uint16_t* p, int64_t i, T v, int, float scale, float offset, uint32_t key)
     {
-        float w = f == 0 ? float(v) - 1.0f : float(v);
-        p[i] = f32_to_bf16(w * scale);
+        p[i] = f32_to_bf16_sr((float(v) - offset) * scale, key);
     }
 };
+
+// BF16 rounding key of field f at cell (x, y, z): its logical element index.
+#define HL_ROUNDING_KEY(a, f, x, y, z)                                                                         \
+    ((uint32_t)(f) * ((uint32_t)(a).nx * (uint32_t)(a).ny * (uint32_t)(a).nz) + (uint32_t)(x) +                  \
+     (uint32_t)(a).nx * ((uint32_t)(y) + (uint32_t)(a).ny * (uint32_t)(z)))
 
 // ------------------------------------------------------- reconstruction
 
@@ -539,6 +563,7 @@ struct EnvArgs {
     float bc_value[6][3];
     float scale[10];
     float inv_scale[10];
+    float offset[10];  // 16-bit storage offset (the density's includes its 1)
     int32_t region[6];  // cells the gather kernel updates: x0, x1, y0, y1, z0, z1 (half-open)
     // cells whose update the near-body fix-up owns: bit t of cut_bits marks
     // region cell t = (x - cx0) + cw * ((y - cy0) + ch * (z - cz0)); null = none

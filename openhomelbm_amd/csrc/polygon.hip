@@ -306,7 +306,7 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
                 const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
                 T m[6];
 #pragma unroll
-                for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f], a.offset[f]);
                 T q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
                 if (q < T(0)) q = T(0);
                 pp[i] = q;
@@ -355,7 +355,8 @@ __device__ __forceinline__ void poly_band_cells(const EnvArgs& a, const PolyArgs
             if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
             const int64_t d = (int64_t)x + (int64_t)nx * y;
 #pragma unroll
-            for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+            for (int f = 0; f < 6; ++f)
+                S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f], a.offset[f], HL_ROUNDING_KEY(a, f, x, y, 0));
         }
     }
 }
@@ -445,7 +446,7 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
                     const int64_t s = (int64_t)(px - D2CX[i]) + (int64_t)nx * (py - D2CY[i]);
                     T m[6];
 #pragma unroll
-                    for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f]);
+                    for (int f = 0; f < 6; ++f) m[f] = S::template load<T>(src, f * a.stride + s, f, a.inv_scale[f], a.offset[f]);
                     q = d2q9_recon<T>(m[0], m[1], m[2], m[3], m[4], m[5], i);
                     if (q < T(0)) q = T(0);
                 }
@@ -468,7 +469,8 @@ __device__ __forceinline__ void poly_band(const EnvArgs& a, const PolyArgs& p, d
             if (!interior) bc_chain<T, 2>(out, x, y, 0, nx, ny, 1, a.bc_type, a.bc_value);
             const int64_t d = (int64_t)x + (int64_t)nx * y;
 #pragma unroll
-            for (int f = 0; f < 6; ++f) S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f]);
+            for (int f = 0; f < 6; ++f)
+                S::template store<T>(dst, f * a.stride + d, out[f], f, a.scale[f], a.offset[f], HL_ROUNDING_KEY(a, f, x, y, 0));
         }
         if (partial) wave_solid_sums(sid, F, kRow2D, lane, acc);
         // the lead lanes' reads of pops end before the next pass writes them

@@ -39,7 +39,7 @@ __device__ __forceinline__ float vel(const EnvArgs& a, int64_t c, int comp)
     using S = Storage<DT>;
     using ST = typename S::type;
     const ST* src = static_cast<const ST*>(a.src);
-    return S::template load<float>(src, (int64_t)(1 + comp) * a.stride + cell_of(a, c), 1 + comp, a.inv_scale[1 + comp]);
+    return S::template load<float>(src, (int64_t)(1 + comp) * a.stride + cell_of(a, c), 1 + comp, a.inv_scale[1 + comp], a.offset[1 + comp]);
 }
 
 __device__ __forceinline__ int64_t cell(const EnvArgs& a, int x, int y, int z)

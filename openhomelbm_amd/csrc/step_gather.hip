@@ -40,7 +40,8 @@ __global__ void __launch_bounds__(256) k_gather_d3q27(EnvArgs a)
     ST* dst = static_cast<ST*>(a.dst);
     const int64_t c = cell_at(a, x, y, z);
 #pragma unroll
-    for (int f = 0; f < 10; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+    for (int f = 0; f < 10; ++f)
+        Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f], a.offset[f], HL_ROUNDING_KEY(a, f, x, y, z));
 }
 
 // D2Q9 gather step that also produces the body forces of the state it reads
@@ -114,7 +115,8 @@ __global__ void __launch_bounds__(256) k_gather_d2q9_fx(EnvArgs a, FixArgs fx)
     ST* dst = static_cast<ST*>(a.dst);
     const int64_t c = (int64_t)x + (int64_t)a.nx * y;
 #pragma unroll
-    for (int f = 0; f < 6; ++f) Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f]);
+    for (int f = 0; f < 6; ++f)
+        Storage<DT>::template store<T>(dst, f * a.stride + c, m[f], f, a.scale[f], a.offset[f], HL_ROUNDING_KEY(a, f, x, y, 0));
 }
 
 template <typename T, int DT>

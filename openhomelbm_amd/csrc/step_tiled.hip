@@ -226,11 +226,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uin
 template <int DT> struct BufIO;
 template <> struct BufIO<HL_F32> {
     static constexpr uint32_t size = 4;
-    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int, float)
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int, float, float)
     {
         return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)v, (int)s, 0));
     }
-    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int, float)
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int, float, float, uint32_t)
     {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, (int)v, (int)s, kNT);
     }
@@ -238,30 +238,28 @@ template <> struct BufIO<HL_F32> {
 // 16-bit storage: the encodings of Storage<HL_F16> / Storage<HL_BF16>
 template <> struct BufIO<HL_F16> {
     static constexpr uint32_t size = 2;
-    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int f, float inv_scale)
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int, float inv_scale, float offset)
     {
         const uint16_t h = __builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)s, 0);
-        const float x = (float)__builtin_bit_cast(_Float16, h) * inv_scale;
-        return f == 0 ? x + 1.0f : x;
+        return (float)__builtin_bit_cast(_Float16, h) * inv_scale + offset;
     }
-    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int f, float scale)
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int, float scale,
+                                 float offset, uint32_t)
     {
-        const float w = f == 0 ? x - 1.0f : x;
-        const _Float16 h = (_Float16)(w * scale);
+        const _Float16 h = (_Float16)((x - offset) * scale);
         __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, h), r, (int)v, (int)s, kNT);
     }
 };
 template <> struct BufIO<HL_BF16> {
     static constexpr uint32_t size = 2;
-    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int f, float inv_scale)
+    __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, int, float inv_scale, float offset)
     {
-        const float x = bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)s, 0)) * inv_scale;
-        return f == 0 ? x + 1.0f : x;
+        return bf16_to_f32(__builtin_amdgcn_raw_buffer_load_b16(r, (int)v, (int)s, 0)) * inv_scale + offset;
     }
-    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int f, float scale)
+    __device__ static void store(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s, float x, int, float scale,
+                                 float offset, uint32_t key)
     {
-        const float w = f == 0 ? x - 1.0f : x;
-        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16(w * scale), r, (int)v, (int)s, kNT);
+        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16_sr((x - offset) * scale, key), r, (int)v, (int)s, kNT);
     }
 };
 
@@ -355,7 +353,9 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
             ST* dst = static_cast<ST*>(a.dst);
             const int64_t d = cell_at(a, x, y, z);
 #pragma unroll
-            for (int f = 0; f < 10; ++f) Storage<DT>::template store<float>(dst, f * a.stride + d, m[f], f, a.scale[f]);
+            for (int f = 0; f < 10; ++f)
+                Storage<DT>::template store<float>(dst, f * a.stride + d, m[f], f, a.scale[f], a.offset[f],
+                                                   HL_ROUNDING_KEY(a, f, x, y, z));
         }
         if (width) wave_solid_sums(sid, F, kRow6, lane, acc);
         wave_sync();  // pops and sums are rewritten by the next pass
@@ -459,17 +459,20 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     auto load_plane = [&](int z, float* m) {
         const uint32_t sz = (uint32_t)z * pbytes;
 #pragma unroll
-        for (int f = 0; f < 10; ++f) m[f] = IO::load(rsrc, vsrc, sz + f * fbytes, f, a.inv_scale[f]);
+        for (int f = 0; f < 10; ++f) m[f] = IO::load(rsrc, vsrc, sz + f * fbytes, f, a.inv_scale[f], a.offset[f]);
     };
     auto load_force = [&](int z, float* F) {
         const uint32_t sz = (uint32_t)z * (uint32_t)(nx * ny) * 4;
 #pragma unroll
-        for (int q = 0; q < 3; ++q) F[q] = BufIO<HL_F32>::load(rfor, vfor, sz + q * (uint32_t)a.fs * 4, 0, 1.0f);
+        for (int q = 0; q < 3; ++q) F[q] = BufIO<HL_F32>::load(rfor, vfor, sz + q * (uint32_t)a.fs * 4, 0, 1.0f, 0.0f);
     };
+    // BF16 rounding key of this thread's destination cell in plane 0
+    const uint32_t kcell = (uint32_t)dx + (uint32_t)nx * (uint32_t)dy, kplane = (uint32_t)nx * (uint32_t)ny;
     auto store_plane = [&](int z, uint32_t v, const float* m) {
-        const uint32_t sz = (uint32_t)z * pbytes;
+        const uint32_t sz = (uint32_t)z * pbytes, k0 = kcell + (uint32_t)z * kplane;
 #pragma unroll
-        for (int f = 0; f < 10; ++f) IO::store(rdst, v, sz + f * fbytes, m[f], f, a.scale[f]);
+        for (int f = 0; f < 10; ++f)
+            IO::store(rdst, v, sz + f * fbytes, m[f], f, a.scale[f], a.offset[f], k0 + (uint32_t)f * kplane * (uint32_t)nz);
     };
 
     // up pops of the last two source planes, alternating by plane parity so

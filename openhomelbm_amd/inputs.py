@@ -110,11 +110,8 @@ def build_lattice(w: Workload, dtype: str = "f32"):
     d = noise.shape[0]
     for b in (0, 1):
         f = lat.fields(b)
-        if dtype in ("f16", "bf16"):
-            cur = f[1:1 + d].float() / lat.scale[1]
-            f[1:1 + d].copy_(((cur + noise) * lat.scale[1]).to(f.dtype))
-        else:
-            f[1:1 + d] += noise.to(f.dtype)
+        for k in range(d):
+            f[1 + k].copy_(lat.encode(1 + k, lat.decode(1 + k, f[1 + k]) + noise[k]))
     return lat
 
 

@@ -140,6 +140,10 @@ class Lattice:
         if scale is None:
             scale = DEFAULT_SCALE_2D if lattice == 9 else DEFAULT_SCALE_3D
         self.scale = list(scale) + [1.0] * (10 - len(scale))
+        #: 16-bit storage offsets: field f is stored as (value - offset[f]) *
+        #: scale[f], the density as (rho - 1 - offset[0]) * scale[0];
+        #: init_uniform sets the velocity offsets to the uniform flow
+        self.offset = [0.0] * 10
         self._forces_out = torch.zeros((max(nsolid, 1), 6), dtype=torch.float64, device=self.device)
         self._mass_out = torch.zeros(1, dtype=torch.float64, device=self.device)
 
@@ -189,6 +193,7 @@ class Lattice:
                 e.bc_value[k][a] = float(v[a]) if a < len(v) else 0.0
         for f in range(10):
             e.scale[f] = float(self.scale[f])
+            e.offset[f] = float(self.offset[f])
         return e
 
     def _step_env(self) -> N.HLEnv:
@@ -199,7 +204,7 @@ class Lattice:
         key = (self.buf.data_ptr(), self.mask.data_ptr() if self.mask is not None else 0, self._mask_epoch,
                tuple(self.mask_box), self.force.data_ptr() if self.force is not None else 0, self.solids.data_ptr(),
                self.nsolid, float(self.nu), tuple(self.bc_type), tuple(tuple(v) for v in self.bc_value),
-               tuple(self.scale))
+               tuple(self.scale), tuple(self.offset))
         if key != self._env_key:
             self._env_cache = self.env()
             self._env_key = key
@@ -214,6 +219,22 @@ class Lattice:
         return self.buf[b, :, : self.ncell].view((self.nfields,) + self.shape)
 
     # ------------------------------------------------------------- state
+    def _field_offset(self, f: int) -> float:
+        return (1.0 if f == 0 else 0.0) + float(self.offset[f])
+
+    def decode(self, f: int, raw: torch.Tensor) -> torch.Tensor:
+        """Values of field f from its stored form (16-bit: raw / scale +
+        offset, the density's offset including its 1; else as stored)."""
+        if self.dtype not in ("f16", "bf16"):
+            return raw
+        return raw.float() / self.scale[f] + self._field_offset(f)
+
+    def encode(self, f: int, values: torch.Tensor) -> torch.Tensor:
+        """Stored form of field f (the inverse of decode), in the storage dtype."""
+        if self.dtype not in ("f16", "bf16"):
+            return values.to(self.buf.dtype)
+        return ((values.float() - self._field_offset(f)) * self.scale[f]).to(self.buf.dtype)
+
     def set_state(self, m: np.ndarray) -> None:
         """Load host moments (oracle layout) into the current state (both buffers)."""
         m = np.asarray(m)
@@ -221,10 +242,10 @@ class Lattice:
             raise ValueError(f"state shape {m.shape} != {(self.nfields,) + self.shape}")
         t = torch.from_numpy(np.ascontiguousarray(m, dtype=np.float64))
         if self.dtype in ("f16", "bf16"):
-            t = t.clone()
-            t[0] -= 1.0
-            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64).view((-1,) + (1,) * len(self.shape))
-            t = (t * sc).to(torch.float32)
+            off = torch.tensor([self._field_offset(f) for f in range(self.nfields)], dtype=torch.float64)
+            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64)
+            shp = (-1,) + (1,) * len(self.shape)
+            t = ((t - off.view(shp)) * sc.view(shp)).to(torch.float32)
         dev = t.to(device=self.device, dtype=self.buf.dtype)
         for b in (0, 1):
             self.fields(b).copy_(dev.reshape((self.nfields,) + self.shape))
@@ -233,14 +254,22 @@ class Lattice:
         """Host copy of the current moments, decoded, in oracle layout."""
         t = self.fields().to(torch.float64 if self.dtype == "f64" else torch.float32).cpu().to(torch.float64)
         if self.dtype in ("f16", "bf16"):
-            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64).view((-1,) + (1,) * len(self.shape))
-            t = t / sc
-            t[0] += 1.0
+            shp = (-1,) + (1,) * len(self.shape)
+            sc = torch.tensor(self.scale[: self.nfields], dtype=torch.float64).view(shp)
+            off = torch.tensor([self._field_offset(f) for f in range(self.nfields)], dtype=torch.float64).view(shp)
+            t = t / sc + off
         return t.numpy().astype(dtype)
 
-    def init_uniform(self, u: Sequence[float] = (0.0, 0.0, 0.0)) -> None:
-        """set_uniform_flow into both buffers (karman_lbm_env.py:15-53)."""
+    def init_uniform(self, u: Sequence[float] = (0.0, 0.0, 0.0), set_offset: bool = True) -> None:
+        """set_uniform_flow into both buffers (karman_lbm_env.py:15-53). With
+        16-bit storage and set_offset, the velocity fields are then stored as
+        their deviation from u (offset[1:4] = u), which keeps the fine end of
+        the 16-bit formats where the flow departs from the uniform state."""
         u = list(u) + [0.0] * (3 - len(u))
+        if set_offset and self.dtype in ("f16", "bf16"):
+            d = 2 if self.lattice == 9 else 3
+            for k in range(d):
+                self.offset[1 + k] = float(u[k])
         e = self.env()
         for b in (0, 1):
             N.check(self.lib.hl_init_uniform(C.byref(e), b, u[0], u[1], u[2], _stream_ptr()), "hl_init_uniform")

@@ -1,6 +1,7 @@
 """Precision sweep: the same D3Q27 run with moments stored in FP64 (the GPU
 oracle: the gather kernel in FP64), FP32, FP16 and BF16 (16-bit storage of
-rho - 1 and per-field scaled u and S, collision in FP32).
+rho - 1, u - U (the inlet flow) and S, each scaled per field; FP16 rounded to
+nearest, BF16 stochastically; collision in FP32).
 
 Every reported error is against the FP64 run of the same inputs: relative L2
 of the velocity field after the last step, and the relative error of the
@@ -39,11 +40,8 @@ def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float
     noise = noise.to(L.device)
     for b in (0, 1):
         f = L.fields(b)
-        if dtype in ("f16", "bf16"):
-            cur = f[1:4].float() / L.scale[1]
-            f[1:4].copy_(((cur + noise) * L.scale[1]).to(f.dtype))
-        else:
-            f[1:4] += noise.to(f.dtype)
+        for k in range(3):
+            f[1 + k].copy_(L.encode(1 + k, L.decode(1 + k, f[1 + k]) + noise[k]))
     caps, poses = eel.tables(0, steps)
     box = eel.box(0, steps)
     caps_d = torch.from_numpy(caps).to(L.device)
@@ -58,9 +56,8 @@ def run(eel: "inputs.Eel3D", dtype: str, steps: int, seed: int = 4004, re: float
             L.step(1, force_history=hist[t:t + 1])
         if log and (t + 1) % 1000 == 0:
             log(f"[precision] {dtype}: step {t + 1}/{steps}")
-    u = L.fields()[1:4].to(torch.float64)
-    if dtype in ("f16", "bf16"):
-        u = u / L.scale[1]
+    raw = L.fields()
+    u = torch.stack([L.decode(1 + k, raw[1 + k]).to(torch.float64) for k in range(3)])
     drag = hist[:, :, 0].sum(1)
     if not bool(torch.isfinite(u).all()):
         raise RuntimeError(f"{dtype} run diverged")
