@@ -58,9 +58,24 @@ hipError_t init_uniform(int lattice, int dtype, const EnvArgs& a, void* buf, con
 
 // ---------------------------------------------------------- state copy
 
-__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n)
+// Streaming copy of 16-byte words: four non-temporal loads in flight per
+// lane before their stores, over a large grid (the fastest of the shapes
+// measured for a whole-state copy on MI355X: 6.4 TB/s, against 5.3-5.9 TB/s
+// with one load in flight per lane).
+typedef unsigned int copy_word __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy16(const copy_word* __restrict__ src, copy_word* __restrict__ dst,
+                                                int64_t n)
 {
-    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) dst[i] = src[i];
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int64_t i = blockIdx.x * 256ll + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        copy_word v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) __builtin_nontemporal_store(v[u], dst + i + u * stride);
+    }
+    for (; i < n; i += stride) __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
 }
 
 __global__ void __launch_bounds__(256) k_copy1(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, int64_t n)
@@ -79,8 +94,8 @@ hipError_t copy_state(int lattice, int dtype, const EnvArgs& a, hipStream_t s)
         body = bytes / 16 * 16;
         if (body > 0) {
             const int64_t n = body / 16, want = (n + 255) / 256;
-            hipLaunchKernelGGL(k_copy16, (int)(want < 2048 ? want : 2048), 256, 0, s,
-                               reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
+            hipLaunchKernelGGL(k_copy16, (int)(want < 16384 ? want : 16384), 256, 0, s,
+                               reinterpret_cast<const copy_word*>(src), reinterpret_cast<copy_word*>(dst), n);
         }
     }
     if (bytes > body) {
