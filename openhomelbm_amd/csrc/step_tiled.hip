@@ -62,7 +62,10 @@ constexpr int kDown[9] = {6, 10, 12, 15, 17, 20, 21, 24, 26}; // cz = -1
 // + 3 sum_{a<b} c_a c_b S_ab, A = 1 - 1.5 u.u - 0.5 tr S, and odd part
 // O = 3 c.u, evaluated as f_{+/-c} = wrho E +/- (3 wrho) c.u with fused
 // multiply-adds. Populations are clamped at 0 as on the reference's plain
-// streaming branch (lbm_func_3d.py:525-526).
+// streaming branch (lbm_func_3d.py:525-526). The clamp cannot change a
+// population when A - 1/2 exceeds the largest possible stress term
+// (4.5 p^2 -/+ 3 p >= -1/2 for every projection p = c.u, and |c_a| <= 1), so
+// a wavefront whose cells all satisfy that bound skips it (same results).
 __device__ __forceinline__ void recon27(const float* m, float* f)
 {
     const float rho = m[0], ux = m[1], uy = m[2], uz = m[3];
@@ -105,8 +108,13 @@ __device__ __forceinline__ void recon27(const float* m, float* f)
     pair(23, 24, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, sxz - sxy - syz, R)), t);
     t = uy + uz - ux;
     pair(25, 26, r3, o3, fmaf(4.5f * t, t, fmaf(3.0f, syz - sxy - sxz, R)), t);
+    const float sd = (fabsf(sxx) + fabsf(syy)) + fabsf(szz), so = (fabsf(sxy) + fabsf(sxz)) + fabsf(syz);
+    const float bound = fmaf(-3.0f, so, fmaf(-1.5f, sd, A - 0.5f));
+    const bool positive = bound > 1e-3f && rho > 0.0f;
+    if (__builtin_amdgcn_ballot_w64(!positive)) {
 #pragma unroll
-    for (int i = 0; i < 27; ++i) f[i] = fmaxf(f[i], 0.0f);
+        for (int i = 0; i < 27; ++i) f[i] = fmaxf(f[i], 0.0f);
+    }
 }
 
 // Direction index of velocity (cx, cy, cz) in the reference's order.
@@ -164,27 +172,43 @@ __device__ __forceinline__ void moments27(const float* g, float* out)
 //   S'_aa = (1 - Om) (dev_aa / rho + u.u / 3) + Om u_a^2 + F_a u_a / rho
 //   S'_ab = (1 - Om) Pi_ab / rho + Om u_a u_b + (F_a u_b + F_b u_a) / (2 rho)
 // which equals the reference's Pi'/rho - 1/3 without the cancellation
-// against 1/3. out: rho, u + F/(2 rho), S (storage order).
+// against 1/3. out: rho, u + F/(2 rho), S (storage order). Without a
+// body-force field the forcing terms are left out at compile time.
+template <bool FORCE>
 __device__ __forceinline__ void collide27(const float* mo, const float* F, float omega, float* out)
 {
     const float rho = mo[0], pxx = mo[4], pyy = mo[5], pzz = mo[6];
     const float inv = 1.0f / rho;
-    const float hx = 0.5f * F[0] * inv, hy = 0.5f * F[1] * inv, hz = 0.5f * F[2] * inv;  // F / (2 rho)
-    const float ux = fmaf(mo[1], inv, hx), uy = fmaf(mo[2], inv, hy), uz = fmaf(mo[3], inv, hz);
     const float t3 = (pxx + pyy + pzz) * (1.0f / 3.0f);
-    const float usq3 = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (1.0f / 3.0f);
     const float om1 = 1.0f - omega;
     out[0] = rho;
-    out[1] = ux + hx;
-    out[2] = uy + hy;
-    out[3] = uz + hz;
-    // forcing terms F_a u_b / rho written as 2 h_a u_b
-    out[4] = fmaf(om1, fmaf(pxx - t3, inv, usq3), fmaf(omega * ux, ux, 2.0f * hx * ux));
-    out[5] = fmaf(om1, fmaf(pyy - t3, inv, usq3), fmaf(omega * uy, uy, 2.0f * hy * uy));
-    out[6] = fmaf(om1, fmaf(pzz - t3, inv, usq3), fmaf(omega * uz, uz, 2.0f * hz * uz));
-    out[7] = fmaf(om1 * mo[7], inv, fmaf(omega * ux, uy, fmaf(hx, uy, hy * ux)));
-    out[8] = fmaf(om1 * mo[8], inv, fmaf(omega * ux, uz, fmaf(hx, uz, hz * ux)));
-    out[9] = fmaf(om1 * mo[9], inv, fmaf(omega * uy, uz, fmaf(hy, uz, hz * uy)));
+    if (FORCE) {
+        const float hx = 0.5f * F[0] * inv, hy = 0.5f * F[1] * inv, hz = 0.5f * F[2] * inv;  // F / (2 rho)
+        const float ux = fmaf(mo[1], inv, hx), uy = fmaf(mo[2], inv, hy), uz = fmaf(mo[3], inv, hz);
+        const float usq3 = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (1.0f / 3.0f);
+        out[1] = ux + hx;
+        out[2] = uy + hy;
+        out[3] = uz + hz;
+        // forcing terms F_a u_b / rho written as 2 h_a u_b
+        out[4] = fmaf(om1, fmaf(pxx - t3, inv, usq3), fmaf(omega * ux, ux, 2.0f * hx * ux));
+        out[5] = fmaf(om1, fmaf(pyy - t3, inv, usq3), fmaf(omega * uy, uy, 2.0f * hy * uy));
+        out[6] = fmaf(om1, fmaf(pzz - t3, inv, usq3), fmaf(omega * uz, uz, 2.0f * hz * uz));
+        out[7] = fmaf(om1 * mo[7], inv, fmaf(omega * ux, uy, fmaf(hx, uy, hy * ux)));
+        out[8] = fmaf(om1 * mo[8], inv, fmaf(omega * ux, uz, fmaf(hx, uz, hz * ux)));
+        out[9] = fmaf(om1 * mo[9], inv, fmaf(omega * uy, uz, fmaf(hy, uz, hz * uy)));
+    } else {
+        const float ux = mo[1] * inv, uy = mo[2] * inv, uz = mo[3] * inv;
+        const float usq3 = fmaf(ux, ux, fmaf(uy, uy, uz * uz)) * (1.0f / 3.0f);
+        out[1] = ux;
+        out[2] = uy;
+        out[3] = uz;
+        out[4] = fmaf(om1, fmaf(pxx - t3, inv, usq3), omega * ux * ux);
+        out[5] = fmaf(om1, fmaf(pyy - t3, inv, usq3), omega * uy * uy);
+        out[6] = fmaf(om1, fmaf(pzz - t3, inv, usq3), omega * uz * uz);
+        out[7] = fmaf(om1 * mo[7], inv, omega * ux * uy);
+        out[8] = fmaf(om1 * mo[8], inv, omega * ux * uz);
+        out[9] = fmaf(om1 * mo[9], inv, omega * uy * uz);
+    }
 }
 
 // One face condition with the equilibrium S in closed form: f_eq_home_3d
@@ -509,7 +533,7 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
 #pragma unroll
             for (int i = 0; i < 27; ++i) g[i] = buf[i][cp - D3CX[i] - D3CY[i] * NX];
             moments27(g, mo);
-            collide27(mo, F, omega, out);
+            collide27<FORCE>(mo, F, omega, out);
             if (dx == 0) bc_face_closed(out, a.bc_type[0], a.bc_value[0], 0);
             else if (dx == nx - 1) bc_face_closed(out, a.bc_type[1], a.bc_value[1], 0);
             if (dy == ny - 1) bc_face_closed(out, a.bc_type[2], a.bc_value[2], 1);
