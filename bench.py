@@ -241,6 +241,42 @@ def bench_solver_worlds(nworld: int = 16, K: int = 200, W: int = 20):
     return out
 
 
+def bench_solver3d_worlds(nworld: int = 16, n: int = 96, K: int = 100, W: int = 30):
+    """LBM_Solver3D(n, n, n, nworld) with a capsule per world (its body at a
+    different place in each world), solver.step() per step: every world in
+    one library call (hl_step_worlds, six worlds per tiled launch) against
+    the worlds stepped one by one (each flow's own hl_step)."""
+    import torch
+
+    from openhomelbm_amd.lbm3d import LBM_Solver3D
+
+    out = {"worlds": nworld, "grid": [n, n, n], "steps": K}
+    s = LBM_Solver3D(n, n, n, solid_num=1, nworld=nworld)
+    for k, f in enumerate(s.flows):
+        f.set_capsule(0, (-0.06 * n, 0.0, 0.0), (0.06 * n, 0.0, 0.0), 0.08 * n)
+        f.solid_position[0] = (0.3 * n + 0.01 * k * n, 0.5 * n, 0.5 * n)
+        f.lattice.bc_type = [0, 1, 2, 2, 2, 2]
+        f.lattice.bc_value = [[0.05, 0.0, 0.0]] + [[0.0, 0.0, 0.0]] * 5
+        f.lattice.init_uniform((0.05, 0.0, 0.0))
+    def per_world():
+        for f in s.flows:
+            f.step()
+    modes = {"one_call": s.step, "per_world": per_world}
+    for step in modes.values():  # both paths warm (clocks, code objects) before either is timed
+        for _ in range(W):
+            step()
+    for name, step in modes.items():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / K
+        out[f"ms_per_step_{name}"] = round(ms, 4)
+        out[f"mlups_{name}"] = round(nworld * n ** 3 / (ms * 1e-3) / 1e6, 1)
+    return out
+
+
 def bench_storage_speed(K: int = 200, W: int = 10):
     """The same sphere step with 16-bit moment storage (FP32 collision):
     whole steps with forces and the step kernel alone, per storage type."""
@@ -484,6 +520,7 @@ def main():
         from openhomelbm_amd import precision
 
         sphere_b["eel"] = precision.compare_boundaries(steps=args.sphere_steps, log=log)
+        sphere_b["lbm_solver3d_worlds"] = bench_solver3d_worlds()
         sphere_b["seconds"] = round(time.perf_counter() - t0s, 1)
         log(f"[bench] sphere boundaries {sphere_b}")
 

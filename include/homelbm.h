@@ -118,6 +118,18 @@ int32_t hl_step(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, 
 int32_t hl_step_noalloc(const hl_env* env, int32_t cur, int32_t nsteps, int32_t kernel, double* force_history,
                         void* stream);
 
+/* hl_step of nworld lattices (one lattice type and dtype, sizes free), all
+ * at buffer `cur`: the world dimension of LBM_Solver3D.step, which launches
+ * stream_and_collide_3d over (nworld, nx, ny, nz) in one grid
+ * (envs/lbm3d/lbm_solver_3d.py:104-108). D3Q27 worlds that the tiled kernel
+ * fits share launches (up to six worlds per launch, each world's near-body
+ * cells and force rows as in hl_step); other worlds step one by one. Each
+ * world ends bitwise where its own hl_step call would. force_history: NULL
+ * or an array of nworld device pointers (each NULL or [nsteps][nsolid][6]).
+ * Errors name the world at fault. */
+int32_t hl_step_worlds(const hl_env* envs, int32_t nworld, int32_t cur, int32_t nsteps, int32_t kernel,
+                       double* const* force_history, void* stream);
+
 /* Per-solid momentum-exchange force and torque of state `cur`, FP64,
  * deterministic. out: device [nsolid][6] = Fx, Fy, Fz, Tx, Ty, Tz.
  * Replaces the atomics of lbm_func.py:392-401 / lbm_func_3d.py:497-511 and the

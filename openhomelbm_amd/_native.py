@@ -24,7 +24,7 @@ EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces"
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
            "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
            "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status",
-           "hl_chain_create", "hl_chain_destroy", "hl_chain_step")
+           "hl_chain_create", "hl_chain_destroy", "hl_chain_step", "hl_step_worlds")
 HL_COUPLE_SCALED, HL_COUPLE_PHYSICAL = 0, 1
 # render kinds (include/homelbm.h)
 HL_RENDER_SPEED, HL_RENDER_VORTICITY, HL_RENDER_VORTICITY_SOLID, HL_RENDER_RGB, HL_RENDER_SOLID = 0, 1, 2, 3, 4
@@ -139,6 +139,7 @@ def load():
     L.hl_step_polygons.argtypes = [env_p, I32, I32, P, P, P, I32, P, P, P]
     L.hl_step_capsules.argtypes = [env_p, P, I32, P, I32, I32, I32, P, P]
     L.hl_step_polygons_worlds.argtypes = [env_p, I32, I32, I32, P, P, P, P, P, P, P]
+    L.hl_step_worlds.argtypes = [env_p, I32, I32, I32, I32, P, P]
     L.hl_kernel_timer.argtypes = [I32, I32]
     L.hl_kernel_timer_read.argtypes = [C.POINTER(C.c_double), PI32]
     L.hl_copy_state.argtypes = [env_p, I32, P]
@@ -152,7 +153,7 @@ def load():
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
                  "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",
                  "hl_step_noalloc", "hl_step_capsules", "hl_batch_run", "hl_batch_run_status",
-                 "hl_chain_create", "hl_chain_destroy", "hl_chain_step"):
+                 "hl_chain_create", "hl_chain_destroy", "hl_chain_step", "hl_step_worlds"):
         getattr(L, name).restype = I32
     if L.hl_env_bytes() != C.sizeof(HLEnv):
         raise HLError(f"hl_env layout mismatch: library {L.hl_env_bytes()} bytes, binding {C.sizeof(HLEnv)}")

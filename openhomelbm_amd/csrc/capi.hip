@@ -129,11 +129,20 @@ struct ListCache {
     const void* mask = nullptr;
     int32_t key[10] = {0};
 };
+// Near-body lists of the worlds of hl_step_worlds in the same workspace, one
+// per world at its offset; valid while no other call used the workspace.
+struct WorldsCache {
+    bool valid = false;
+    void* ws = nullptr;
+    std::vector<size_t> offsets;
+    std::vector<ListCache> lists;
+};
 struct StreamCtx {
     std::mutex mutex;  // held for the whole call that uses this context
     void* ws = nullptr;
     size_t bytes = 0;
     ListCache list;
+    WorldsCache worlds;
 };
 std::mutex g_ctx_mutex;
 std::map<std::pair<int, hipStream_t>, StreamCtx> g_ctx;
@@ -162,11 +171,13 @@ int32_t workspace(StreamCtx* c, hipStream_t s, size_t bytes, void** out)
             c->bytes = 0;
         }
         c->list.valid = false;
+        c->worlds.valid = false;
         e = hipMalloc(&c->ws, bytes);
         if (e != hipSuccess) return fail(HL_E_NOMEM, "hipMalloc(%zu) for the reduction workspace: %s", bytes, hipGetErrorString(e));
         c->bytes = bytes;
     }
     *out = c->ws;
+    c->worlds.valid = false;  // the caller may overwrite the worlds' lists
     return HL_OK;
 }
 
@@ -494,6 +505,146 @@ int32_t hl_step_noalloc(const hl_env* env, int32_t cur, int32_t nsteps, int32_t 
                          *ws = d.ptr;
                          return HL_OK;
                      });
+}
+
+int32_t hl_step_worlds(const hl_env* envs, int32_t nworld, int32_t cur, int32_t nsteps, int32_t kernel,
+                       double* const* force_history, void* stream)
+{
+    if (!envs) return fail(HL_E_INVALID, "envs is NULL");
+    if (nworld < 1 || nworld > (1 << 16)) return fail(HL_E_INVALID, "nworld must be in [1, 65536], got %d", nworld);
+    if (cur != 0 && cur != 1) return fail(HL_E_INVALID, "cur must be 0 or 1");
+    if (nsteps < 0) return fail(HL_E_INVALID, "nsteps must be >= 0");
+    if (kernel < HL_KERNEL_AUTO || kernel > HL_KERNEL_TILED) return fail(HL_E_INVALID, "unknown kernel %d", kernel);
+    for (int32_t w = 0; w < nworld; ++w) {
+        const int32_t rc = validate(envs + w);
+        if (rc) {
+            const std::string m = g_error;
+            return fail(rc, "world %d: %s", w, m.c_str());
+        }
+        if (envs[w].dtype != envs[0].dtype) return fail(HL_E_INVALID, "world %d: every world needs the same dtype", w);
+        if (envs[w].lattice != envs[0].lattice)
+            return fail(HL_E_INVALID, "world %d: every world needs the same lattice", w);
+    }
+    if (nsteps == 0) return HL_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    StreamCtx* ctx = nullptr;
+    int32_t rc = stream_ctx(s, &ctx);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> ctx_lock(ctx->mutex);
+    const int dtype = envs[0].dtype;
+    auto history = [&](int w) { return force_history && envs[w].nsolid > 0 ? force_history[w] : nullptr; };
+    // D3Q27 worlds the tiled kernel fits share launches; anything else steps
+    // world by world exactly as hl_step does
+    bool batched = nworld > 1 && kernel != HL_KERNEL_GATHER && envs[0].lattice == HL_D3Q27;
+    for (int32_t w = 0; batched && w < nworld; ++w) batched = hl::tiled_fits(make_args(envs + w, cur), dtype);
+    if (!batched) {
+        for (int32_t w = 0; w < nworld; ++w) {
+            rc = step_core(envs + w, cur, nsteps, kernel, history(w), s, ctx->list,
+                           [&](size_t bytes, void** ws) { return workspace(ctx, s, bytes, ws); });
+            if (rc) {
+                const std::string m = g_error;
+                return fail(rc, "world %d: %s", w, m.c_str());
+            }
+        }
+        return HL_OK;
+    }
+    // one workspace, each world's near-body area at its own 256-byte aligned offset
+    std::vector<size_t> off((size_t)nworld);
+    size_t total = 0;
+    for (int32_t w = 0; w < nworld; ++w) {
+        off[(size_t)w] = total;
+        if (envs[w].mask) total += (hl::forces_workspace_bytes(make_args(envs + w, cur), HL_D3Q27) + 255) / 256 * 256;
+    }
+    WorldsCache& wc = ctx->worlds;
+    const bool kept = wc.valid;  // no other call used the workspace since the last worlds call
+    void* ws = nullptr;
+    rc = workspace(ctx, s, total > 256 ? total : 256, &ws);
+    if (rc) return rc;
+    ctx->list.valid = false;  // the single-lattice list shares this workspace
+    if (!kept || wc.ws != ws || wc.offsets != off) {
+        wc.ws = ws;
+        wc.offsets = off;
+        wc.lists.assign((size_t)nworld, ListCache());
+    }
+    wc.valid = true;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) != hipSuccess) cap = hipStreamCaptureStatusActive;
+    auto wsp = [&](int w) { return static_cast<void*>(static_cast<char*>(ws) + off[(size_t)w]); };
+    for (int32_t w = 0; w < nworld; ++w) {
+        const hl_env* env = envs + w;
+        if (!env->mask) continue;
+        // the list depends on the mask, its box and the lattice (see step_core)
+        const int32_t key[10] = {env->lattice, env->nx, env->ny, env->nz, env->mask_box[0], env->mask_box[1],
+                                 env->mask_box[2], env->mask_box[3], env->mask_box[4], env->mask_box[5]};
+        ListCache& lc = wc.lists[(size_t)w];
+        const bool reuse = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone && lc.valid &&
+                           lc.epoch == env->mask_epoch && lc.mask == env->mask &&
+                           std::memcmp(lc.key, key, sizeof(key)) == 0;
+        if (reuse) continue;
+        hipError_t e = hl::cut_list(HL_D3Q27, make_args(env, cur), wsp(w), s);
+        if (e != hipSuccess) return hip_fail(e, "cut-list kernel launch");
+        lc.valid = env->mask_epoch != 0 && cap == hipStreamCaptureStatusNone;
+        lc.epoch = env->mask_epoch;
+        lc.mask = env->mask;
+        std::memcpy(lc.key, key, sizeof(key));
+    }
+    // force rows a launch left for the next launch (or the closing sums)
+    struct Pending {
+        const double* rows = nullptr;
+        int nrows = 0, width = 0;
+        double* out = nullptr;
+    };
+    std::vector<Pending> pend((size_t)nworld);
+    for (int k = 0; k < nsteps; ++k) {
+        const int c = cur ^ (k & 1);
+        // chunks of consecutive worlds that agree on the body-force field
+        for (int32_t w0 = 0; w0 < nworld;) {
+            int n = 1;
+            while (n < hl::kTiledWorlds && w0 + n < nworld && (envs[w0 + n].force != nullptr) == (envs[w0].force != nullptr))
+                ++n;
+            hl::EnvArgs a[hl::kTiledWorlds];
+            hl::FixArgs fx[hl::kTiledWorlds];
+            hl::FixArgs* fxp[hl::kTiledWorlds];
+            for (int i = 0; i < n; ++i) {
+                const int w = w0 + i;
+                double* fout = history(w) ? history(w) + (int64_t)k * envs[w].nsolid * 6 : nullptr;
+                a[i] = make_args(envs + w, c);
+                std::memset(&fx[i], 0, sizeof(fx[i]));
+                fxp[i] = nullptr;
+                if (envs[w].mask) {
+                    hl::cut_bits_view(HL_D3Q27, a[i], wsp(w));
+                    fx[i] = hl::fixup_args(HL_D3Q27, a[i], wsp(w), fout != nullptr, k);
+                    if (fx[i].list) {
+                        fx[i].prev = pend[(size_t)w].rows;
+                        fx[i].prev_rows = pend[(size_t)w].nrows;
+                        fx[i].prev_out = pend[(size_t)w].out;
+                        fxp[i] = &fx[i];
+                    }
+                }
+            }
+            hipError_t e = hl::step_tiled_worlds(dtype, a, fxp, n, s);
+            if (e != hipSuccess) return hip_fail(e, "worlds step launch");
+            for (int i = 0; i < n; ++i) {
+                const int w = w0 + i;
+                double* fout = history(w) ? history(w) + (int64_t)k * envs[w].nsolid * 6 : nullptr;
+                if (fxp[i]) {
+                    pend[(size_t)w] = Pending();  // summed by this launch
+                    if (fout) pend[(size_t)w] = Pending{fx[i].partial, fx[i].nfix, fx[i].width, fout};
+                } else if (fout) {
+                    // no near-body cells: the forces are zero
+                    e = hl::fixup(HL_D3Q27, dtype, a[i], false, fout, wsp(w), s);
+                    if (e != hipSuccess) return hip_fail(e, "force launch");
+                }
+            }
+            w0 += n;
+        }
+    }
+    for (const Pending& p : pend) {
+        if (!p.rows) continue;
+        hipError_t e = hl::sum_partials(p.rows, p.nrows, p.width, p.out, s);
+        if (e != hipSuccess) return hip_fail(e, "force sum launch");
+    }
+    return HL_OK;
 }
 
 int32_t hl_step_capsules(const hl_env* env, const float* caps, int32_t ncap, const int32_t box[6], int32_t cur,

@@ -52,6 +52,23 @@ struct WorldArgs {
 // D2Q9 gather step that also writes fx.nfix (set here) force rows of the
 // state it reads into fx.partial and sums fx.prev into fx.prev_out.
 hipError_t step_gather_forces(int dtype, const EnvArgs& a, FixArgs& fx, hipStream_t s);
+// Several D3Q27 lattices stepped by one tiled launch: their arguments
+// travel in the kernel-argument block (about 3.1 KB for six).
+constexpr int kTiledWorlds = 6;
+struct TiledWorlds {
+    EnvArgs a[kTiledWorlds];
+    FixArgs fx[kTiledWorlds];
+    int base[kTiledWorlds + 1];  // first workgroup of each world (multiples of 8)
+    int gx[kTiledWorlds], gy[kTiledWorlds], ntile[kTiledWorlds];
+    int zc, n;
+};
+// step_tiled of n <= kTiledWorlds worlds (one dtype) in one launch; fx[k] is
+// null or world k's near-body arguments (its nfix is set here). The results
+// are those of step_tiled world by world. hipErrorNotSupported when a world
+// has no tiled kernel or the worlds disagree on the body-force field.
+hipError_t step_tiled_worlds(int dtype, const EnvArgs* a, FixArgs* const* fx, int n, hipStream_t s);
+// Whether the tiled kernel can step this lattice with the current tile shape.
+bool tiled_fits(const EnvArgs& a, int dtype);
 // Returns hipErrorNotSupported when no tiled kernel exists for this case.
 // With fx (near-body list built by cut_list, a.cut_bits set by
 // cut_bits_view), the same launch also updates the listed cells and, with

@@ -395,23 +395,24 @@ __device__ __forceinline__ void fixup_block(const EnvArgs& a, const FixArgs& fx,
 }
 
 // The previous step's force rows: output k is summed by wave k % W of
-// workgroup (k / W) % grid, its lanes taking rows lane, lane + 64, ... in order and a
-// butterfly combining them (the order of sum_partials, so the sums agree
-// bitwise).
+// workgroup (k / W) % nblk (the world's workgroups), its lanes taking rows
+// lane, lane + 64, ... in order and a butterfly combining them (the order of
+// sum_partials, so the sums agree bitwise).
 template <int NT>
-__device__ __forceinline__ void sum_previous(const FixArgs& fx, int blk)
+__device__ __forceinline__ void sum_previous(const FixArgs& fx, int blk, int nblk)
 {
     constexpr int W = NT / 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int k = blk * W + wave; k < fx.width; k += (int)gridDim.x * W) {
+    for (int k = blk * W + wave; k < fx.width; k += nblk * W) {
         const double v = wave_sum(lane_rows_sum(fx.prev, fx.prev_rows, fx.width, k, lane));
         if (lane == 0) fx.prev_out[k] = v;
     }
 }
 
+// The tiled pass of workgroup L of one lattice's ntile workgroups.
 template <int DT, int TX, int TY, bool FORCE>
-__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) WAVES_ATTR
-k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
+__device__ __forceinline__ void tiled_pass(const EnvArgs& a, int zc, int gx, int gy, int ntile, const FixArgs& fx,
+                                           int L)
 {
     constexpr int NX = TX + 2, NY = TY + 2, NCOL = NX * NY;
     constexpr int NT = (NCOL + 63) / 64 * 64;
@@ -420,7 +421,6 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
     __shared__ float sp[2][27][NCOL];  // double-buffered: one barrier per plane
     static_assert(NT / 64 * (96 * 4 + HL_MAX_SOLIDS * 6 * 8) + HL_MAX_SOLIDS * 24 * 4 <= sizeof(sp),
                   "near-body work fits the LDS");
-    const int L = (int)blockIdx.x;
     const int tid = threadIdx.x;
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin in
@@ -625,7 +625,30 @@ k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
         }
     }
     // the previous step's force rows (a few waves of the grid), after the march
-    if (fx.list && fx.prev) sum_previous<NT>(fx, L);
+    if (fx.list && fx.prev) sum_previous<NT>(fx, L, ntile);
+}
+
+template <int DT, int TX, int TY, bool FORCE>
+__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) WAVES_ATTR
+k_tiled_d3q27(EnvArgs a, int zc, int gx, int gy, int ntile, FixArgs fx)
+{
+    tiled_pass<DT, TX, TY, FORCE>(a, zc, gx, gy, ntile, fx, (int)blockIdx.x);
+}
+
+// Several lattices in one launch: world k owns workgroups [base[k],
+// base[k] + ntile[k]) of the grid; base[k] is a multiple of 8, so a world's
+// local workgroup index has the same XCD as its global one and each world's
+// tiles keep the XCD-aware order (the workgroups past ntile[k] up to the next
+// world's base return at once).
+template <int DT, int TX, int TY, bool FORCE>
+__global__ void __launch_bounds__(((TX + 2) * (TY + 2) + 63) / 64 * 64) WAVES_ATTR
+k_tiled_worlds(TiledWorlds w)
+{
+    int k = 0;
+    while (k + 1 < w.n && (int)blockIdx.x >= w.base[k + 1]) ++k;
+    const int L = (int)blockIdx.x - w.base[k];
+    if (L >= w.ntile[k]) return;
+    tiled_pass<DT, TX, TY, FORCE>(w.a[k], w.zc, w.gx[k], w.gy[k], w.ntile[k], w.fx[k], L);
 }
 
 // Tile shape of the D3Q27 kernel (set by hl_set_tiling / the autotuner).
@@ -679,6 +702,56 @@ hipError_t launch_tiled(const EnvArgs& a, int zc, FixArgs* fxp, hipStream_t s)
     return hipGetLastError();
 }
 
+// Launch of several worlds: each world's tiles (padded to a multiple of 8
+// workgroups) in one grid, FixArgs set up as launch_tiled does per world.
+template <int DT, int TX, int TY>
+hipError_t launch_tiled_worlds(const EnvArgs* a, FixArgs* const* fxp, int n, int zc, hipStream_t s)
+{
+    if (n < 1 || n > kTiledWorlds || zc < 1 || zc > 64) return hipErrorInvalidValue;
+    TiledWorlds w;
+    std::memset(&w, 0, sizeof(w));
+    w.zc = zc;
+    w.n = n;
+    const bool force = a[0].force != nullptr;
+    int base = 0;
+    for (int k = 0; k < n; ++k) {
+        const EnvArgs& e = a[k];
+        if ((e.force != nullptr) != force) return hipErrorNotSupported;
+        if ((e.nx % TX) == 1 || (e.nx % TX) == 2 || (e.ny % TY) == 1 || (e.ny % TY) == 2) return hipErrorNotSupported;
+        if ((uint64_t)state_elems(e) * BufIO<DT>::size >= (1ull << 31) || (e.force && (uint64_t)e.fs * 3 * 4 >= (1ull << 31)))
+            return hipErrorNotSupported;
+        w.a[k] = e;
+        w.gx[k] = (e.nx + TX - 1) / TX;
+        w.gy[k] = (e.ny + TY - 1) / TY;
+        w.ntile[k] = w.gx[k] * w.gy[k] * ((e.nz - 2 + zc - 1) / zc);
+        w.base[k] = base;
+        if (fxp[k] && fxp[k]->list) {
+            w.fx[k] = *fxp[k];
+            w.fx[k].nfix = w.ntile[k] < w.fx[k].max_waves ? w.ntile[k] : w.fx[k].max_waves;
+            if (w.fx[k].nfix < 1) return hipErrorInvalidValue;
+            fxp[k]->nfix = w.fx[k].nfix;
+        }
+        base += (w.ntile[k] + 7) / 8 * 8;
+    }
+    w.base[n] = base;
+    constexpr int threads = ((TX + 2) * (TY + 2) + 63) / 64 * 64;
+    if (force) hipLaunchKernelGGL((k_tiled_worlds<DT, TX, TY, true>), dim3(base), dim3(threads), 0, s, w);
+    else hipLaunchKernelGGL((k_tiled_worlds<DT, TX, TY, false>), dim3(base), dim3(threads), 0, s, w);
+    return hipGetLastError();
+}
+
+template <int DT>
+hipError_t dispatch_tile_worlds(const EnvArgs* a, FixArgs* const* fx, int n, hipStream_t s)
+{
+    const TileCfg c = tile_config(a[0]);
+    if (c.tx == 32 && c.ty == 16) return launch_tiled_worlds<DT, 32, 16>(a, fx, n, c.zc, s);
+    if (c.tx == 32 && c.ty == 8) return launch_tiled_worlds<DT, 32, 8>(a, fx, n, c.zc, s);
+    if (c.tx == 64 && c.ty == 8) return launch_tiled_worlds<DT, 64, 8>(a, fx, n, c.zc, s);
+    if (c.tx == 64 && c.ty == 4) return launch_tiled_worlds<DT, 64, 4>(a, fx, n, c.zc, s);
+    if (c.tx == 16 && c.ty == 16) return launch_tiled_worlds<DT, 16, 16>(a, fx, n, c.zc, s);
+    return hipErrorNotSupported;
+}
+
 template <int DT>
 hipError_t dispatch_tile(const EnvArgs& a, FixArgs* fx, hipStream_t s)
 {
@@ -692,6 +765,26 @@ hipError_t dispatch_tile(const EnvArgs& a, FixArgs* fx, hipStream_t s)
 }
 
 }  // namespace
+
+bool tiled_fits(const EnvArgs& a, int dtype)
+{
+    if (dtype != HL_F32 && dtype != HL_F16 && dtype != HL_BF16) return false;
+    const TileCfg c = tile_config(a);
+    if (!tiling_supported(HL_D3Q27, c.tx, c.ty, c.zc)) return false;
+    if ((a.nx % c.tx) == 1 || (a.nx % c.tx) == 2 || (a.ny % c.ty) == 1 || (a.ny % c.ty) == 2) return false;
+    const uint64_t esize = dtype == HL_F32 ? 4 : 2;
+    return (uint64_t)state_elems(a) * esize < (1ull << 31) && !(a.force && (uint64_t)a.fs * 3 * 4 >= (1ull << 31));
+}
+
+hipError_t step_tiled_worlds(int dtype, const EnvArgs* a, FixArgs* const* fx, int n, hipStream_t s)
+{
+    switch (dtype) {
+    case HL_F32: return dispatch_tile_worlds<HL_F32>(a, fx, n, s);
+    case HL_F16: return dispatch_tile_worlds<HL_F16>(a, fx, n, s);
+    case HL_BF16: return dispatch_tile_worlds<HL_BF16>(a, fx, n, s);
+    default: return hipErrorNotSupported;
+    }
+}
 
 hipError_t step_tiled(int lattice, int dtype, const EnvArgs& a, FixArgs* fx, hipStream_t s)
 {

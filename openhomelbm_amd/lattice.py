@@ -520,10 +520,29 @@ class Lattice:
         self._mask_dirty = tuple(box)
         self.mask_changed()
 
-    def set_poses(self, poses: np.ndarray) -> None:
-        """Pose table [nsolid, 8 | 24] (see include/homelbm.h)."""
+    def set_poses(self, poses: np.ndarray, stream=None) -> None:
+        """Pose table [nsolid, 8 | 24] (see include/homelbm.h), copied on
+        `stream` (torch's current stream by default) from a pinned staging
+        ring: the host does not wait for the device, and a staging slot is
+        rewritten only after the copy that read it has finished."""
         p = np.ascontiguousarray(poses, dtype=np.float32).reshape(self.nsolid, self.pose_width)
-        self.solids[: self.nsolid].copy_(torch.from_numpy(p))
+        if self.nsolid == 0:
+            return
+        ring = getattr(self, "_pose_ring", None)
+        if ring is None:
+            ring = self._pose_ring = [[torch.empty((self.nsolid, self.pose_width), dtype=torch.float32,
+                                                   pin_memory=True), torch.cuda.Event(), False] for _ in range(4)]
+            self._pose_slot = 0
+        slot = ring[self._pose_slot]
+        self._pose_slot = (self._pose_slot + 1) % len(ring)
+        if slot[2]:
+            slot[1].synchronize()
+        slot[0].numpy()[...] = p
+        s = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(s):
+            self.solids[: self.nsolid].copy_(slot[0], non_blocking=True)
+            slot[1].record(s)
+        slot[2] = True
 
     def set_force(self, force: Optional[np.ndarray]) -> None:
         if force is None:
@@ -611,6 +630,71 @@ class StepGraph:
         current stream, asynchronously)."""
         self.graphs[self.lat.cur].replay()
         self.lat.cur ^= self.nsteps & 1
+
+
+def _env_from(L: "Lattice") -> N.HLEnv:
+    """L's step descriptor with its current state as buffer 0 (the two
+    moment pointers swapped when L.cur is 1), so that lattices at different
+    buffer parities share one call; cached like _step_env."""
+    e = L._step_env()
+    if L.cur == 0:
+        return e
+    cached = getattr(L, "_swapped_env", None)
+    if cached is None or cached[0] is not e:
+        w = N.HLEnv()
+        C.memmove(C.byref(w), C.byref(e), C.sizeof(N.HLEnv))
+        w.moments[0], w.moments[1] = e.moments[1], e.moments[0]
+        cached = (e, w)
+        L._swapped_env = cached
+    return cached[1]
+
+
+def step_worlds(lattices: Sequence["Lattice"], nsteps: int = 1, kernel: int = N.HL_KERNEL_AUTO,
+                force_histories=None, stream=None) -> None:
+    """Lattice.step of several lattices of one type and dtype in one library
+    call (hl_step_worlds): the world dimension of LBM_Solver3D.step
+    (envs/lbm3d/lbm_solver_3d.py:104-108). D3Q27 worlds share tiled launches
+    (up to six per launch); each world ends bitwise where its own step call
+    would, whatever its buffer parity. force_histories None or one tensor
+    (or None) per world."""
+    lats = list(lattices)
+    n = len(lats)
+    if n == 0:
+        return
+    if force_histories is not None and len(force_histories) != n:
+        raise ValueError("one force history (or None) per world")
+    L0 = lats[0]
+    for L in lats:
+        if L.dtype_code != L0.dtype_code or L.device != L0.device or L.lattice != L0.lattice:
+            raise ValueError("the worlds must share lattice, dtype and device")
+    if L0.lattice == 27:
+        autotune.apply(L0)
+    hists = [None] * n if force_histories is None else list(force_histories)
+    envs = [_env_from(L) for L in lats]
+    # argument arrays per pattern of buffer parities (a call flips them all,
+    # so steady stepping alternates between two): they stand while the
+    # worlds, their descriptors and histories are the same objects
+    table = getattr(L0, "_step_worlds_args", None)
+    if table is None:
+        table = L0._step_worlds_args = {}
+    key = (nsteps, tuple(L.cur for L in lats))
+    cached = table.get(key)
+    same = cached is not None and len(cached[0]) == n and all(a() is b for a, b in zip(cached[0], lats)) and all(
+        a is b for a, b in zip(cached[1], envs)) and all(a is b for a, b in zip(cached[2], hists))
+    if not same:
+        if len(table) > 8:
+            table.clear()
+        fhs = [L._history_ptr(h, nsteps) for L, h in zip(lats, hists)]
+        args = ((N.HLEnv * n)(*envs), (C.c_void_p * n)(*fhs) if force_histories is not None else None)
+        # the worlds by weak reference (the first one holds the table: no
+        # cycle keeps their buffers alive)
+        cached = ([weakref.ref(L) for L in lats], envs, hists, args)
+        table[key] = cached
+    a = cached[3]
+    N.check(L0.lib.hl_step_worlds(a[0], n, 0, int(nsteps), int(kernel), a[1], _stream_ptr(stream)),
+            "hl_step_worlds")
+    for L in lats:
+        L.cur ^= nsteps & 1
 
 
 def step_polygons_worlds(lattices: Sequence["Lattice"], nsteps: int, tables, force_histories=None,

@@ -330,7 +330,7 @@ class HomeFlow:
         last = now if self._last_pose is None else self._last_pose
         table = np.concatenate([now, last], axis=1)[: self.n_objects].astype(np.float32)
         if self._uploaded is None or not np.array_equal(table, self._uploaded):  # static bodies: upload once
-            L.set_poses(table)
+            L.set_poses(table, stream=stream)
             self._uploaded = table
         self._at_rest = bool(np.array_equal(now, last))
         self._last_pose = now

@@ -29,7 +29,7 @@ import torch
 
 from . import _native as N
 from .arrays import ForceField, HostArray, LatticeField, as_numpy
-from .lattice import Lattice, capsule_box, union_box
+from .lattice import Lattice, capsule_box, step_worlds, union_box
 
 
 def _tri_box(tris: np.ndarray, L: Lattice):
@@ -81,6 +81,7 @@ class HomeFlow3D:
         self.grid_length = 1.0 / nx
         self.velocity_scale = self.time_step / self.grid_length
         self._lat = Lattice(27, self.nx, self.ny, self.nz, dtype=self._dtype, nsolid=n_objects, device=self._device)
+        self._uploaded = None
         L = self._lat
         L.nu = 0.1
         L.bc_type = [1] * 6
@@ -106,6 +107,7 @@ class HomeFlow3D:
         self._shapes: Dict[int, Tuple[np.ndarray, np.ndarray, float]] = {}  # body-frame capsules (lattice units)
         self._meshes: Dict[int, Tuple[np.ndarray, np.ndarray]] = {}  # body-frame meshes (lattice units)
         self._last: Optional[np.ndarray] = None  # pose table of the previous step
+        self._uploaded: Optional[np.ndarray] = None  # the [now | last] table the lattice holds
         self.capsules = np.zeros((0, 8), np.float32)  # world-frame capsules of the last voxelization
         self._caps_dev: Optional[torch.Tensor] = None  # the same table on the device (capsule cut links)
         self._caps_box = (0, 0, 0, 0, 0, 0)
@@ -272,7 +274,12 @@ class HomeFlow3D:
                 last[k] = now[k]
         init[: self.n_objects] = 1
         self._solid["mesh_transforms_initialized"]._a[...] = init
-        L.set_poses(np.concatenate([now, last], axis=1)[: self.n_objects].astype(np.float32))
+        table = np.concatenate([now, last], axis=1)[: self.n_objects].astype(np.float32)
+        # the lattice's table is this object's alone: an unchanged table (bodies
+        # at rest) is not copied again
+        if self._uploaded is None or not np.array_equal(table, self._uploaded):
+            L.set_poses(table, stream=stream)
+            self._uploaded = table
         self._last = now
         if not self._dirty:
             return
@@ -310,6 +317,10 @@ class HomeFlow3D:
 
     def step(self, stream=None) -> None:
         self._prepare(stream)
+        self._step_prepared(stream)
+
+    def _step_prepared(self, stream=None) -> None:
+        """The lattice step after _prepare (pose table and mask current)."""
         fh = self._forces if self.n_objects > 0 else None
         if self._uses_capsule_cuts():
             self.lattice.step_capsules(1, self._caps_dev, self._caps_box, force_history=fh, stream=stream)
@@ -353,9 +364,20 @@ class LBM_Solver3D:
         self.solid_id_to_index: Dict[int, int] = {}
 
     def step(self) -> None:
-        """Advance every world one step (asynchronous)."""
-        for flow in self.flows:
-            flow.step()
+        """Advance every world one step (asynchronous). The reference launches
+        every world in one grid (lbm_solver_3d.py:104-108); here the worlds go
+        to the library in one call (hl_step_worlds: up to six worlds per tiled
+        launch, each world bitwise as if stepped alone). Worlds with capsule
+        cut links step one by one."""
+        flows = self.flows
+        for flow in flows:
+            flow._prepare()
+        if len(flows) > 1 and not any(flow._uses_capsule_cuts() for flow in flows):
+            step_worlds([flow.lattice for flow in flows], 1, N.HL_KERNEL_AUTO,
+                        [flow._forces if flow.n_objects > 0 else None for flow in flows])
+        else:
+            for flow in flows:
+                flow._step_prepared()
         self.captured = True
 
     def _check_solid(self, solid_id):
