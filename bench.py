@@ -134,8 +134,27 @@ def cpu_baseline(w, seconds: float):
         el = time.perf_counter() - t0
         if el >= seconds or steps >= 500:
             break
+    # the same restatement on one thread (its scalar build), a shorter sample
+    steps1 = 0
+    t1 = time.perf_counter()
+    cur = m
+    while True:
+        cur = orc.step(env, cur, "f32", omp=False)
+        steps1 += 1
+        el1 = time.perf_counter() - t1
+        if el1 >= seconds / 2 or steps1 >= 100:
+            break
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(w.ncell * steps / el / 1e6, 3), "unit": "MLUPS", "cores": cores, "kind": "port",
-            "sample": f"{steps} FP32 steps of the full {w.name} lattice, OpenMP oracle, {el:.1f} s"}
+            "sample": f"{steps} FP32 steps of the full {w.name} lattice, OpenMP oracle, {el:.1f} s",
+            "single_thread": {"value": round(w.ncell * steps1 / el1 / 1e6, 3), "unit": "MLUPS", "cores": 1,
+                              "sample": f"{steps1} FP32 steps, scalar oracle build, {el1:.1f} s"},
+            "cpu_model": model, "host_cpus": os.cpu_count()}
 
 
 def bench_capsule_speed(lat, w, K: int, W: int = 20):
