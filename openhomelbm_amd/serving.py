@@ -12,9 +12,8 @@ requests from many resident swimmer environments, with continuous batching.
   * all such D2Q9 environments advance m steps in one call of the batched
     engine (Batch.step with an active mask and an m-step device force
     history);
-  * all such D3Q27 environments advance m steps on a side stream (so they
-    overlap the D2Q9 batch), each step one library call for all of them
-    (hl_step_worlds: up to six environments per tiled launch).
+  * each such D3Q27 environment advances m steps on one of two side
+    streams, so their kernels overlap on the device.
   A request that arrives while others are running joins at the next tick.
 * A request completes when its k-th step has run. Its result is the FP64
   force/torque history of its k steps, [k, nsolid, 6], gathered on the
@@ -22,8 +21,8 @@ requests from many resident swimmer environments, with continuous batching.
   runs from arrival to when the host sees the completion event.
 * Results do not depend on what else is in the batch. The batched D2Q9
   kernels give every environment the same arithmetic as stepping it alone, and
-  so do the shared D3Q27 launches, so a request's result is bitwise the same
-  alone and in the mix.
+  each D3Q27 environment runs its own kernels, so a request's result is
+  bitwise the same alone and in the mix.
 
 There is no reference counterpart: the reference steps every world on every
 call (lbm_solver.py:97-141).
@@ -40,7 +39,7 @@ import torch
 
 from . import inputs
 from .batch import MAX_SOLIDS, Batch
-from .lattice import Lattice, step_worlds
+from .lattice import Lattice
 
 
 @dataclass
@@ -133,13 +132,13 @@ class Server:
 
     Args:
         hbm_budget: bytes of device memory the resident environments may use.
-        streams: side streams (at least one; the D3Q27 environments of a
-            tick run on the first, beside the D2Q9 batch on the caller's
-            stream: within the four hardware queues HIP gives a process).
+        streams: side streams over which D3Q27 environments are spread per
+            tick (default 2: with the caller's stream that stays within the
+            four hardware queues HIP gives a process).
         max_tick_steps: the most steps one tick advances.
     """
 
-    def __init__(self, hbm_budget: float = 200e9, streams: int = 1, device=None, max_tick_steps: int = 16):
+    def __init__(self, hbm_budget: float = 200e9, streams: int = 2, device=None, max_tick_steps: int = 16):
         self.max_tick = int(max_tick_steps)
         if self.max_tick < 1:
             raise ValueError("max_tick_steps must be >= 1")
@@ -269,32 +268,29 @@ class Server:
                 k = self._batch_ids.index(i)
                 req._dev[req._off:req._off + m].copy_(self._bhist[:m, k, : r.spec.nsolid])
                 self.cells_stepped += r.spec.cells * m
-        # D3Q27: every ready environment, m steps, on a side stream (so they
-        # overlap the D2Q9 batch); per step one library call for all of them
-        # (hl_step_worlds: up to six environments per tiled launch)
+        # D3Q27: per environment, m steps, spread over the side streams
         ready3 = [i for i in ready if self.envs[i].spec.lattice == 27]
         if ready3:
             fork = torch.cuda.Event()
             fork.record(cur)
-            s = self._streams[0]
-            s.wait_event(fork)
-            res3 = [self.envs[i] for i in ready3]
-            lats = [r.lat for r in res3]
-            with torch.cuda.stream(s):
-                for q in range(m):
-                    for r in res3:
+            for n, i in enumerate(ready3):
+                r = self.envs[i]
+                req = r.queue[0]
+                s = self._streams[n % len(self._streams)]
+                s.wait_event(fork)
+                with torch.cuda.stream(s):
+                    for q in range(m):
                         j = (r.time + q) % r.spec.horizon
                         r.lat.voxelize_device(r.caps[j], r.box, stream=s)
                         r.lat.solids[: r.spec.nsolid].copy_(r.poses[j])
-                    step_worlds(lats, 1, force_histories=[r.hist[q] for r in res3], stream=s)
-                for r in res3:
-                    req = r.queue[0]
+                        r.lat.step(1, force_history=r.hist[q], stream=s)
                     req._dev[req._off:req._off + m].copy_(r.hist[:m, 0])
-                    req._dev.record_stream(s)
-                    self.cells_stepped += r.spec.cells * m
-            j = torch.cuda.Event()
-            j.record(s)
-            cur.wait_event(j)
+                req._dev.record_stream(s)
+                self.cells_stepped += r.spec.cells * m
+            for s in self._streams[: len(ready3)]:
+                j = torch.cuda.Event()
+                j.record(s)
+                cur.wait_event(j)
         for i in ready:
             r = self.envs[i]
             r.time += m
