@@ -1,34 +1,38 @@
 // Batched D2Q9 engine: many resident environments of uneven grid sizes
 // advanced together, one launch per kernel kind per step for all of them.
 //
-// Work-lists map each block to an (environment, tile). Per-environment state
+// Work-lists map each wavefront to an (environment, strip). Per-environment state
 // that changes during an advance (ping-pong parity, kinematics time index)
 // lives in device memory and is advanced by the last kernel of each step, so
 // a multi-step advance needs no host involvement and can be captured in a
 // graph and replayed.
 //
 // Per step, for all active environments:
-//   k_bvox      rebuild the body mask in the body box from the capsule table
-//               at the environment's time t
-//   k_bstep     the D2Q9 update of one 64x8 tile staged with its halo in LDS
-//               (reference operation order, mask wall rule); tiles that
-//               meet the body also reduce the per-solid momentum-exchange
-//               force of the state they read into an FP64 partial
-//   k_badvance  per environment: fixed-order sum of its tiles' partials into
+//   k_bstep     the D2Q9 update in strips of up to 62 columns, one wavefront
+//               each marching up its rows (march): every source row's
+//               populations rebuilt once and passed between lanes, the
+//               reference's operation order, the mask wall rule. The strips
+//               that reach a body box run the masked march (k_bstep<true>):
+//               their mask values voxelized from the capsule table at the
+//               environment's time t (their own cells written to the mask),
+//               and those near the body reduce the per-solid momentum-exchange
+//               force of the state they read into an FP64 partial; the others
+//               the plain march (k_bstep<false>)
+//   k_bvox      only for an environment with more capsules than the masked
+//               march's LDS table: its mask rebuilt in the body box first
+//   k_badvance  per environment: fixed-order sum of its strips' partials into
 //               the force history, then flip the parity and advance time.
 // The same kernels can be launched one environment at a time (the
 // per-environment baseline); every cell's arithmetic and every sum's order
 // are the same, so the two schedules give bitwise equal results.
 //
 // The persistent engine (k_brun, hl_batch_run) advances every environment K
-// steps in one cooperative launch: the workgroups deal out the step tiles of
-// every environment, each tile doing k_bstep's update with k_bvox fused into
-// its staging (each staged cell's mask value is computed from the capsule
-// table at the environment's time and the tile's own cells are written to
-// the mask), then a grid barrier ends the step; the first pass of step k+1
-// sums step k's force rows into the history. The cell arithmetic, the tile
-// slots and the summation order are those of the three-kernel schedule, so
-// states and forces agree with it bit for bit.
+// steps in one cooperative launch: the wavefronts deal out the strips of
+// every environment (the march with the voxelizer, as in k_bstep), then a
+// barrier ends the step; the first pass of step k+1 sums step k's force rows
+// into the history. The cell arithmetic, the strips and the summation order
+// are those of the three-kernel schedule, so states and forces agree with it
+// bit for bit.
 #pragma clang fp contract(off)
 
 #include <algorithm>
@@ -45,14 +49,15 @@
 namespace hl {
 
 constexpr int kBatchMaxSolids = 16;
-// step tile: kTX x kTY cells, 512 threads; lane = column, each wave kRows rows
-constexpr int kTX = 64, kTY = 8;
-constexpr int kThreads = 512, kWaves = kThreads / 64, kRows = kTY / kWaves;
-constexpr int kLX = kTX + 3, kLY = kTY + 3;  // staged region: 2 cells below, 1 above
-constexpr int kStage = (kLX * kLY + kThreads - 1) / kThreads;
+// step item: a strip of up to kSW columns and kSH rows, one wavefront; lane l
+// holds column x0 - 1 + l, so lanes 1..w are the strip's cells and lanes 0
+// and w + 1 its halo columns
+constexpr int kSW = 62, kSH = 16, kSHB = 4;
+constexpr int kStepWaves = 4;                  // k_bstep: one step item per wavefront
+constexpr int kThreads = 512, kWaves = kThreads / 64;  // persistent engine workgroup
 constexpr int kVoxW = 16;         // voxelizer item: 16 x 16 cells
-constexpr int kPartialWidth = kBatchMaxSolids * 3;  // Fx, Fy, Tz per solid
 constexpr int kRunCaps = 32;  // capsules per environment the persistent engine stages in LDS
+constexpr int kPartialWidth = kBatchMaxSolids * 3;  // Fx, Fy, Tz per solid
 
 struct BatchEnv {
     EnvArgs base;         // arguments for parity 0 (moments[0] -> moments[1])
@@ -60,7 +65,8 @@ struct BatchEnv {
     const float* poses;   // [T][nsolid][8] pose table (null: static poses in base.solids)
     int32_t ncap, T;
     int32_t vox_begin, vox_count;    // voxelizer items
-    int32_t step_begin, step_count;  // step tiles
+    int32_t step_begin, step_count;    // plain strips
+    int32_t mstep_begin, mstep_count;  // strips that reach the body box (masked march)
     int32_t slot_begin, slot_count;  // force partial slots (step tiles meeting the body)
 };
 
@@ -157,210 +163,394 @@ __device__ __forceinline__ double bwave_sum(double v)
     return v;
 }
 
-// One step of tile `it` of environment be at buffer parity par and time t,
-// its force row (if any) into part. VOX: the staged cells are voxelized from
-// the capsule table at t on the spot (and the tile's own cells of the mask
-// written) instead of read from the mask. The block's threads all call it
-// for the same tile.
-template <bool VOX>
-__device__ __forceinline__ void bstep_tile(const BatchEnv& be, const BatchItem it, int par, int t, double* part)
+// Raw buffer access to one environment's fields: the lane offset selects the
+// column (kOOB: loads read 0, the range check drops stores), the scalar
+// offset the field and row.
+constexpr uint32_t kOOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t batch_rsrc(const void* p, uint32_t bytes)
 {
-    __shared__ EnvArgs ea;
-    __shared__ float tile[6][kLY][kLX];   // moments (near the body only)
-    __shared__ float ptile[9][kLY][kLX];  // outgoing populations of every staged cell
-    __shared__ uint8_t mtile[kLY][kLX + 1];
-    __shared__ float pose[kBatchMaxSolids * 8];
-    __shared__ double acc[kWaves][kPartialWidth];
-    __shared__ float vcaps[kRunCaps * 8];  // VOX: the capsule table at t
-    const int tid = threadIdx.x;
-    const bool near = it.slot >= 0;
-    const int nx = uni(be.base.nx), ny = uni(be.base.ny), nsolid = uni(be.base.nsolid);
-    // an environment's fields hold fewer than 2^31 values: 32-bit offsets
-    const int st = uni((int)be.base.stride);
-    const float* src_p = uni(static_cast<const float*>(par ? be.base.dst : be.base.src));
-    const auto src = as_global(src_p);
-    const auto mask = as_global(uni(const_cast<uint8_t*>(be.base.mask)));
-    // the staged region [x0-2, x0+kTX] x [y0-2, y0+kTY] reaches the body box
-    // (the mask is 0 outside it): stage the mask and evaluate the wall rule
-    const bool masked = mask != nullptr && it.x0 - 2 <= be.base.box[1] && it.x0 + kTX >= be.base.box[0] - 1 &&
-                        it.y0 - 2 <= be.base.box[3] && it.y0 + kTY >= be.base.box[2] - 1;
-    const int T = uni(be.T), ncap = uni(be.ncap);
-    const float* bposes = uni(be.poses);
-    const float* caps = uni(be.caps);
-    const float* poses = bposes && T > 0 ? bposes + (t % T) * nsolid * 8 : uni(be.base.solids);
-    const bool vox = VOX && masked && caps && T > 0;
-    const int bx0 = uni(be.base.box[0]), bx1 = uni(be.base.box[1]), by0 = uni(be.base.box[2]), by1 = uni(be.base.box[3]);
-    // issue every load of the tile before the first wait: the capsule table
-    // and poses at t, then the tile with a halo of two cells below and one
-    // above in x and y (so every face cell's interior projection finds its
-    // sources here) and its mask
-    float cv = 0.0f, pv = 0.0f;
-    if (vox && tid < ncap * 8) cv = as_global(caps + (t % T) * ncap * 8)[tid];
-    if (masked && poses && tid < nsolid * 8) pv = as_global(poses)[tid];
-    float v[kStage][6];
-    int mv[kStage];
-    bool in[kStage];
-#pragma unroll
-    for (int r = 0; r < kStage; ++r) {
-        const int k = tid + r * kThreads;
-        const int lx = k % kLX, ly = k / kLX;
-        const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
-        in[r] = k < kLX * kLY && gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const int c = (in[r] ? gx : 0) + nx * (in[r] ? gy : 0);
-#pragma unroll
-        for (int f = 0; f < 6; ++f) v[r][f] = in[r] ? src[f * st + c] : 0.0f;
-        mv[r] = (!vox && masked && in[r]) ? mask[c] : 0;
-    }
-    if (vox) {
-        if (tid < ncap * 8) vcaps[tid] = cv;
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < kStage; ++r) {
-            const int k = tid + r * kThreads;
-            const int lx = k % kLX, ly = k / kLX;
-            const int gx = it.x0 - 2 + lx, gy = it.y0 - 2 + ly;
-            if (in[r] && gx >= bx0 && gx < bx1 && gy >= by0 && gy < by1) {
-                mv[r] = capsule_voxel(vcaps, ncap, gx, gy);
-                // the tile's own cells of the mask
-                if (lx >= 2 && lx < kTX + 2 && ly >= 2 && ly < kTY + 2) mask[gx + nx * gy] = (uint8_t)mv[r];
-            }
-        }
-    }
-    static_assert(sizeof(EnvArgs) % 4 == 0 && sizeof(EnvArgs) / 4 <= kThreads, "EnvArgs copy");
-    if (tid < sizeof(EnvArgs) / 4)
-        reinterpret_cast<uint32_t*>(&ea)[tid] = reinterpret_cast<const uint32_t*>(&be.base)[tid];
-    if (near)
-        for (int k = tid; k < kWaves * kPartialWidth; k += kThreads) (&acc[0][0])[k] = 0.0;
-    // each staged cell rebuilds its 9 outgoing populations once (the pull of
-    // a neighbour reads them instead of rebuilding from the moments); the
-    // values are those the reference's per-link reconstruction gives
-#pragma unroll
-    for (int r = 0; r < kStage; ++r) {
-        const int k = tid + r * kThreads;
-        if (k < kLX * kLY) {
-            const int lx = k % kLX, ly = k / kLX;
-            if (masked) {
-#pragma unroll
-                for (int f = 0; f < 6; ++f) tile[f][ly][lx] = v[r][f];
-                mtile[ly][lx] = (uint8_t)mv[r];
-            }
-            if (in[r]) {
-                float f9[9];
-                d2q9_pops<float>(v[r], f9);
-#pragma unroll
-                for (int i = 0; i < 9; ++i) ptile[i][ly][lx] = f9[i];
-            }
-        }
-    }
-    if (tid < kBatchMaxSolids * 8) pose[tid] = pv;
-    __syncthreads();
-    if (tid == 0) {
-        ea.src = src_p;
-        ea.dst = const_cast<void*>(par ? be.base.src : be.base.dst);
-        ea.solids = pose;  // wall velocities read the staged pose table
-    }
-    __syncthreads();
-    const int lane = tid & 63, wave = tid >> 6;
-    const int x = it.x0 + lane;
-    const auto dst = as_global(uni(static_cast<float*>(const_cast<void*>(par ? be.base.src : be.base.dst))));
-    uint32_t cutbits[kRows];
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-        const int y = it.y0 + wave * kRows + r;
-        cutbits[r] = 0;
-        if (x >= nx || y >= ny) continue;
-        // a face cell takes its interior projection's update, then the face chain
-        const int px = min(max(x, 1), nx - 2), py = min(max(y, 1), ny - 2);
-        const int tx = px - it.x0 + 2, ty = py - it.y0 + 2;
-        // stream_and_collide (lbm_func.py:322-467): the staged populations,
-        // or the wall rule from the cell's own moments on a cut link
-        float pop[9];
-        uint32_t cb = 0;
-#pragma unroll
-        for (int i = 0; i < 9; ++i) pop[i] = ptile[i][ty - D2CY[i]][tx - D2CX[i]];
-        if (masked) {
-            const int mval_c = mtile[ty][tx];
-            float mc[6];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) {
-                const int mval_s = mtile[ty - D2CY[i]][tx - D2CX[i]];
-                if ((mval_s != 0) != (mval_c != 0)) {
-                    cb |= 1u << i;
-                    pop[i] = d2q9_link_pop<float>(ea, px, py, i, mc, 1, mval_c ? mval_c - 1 : mval_s - 1, mc);
-                }
-            }
-        }
-        float m[6];
-        d2q9_collide<float>(ea, pop, 0.0f, 0.0f, m);
-        if (px != x || py != y) bc_chain<float, 2>(m, x, y, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
-        else cutbits[r] = cb;  // forces come from interior cells only
-        const int c = x + nx * y;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) dst[f * st + c] = m[f];
-    }
-    if (!near) return;
-    // momentum exchange of the tile's cut links per solid: each lane's cells
-    // and links in order, a wavefront butterfly, then the block's waves in order
-    uint32_t sids = 0;
-#pragma unroll
-    for (int r = 0; r < kRows; ++r) {
-        const int ty = wave * kRows + r + 2, tx = lane + 2;
-        for (int i = 1; i < 9; ++i)
-            if ((cutbits[r] >> i) & 1u) {
-                const int mvc = mtile[ty][tx], mvs = mtile[ty - D2CY[i]][tx - D2CX[i]];
-                const int s = mvc ? mvc - 1 : mvs - 1;
-                if (s < nsolid) sids |= 1u << s;
-            }
-    }
-    uint32_t uni = sids;
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) uni |= (uint32_t)__shfl_xor((int)uni, off, 64);
-    while (uni) {
-        const int s = __builtin_ctz(uni);
-        uni &= uni - 1;
-        double sum[3] = {0.0, 0.0, 0.0};
-        if ((sids >> s) & 1u) {
-#pragma unroll
-            for (int r = 0; r < kRows; ++r) {
-                if (!cutbits[r]) continue;
-                const int ty = wave * kRows + r + 2, tx = lane + 2;
-                const int y = it.y0 + ty - 2;
-                const int mvc = mtile[ty][tx];
-                float mc[6];
-#pragma unroll
-                for (int f = 0; f < 6; ++f) mc[f] = tile[f][ty][tx];
-                for (int i = 1; i < 9; ++i) {
-                    if (!((cutbits[r] >> i) & 1u)) continue;
-                    const int mvs = mtile[ty - D2CY[i]][tx - D2CX[i]];
-                    if ((mvc ? mvc - 1 : mvs - 1) != s) continue;
-                    float F[6];
-                    link_force<float, HL_F32, 9>(ea, mc, x, y, 0, i, s, F);
-                    sum[0] += (double)F[0];
-                    sum[1] += (double)F[1];
-                    sum[2] += (double)F[5];
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < 3; ++q) sum[q] = bwave_sum(sum[q]);
-        if (lane == 0)
-            for (int q = 0; q < 3; ++q) acc[wave][s * 3 + q] = sum[q];
-    }
-    __syncthreads();
-    if (tid < kPartialWidth) {
-        double sum = 0.0;
-        for (int w = 0; w < kWaves; ++w) sum += acc[w][tid];
-        as_global(part)[(int64_t)it.slot * kPartialWidth + tid] = sum;
-    }
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
 }
 
-__global__ void __launch_bounds__(kThreads) k_bstep(BatchArgs b, int item0)
+// Wavefront lane shifts (DPP): lane l receives lane l - 1's value (from_lower;
+// lane 0 receives 0) or lane l + 1's (from_upper; lane 63 receives 0).
+__device__ __forceinline__ int from_lower(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ int from_upper(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ float from_lower(float v) { return __builtin_bit_cast(float, from_lower(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ float from_upper(float v) { return __builtin_bit_cast(float, from_upper(__builtin_bit_cast(int, v))); }
+
+// Slot n of a list held across the lanes of a wavefront (six moments and a
+// row) takes lane src's moments m and row y.
+__device__ __forceinline__ void list_put(float* L, int& Ly, int n, const float* m, int src, int y)
 {
-    const BatchItem it = b.step_items[item0 + blockIdx.x];
-    if (!env_active(b, it.env)) return;
-    bstep_tile<false>(b.envs[it.env], it, b.parity[it.env] & 1, b.time[it.env], b.partial);
+    const bool mine = (int)(threadIdx.x & 63) == n;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const float v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, m[f]), src));
+        L[f] = mine ? v : L[f];
+    }
+    Ly = mine ? y : Ly;
+}
+
+// Whether strip `it` reads mask values: its source rows and columns reach
+// the body box (the mask is 0 outside it).
+__device__ __forceinline__ bool strip_masked(const BatchEnv& be, const BatchItem& it)
+{
+    const int x0 = uni(it.x0), w = uni(it.w), ny = uni(be.base.ny);
+    const int ya = max(uni(it.y0), 1), yb = min(uni(it.y0) + uni(it.h), ny - 1);
+    return uni(be.base.mask) != nullptr && x0 - 2 <= uni(be.base.box[1]) && x0 + w + 1 >= uni(be.base.box[0]) - 1 &&
+           ya - 2 <= uni(be.base.box[3]) && yb + 1 >= uni(be.base.box[2]) - 1;
+}
+
+// One step of step item `it` (a strip of environment be) at buffer parity par
+// and time t, by one wavefront, its force row (if any) into part. Lane l holds
+// column x = x0 - 1 + l; the wavefront marches up the strip's rows, and every
+// source row rebuilds its 9 outgoing populations once (d2q9_pops, the
+// reference's per-link reconstruction and clamp), shifting those that move in
+// x by one lane; a row's populations are pulled by the rows below, at and
+// above it over three consecutive steps. Each cell then takes the reference's
+// stream_and_collide (lbm_func.py:322-467) on those populations or, MASKED,
+// the wall rule from its own moments on a cut link. Face cells take their
+// interior projection's update and then the face chain: the face columns
+// (one cell per row) are collected across the lanes and chained together at
+// the end, with the face rows. The loads of a row are issued three rows
+// ahead (a ring of four buffers, the march unrolled by four so that each
+// load lands in the buffer it is consumed from). VOX: the source rows' mask
+// values come from the capsule table at t (the strip's own cells written to
+// the mask) instead of the mask.
+template <bool VOX, bool MASKED>
+__device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, int par, int t, double* part)
+{
+    const int lane = (int)(threadIdx.x & 63);
+    // every value below is the same across the wavefront: moved to scalar
+    // registers (buffer descriptors built from vector registers would make
+    // the compiler loop over the lanes around every load and store)
+    const int nx = uni(be.base.nx), ny = uni(be.base.ny), nsolid = uni(be.base.nsolid);
+    const uint32_t fbytes = (uint32_t)uni((int)be.base.stride) * 4u;  // below 2^31 / 6 (checked on the host)
+    const void* src = uni(par ? be.base.dst : be.base.src);
+    void* dst = uni(const_cast<void*>(par ? be.base.src : be.base.dst));
+    const auto rs = batch_rsrc(src, 6u * fbytes);
+    const auto rd = batch_rsrc(dst, 6u * fbytes);
+    uint8_t* mask = uni(const_cast<uint8_t*>(be.base.mask));
+    const int x0 = uni(it.x0), w = uni(it.w), y0 = uni(it.y0), h = uni(it.h), slot = uni(it.slot);
+    const int bx0 = uni(be.base.box[0]), bx1 = uni(be.base.box[1]), by0 = uni(be.base.box[2]), by1 = uni(be.base.box[3]);
+    const int T = uni(be.T), ncap = uni(be.ncap);
+    const float* bcaps = uni(be.caps);
+    const float* bposes = uni(be.poses);
+    // (more capsules than the LDS table holds: the mask, written by k_bvox)
+    const bool vox = VOX && MASKED && bcaps != nullptr && T > 0 && ncap <= kRunCaps;
+    const float* gposes = bposes && T > 0 ? bposes + (int64_t)(t % T) * nsolid * 8 : uni(be.base.solids);
+    // MASKED: this wavefront's copies of the pose table at t and (VOX) the
+    // capsule table at t in LDS, read by the wall rule, the forces and the
+    // voxelizer (one round of loads instead of one round trip per link)
+    __shared__ float spose[kWaves][kBatchMaxSolids * 8];
+    __shared__ float scaps[kWaves][kRunCaps * 8];
+    // MASKED: a row's cut links listed across the wavefront (lane | link << 6
+    // | (solid + 1) << 10), the cells' moments and the links' populations
+    __shared__ uint32_t clist[kWaves][8 * 64];
+    __shared__ float cmom[kWaves][6][64];
+    __shared__ float cpop[kWaves][9][64];
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const float* poses = gposes;
+    const float* caps = nullptr;
+    if (MASKED) {
+        if (gposes && lane < nsolid * 8) spose[wave][lane] = as_global(gposes)[lane];
+        if (gposes && lane + 64 < nsolid * 8) spose[wave][lane + 64] = as_global(gposes)[lane + 64];
+        poses = gposes ? &spose[wave][0] : nullptr;
+        if (vox) {
+            const float* gcaps = bcaps + (int64_t)(t % T) * ncap * 8;
+#pragma unroll
+            for (int k = 0; k < kRunCaps * 8; k += 64)
+                if (k + lane < ncap * 8) scaps[wave][k + lane] = as_global(gcaps)[k + lane];
+            caps = &scaps[wave][0];
+        }
+    }
+    // the fields the cell functions read (the viscosity, the pose table, the
+    // solid count, the width and the face conditions)
+    EnvArgs ea = {};
+    ea.nx = nx;
+    ea.ny = ny;
+    ea.nsolid = nsolid;
+    ea.solids = poses;
+    ea.nu = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, be.base.nu)));
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        ea.bc_type[k] = uni(be.base.bc_type[k]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ea.bc_value[k][c] = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, be.base.bc_value[k][c])));
+    }
+    const int x = x0 - 1 + lane;
+    const bool xin = x >= 0 && x < nx;
+    const uint32_t vcol = xin ? (uint32_t)x * 4u : kOOB;
+    const bool cell = lane >= 1 && lane <= w;          // a cell of this strip
+    const bool inner = cell && x >= 1 && x <= nx - 2;  // an interior column
+    const uint32_t vst = inner ? (uint32_t)x * 4u : kOOB;
+    const int ya = max(y0, 1), yb = min(y0 + h, ny - 1);  // interior rows [ya, yb)
+    const bool near = slot >= 0;
+    const auto rm = batch_rsrc(mask, MASKED ? (uint32_t)nx * (uint32_t)ny : 0u);
+    const uint32_t vmc = (MASKED && !vox && xin) ? (uint32_t)x : kOOB;  // mask bytes read from memory
+    const bool xbox = xin && x >= bx0 && x < bx1;
+    double acc = 0.0;  // lane 3 s + q: component q of solid s's force row
+    if (ya >= yb) {  // no interior row (the host never builds such a strip)
+        if (near && lane < kPartialWidth) as_global(part)[(int64_t)slot * kPartialWidth + lane] = acc;
+        return;
+    }
+
+    auto load_row = [&](int r, float* M, int& K) {
+        const uint32_t so = (uint32_t)(r * nx) * 4u;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            M[f] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs, (int)vcol, (int)(so + f * fbytes), 0));
+        if (MASKED) K = (int)__builtin_amdgcn_raw_buffer_load_b8(rm, (int)vmc, r * nx, 0);
+    };
+    auto store_row = [&](int r, uint32_t v, const float* m) {
+        const uint32_t so = (uint32_t)(r * nx) * 4u;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, m[f]), rd, (int)v, (int)(so + f * fbytes), 0);
+    };
+    // mask values of source row r (VOX: from the capsules, the strip's own cells written)
+    auto row_mask = [&](int r, int K) -> int {
+        if (!vox) return K;
+        int k = 0;
+        if (r >= by0 && r < by1 && xbox) {
+            k = capsule_voxel(caps, ncap, x, r);
+            if (cell && r >= y0 && r < y0 + h) as_global(mask)[x + nx * r] = (uint8_t)k;
+        }
+        return k;
+    };
+    // a source row's populations as the lanes pull them: kept (cy = 0), sent
+    // down (cy = -1, pulled by the row below) and up (cy = 1, by the row above)
+    auto send = [&](const float* M, float* dn, float* md, float* up) {
+        float f[9];
+        d2q9_pops<float>(M, f);
+        md[0] = f[0];
+        md[1] = from_lower(f[1]);
+        md[2] = from_upper(f[2]);
+        up[0] = f[3];
+        up[1] = from_lower(f[5]);
+        up[2] = from_upper(f[6]);
+        dn[0] = f[4];
+        dn[1] = from_upper(f[7]);
+        dn[2] = from_lower(f[8]);
+    };
+
+    // the march's carried state: moments of row y (My), the populations row
+    // y keeps (md) and sends up (uy), those row y - 1 sends up (um); the mask
+    // values of rows y - 1 and y and of their lower and upper neighbours; the
+    // updates of the first and last rows (the face rows' projections)
+    float My[6], md[3], um[3], uy[3], mfirst[6], mlast[6];
+    int Km = 0, KmL = 0, KmU = 0, Ky = 0, KyL = 0, KyU = 0;
+    // face columns: x = 0 from the projection x = 1 (lane 2), x = nx - 1 from
+    // x = nx - 2 (lane nx - x0 - 1); slot n of a list in lane n
+    const bool left = x0 == 0, right = x0 + w == nx;
+    const int lsrc = 2, rsrc = nx - x0 - 1;
+    float Lf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, Rf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int Ly = 0, Ry = 0, nl = 0, nr = 0;
+
+    auto dest_row = [&](int y, const float* Mn, int Kraw, float* mout) {
+        float dn[3], mdn[3], uyn[3];
+        send(Mn, dn, mdn, uyn);
+        int Kn = 0, KnL = 0, KnU = 0;
+        if (MASKED) {
+            Kn = row_mask(y + 1, Kraw);
+            KnL = from_lower(Kn);
+            KnU = from_upper(Kn);
+        }
+        float pop[9] = {md[0], md[1], md[2], um[0], dn[0], um[1], um[2], dn[1], dn[2]};
+        if (MASKED) {
+            // mask values of the sources (x - c_ix, y - c_iy) in direction order
+            const int ks[9] = {Ky, KyL, KyU, Km, Kn, KmL, KmU, KnU, KnL};
+            uint32_t cutm = 0;  // cut links of an interior cell (the others' pulls are not kept)
+#pragma unroll
+            for (int i = 1; i < 9; ++i)
+                if (inner && (ks[i] != 0) != (Ky != 0)) cutm |= 1u << i;
+            if (__builtin_amdgcn_ballot_w64(cutm != 0)) {
+                // the row's cut links listed in lane and link order: the
+                // exclusive prefix count of the lanes' links (four bit planes)
+                const uint32_t cnt = (uint32_t)__builtin_popcount(cutm);
+                uint32_t off = 0, K = 0;
+#pragma unroll
+                for (int bp = 0; bp < 4; ++bp) {
+                    const uint64_t bal = __builtin_amdgcn_ballot_w64((cnt >> bp) & 1u);
+                    off += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << bp;
+                    K += (uint32_t)__builtin_popcountll(bal) << bp;
+                }
+#pragma unroll
+                for (int i = 1; i < 9; ++i)
+                    if ((cutm >> i) & 1u) {
+                        const int sd = Ky ? Ky - 1 : ks[i] - 1;
+                        clist[wave][off++] = (uint32_t)lane | ((uint32_t)i << 6) | ((uint32_t)(sd + 1) << 10);
+                    }
+#pragma unroll
+                for (int f = 0; f < 6; ++f) cmom[wave][f][lane] = My[f];
+                // each listed link by one lane: the wall-rule population and,
+                // near the body, the force (per solid: the lanes' links of a
+                // pass by a butterfly, the passes in order)
+                for (uint32_t k0 = 0; k0 < K; k0 += 64) {
+                    const uint32_t k = k0 + (uint32_t)lane;
+                    float F[3] = {0.0f, 0.0f, 0.0f};
+                    int sd = -1;
+                    if (k < K) {
+                        const uint32_t en = clist[wave][k];
+                        const int l = (int)(en & 63u), i = (int)((en >> 6) & 15u);
+                        sd = (int)(en >> 10) - 1;
+                        float mc[6];
+#pragma unroll
+                        for (int f = 0; f < 6; ++f) mc[f] = cmom[wave][f][l];
+                        const int xl = x0 - 1 + l;
+                        float pl;
+                        if (near) pl = d2q9_cut_link<true>(ea, mc, xl, y, i, sd, poses, F);
+                        else pl = d2q9_cut_link<false>(ea, mc, xl, y, i, sd, poses, F);
+                        cpop[wave][i][l] = pl;
+                        if (sd >= nsolid) sd = -1;
+                    }
+                    if (near) {
+                        uint32_t uni = sd >= 0 ? 1u << sd : 0u;
+#pragma unroll
+                        for (int o = 32; o >= 1; o >>= 1) uni |= (uint32_t)__shfl_xor((int)uni, o, 64);
+                        while (uni) {
+                            const int sv = __builtin_ctz(uni);
+                            uni &= uni - 1;
+                            const bool mine = sd == sv;
+                            const double s0 = bwave_sum(mine ? (double)F[0] : 0.0);
+                            const double s1 = bwave_sum(mine ? (double)F[1] : 0.0);
+                            const double s2 = bwave_sum(mine ? (double)F[2] : 0.0);
+                            const int q = lane - 3 * sv;
+                            acc = q == 0 ? acc + s0 : q == 1 ? acc + s1 : q == 2 ? acc + s2 : acc;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int i = 1; i < 9; ++i)
+                    if ((cutm >> i) & 1u) pop[i] = cpop[wave][i][lane];
+            }
+        }
+        d2q9_collide<float>(ea, pop, 0.0f, 0.0f, mout);
+        store_row(y, vst, mout);
+        // face columns to the lists
+        if (left) list_put(Lf, Ly, nl++, mout, lsrc, y);
+        if (right) list_put(Rf, Ry, nr++, mout, rsrc, y);
+        // march on
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            um[k] = uy[k];
+            uy[k] = uyn[k];
+            md[k] = mdn[k];
+        }
+#pragma unroll
+        for (int f = 0; f < 6; ++f) My[f] = Mn[f];
+        if (MASKED) {
+            Km = Ky; KmL = KyL; KmU = KyU;
+            Ky = Kn; KyL = KnL; KyU = KnU;
+        }
+    };
+
+    // prologue: rows ya - 1 and ya rebuilt, rows ya + 1 .. ya + 3 requested
+    float R0[6], R1[6], R2[6], R3[6];
+    int K0 = 0, K1 = 0, K2 = 0, K3 = 0;
+    {
+        float Ma[6], Mb[6];
+        int Ka = 0, Kb = 0;
+        load_row(ya - 1, Ma, Ka);
+        load_row(ya, Mb, Kb);
+        load_row(ya + 1, R0, K0);
+        load_row(min(ya + 2, yb), R1, K1);
+        load_row(min(ya + 3, yb), R2, K2);
+        float dn[3], mk[3];
+        send(Ma, dn, mk, um);
+        send(Mb, dn, md, uy);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) My[f] = Mb[f];
+        if (MASKED) {
+            Km = row_mask(ya - 1, Ka);
+            KmL = from_lower(Km);
+            KmU = from_upper(Km);
+            Ky = row_mask(ya, Kb);
+            KyL = from_lower(Ky);
+            KyU = from_upper(Ky);
+        }
+    }
+    // step y: row y + 1 in ring buffer (y - ya) % 4, row y + 4 (past the last
+    // row, the last row again, so every step issues the same loads) requested
+    // into the buffer row y left
+    auto step = [&](int y, const float* Mn, int Kn, float* Mp, int& Kp, float* mout) {
+        load_row(min(y + 4, yb), Mp, Kp);
+        dest_row(y, Mn, Kn, mout);
+    };
+    step(ya, R0, K0, R3, K3, mfirst);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) mlast[f] = mfirst[f];
+    int y = ya + 1;
+    for (; y + 3 < yb; y += 4) {
+        step(y, R1, K1, R0, K0, mlast);
+        step(y + 1, R2, K2, R1, K1, mlast);
+        step(y + 2, R3, K3, R2, K2, mlast);
+        step(y + 3, R0, K0, R3, K3, mlast);
+    }
+    if (y < yb) {
+        step(y, R1, K1, R0, K0, mlast);
+        if (y + 1 < yb) {
+            step(y + 1, R2, K2, R1, K1, mlast);
+            if (y + 2 < yb) step(y + 2, R3, K3, R2, K2, mlast);
+        }
+    }
+
+    // face rows from their projection rows over the interior columns, the
+    // corners to the face-column lists
+    if (y0 == 0) {  // row 0 from row 1 (the first row)
+        float mf[6] = {mfirst[0], mfirst[1], mfirst[2], mfirst[3], mfirst[4], mfirst[5]};
+        bc_chain<float, 2>(mf, inner ? x : 1, 0, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
+        store_row(0, vst, mf);
+        if (left) list_put(Lf, Ly, nl++, mfirst, lsrc, 0);
+        if (right) list_put(Rf, Ry, nr++, mfirst, rsrc, 0);
+    }
+    if (y0 + h == ny) {  // row ny - 1 from row ny - 2 (the last row)
+        float mf[6] = {mlast[0], mlast[1], mlast[2], mlast[3], mlast[4], mlast[5]};
+        bc_chain<float, 2>(mf, inner ? x : 1, ny - 1, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
+        store_row(ny - 1, vst, mf);
+        if (left) list_put(Lf, Ly, nl++, mlast, lsrc, ny - 1);
+        if (right) list_put(Rf, Ry, nr++, mlast, rsrc, ny - 1);
+    }
+    // face columns: the face chain over the collected cells, one lane each
+    if (nl) {
+        float mf[6] = {Lf[0], Lf[1], Lf[2], Lf[3], Lf[4], Lf[5]};
+        if (lane < nl) bc_chain<float, 2>(mf, 0, Ly, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
+        const uint32_t v = lane < nl ? (uint32_t)(Ly * nx) * 4u : kOOB;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, mf[f]), rd, (int)v, (int)(f * fbytes), 0);
+    }
+    if (nr) {
+        float mf[6] = {Rf[0], Rf[1], Rf[2], Rf[3], Rf[4], Rf[5]};
+        if (lane < nr) bc_chain<float, 2>(mf, nx - 1, Ry, 0, nx, ny, 1, ea.bc_type, ea.bc_value);
+        const uint32_t v = lane < nr ? (uint32_t)(Ry * nx + nx - 1) * 4u : kOOB;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, mf[f]), rd, (int)v, (int)(f * fbytes), 0);
+    }
+    if (near && lane < kPartialWidth) as_global(part)[(int64_t)slot * kPartialWidth + lane] = acc;
+}
+
+template <bool VOX>
+__device__ __forceinline__ void march_strip(const BatchEnv& be, const BatchItem& it, int par, int t, double* part)
+{
+    if (strip_masked(be, it)) march<VOX, true>(be, it, par, t, part);
+    else march<VOX, false>(be, it, par, t, part);
+}
+
+// The step of the strips [item0, item0 + count), one per wavefront: the
+// strips that reach a body box (MASKED) and the others are listed apart, so
+// the plain strips' kernel keeps the registers of the plain march only.
+template <bool MASKED>
+__global__ void __launch_bounds__(64 * kStepWaves) k_bstep(BatchArgs b, int item0, int count)
+{
+    const int i = (int)blockIdx.x * kStepWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (i >= count) return;
+    const BatchItem it = b.step_items[item0 + i];
+    const int e = uni(it.env);
+    if (!env_active(b, e)) return;
+    march<false, MASKED>(b.envs[e], it, uni(b.parity[e]) & 1, uni(b.time[e]), b.partial);
 }
 
 // Per environment (one block): the fixed-order sum of its partial slots into
@@ -468,44 +658,28 @@ __device__ __forceinline__ void sum_output(const BatchArgs& b, const double* par
     history[((int64_t)k * nenv_total + e) * kBatchMaxSolids * 6 + r] = v;
 }
 
-// K steps of the step tiles [i0, i0 + ni) (environments [e0, e0 + ne)) in
-// one launch: the tiles dealt to the workgroups in order, each tile the
-// three-kernel schedule's step with the voxelizer fused into its staging; a
+// K steps of the step items [i0, i0 + ni) and [j0, j0 + nj) (environments
+// [e0, e0 + ne)) in
+// one launch: the items dealt to the wavefronts in order, each the
+// three-kernel schedule's step with the voxelizer fused into its row loads; a
 // grid barrier after every step; step k's force rows go to bank k & 1 and
 // are summed by step k + 1 (the last step's after the loop); then parity and
 // time advance by K.
-__global__ void __launch_bounds__(kThreads) k_brun(BatchArgs b, int i0, int ni, int K, double* history,
-                                                   int nenv_total, int e0, int ne, int nslots, unsigned* bar,
-                                                   int* flag)
+__global__ void __launch_bounds__(kThreads) k_brun(BatchArgs b, int i0, int ni, int j0, int nj, int K,
+                                                   double* history, int nenv_total, int e0, int ne, int nslots,
+                                                   unsigned* bar, int* flag)
 {
-    static_assert(sizeof(BatchEnv) % 4 == 0 && sizeof(BatchEnv) / 4 < kThreads, "BatchEnv copy");
-    __shared__ BatchEnv hdr;
-    __shared__ int hst[3];  // parity, time, active
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (int k = 0; k < K; ++k) {
         if (k > 0 && history)
             run_sums(b, b.partial + (size_t)((k - 1) & 1) * nslots * kPartialWidth, k - 1, history, nenv_total, e0, ne);
-        for (int i = i0 + blockIdx.x; i < i0 + ni; i += gridDim.x) {
-            // the tile's environment copied to LDS in one round of loads (its
-            // fields then read from LDS, not one dependent load at a time)
-            const BatchItem it = b.step_items[i];
-            const int e = it.env;
-            __syncthreads();  // the previous tile's LDS reads are done
-            if (threadIdx.x < sizeof(BatchEnv) / 4) {
-                reinterpret_cast<uint32_t*>(&hdr)[threadIdx.x] =
-                    as_global(reinterpret_cast<const uint32_t*>(b.envs + e))[threadIdx.x];
-            } else if (threadIdx.x >= kThreads - 64) {  // the last wave: loads issued together
-                const int p = as_global(b.parity)[e], tm = as_global(b.time)[e];
-                const int a = b.active ? (int)as_global(b.active)[e] : 1;
-                if (threadIdx.x == kThreads - 1) {
-                    hst[0] = p;
-                    hst[1] = tm;
-                    hst[2] = a;
-                }
-            }
-            __syncthreads();
-            if (!hst[2]) continue;
-            bstep_tile<true>(hdr, it, (hst[0] + k) & 1, hst[1] + k,
-                             b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
+        // step items dealt to the wavefronts in order
+        for (int u = (int)blockIdx.x * kWaves + wave; u < ni + nj; u += (int)gridDim.x * kWaves) {
+            const BatchItem it = b.step_items[u < ni ? i0 + u : j0 + (u - ni)];
+            const int e = uni(it.env);
+            if (!env_active(b, e)) continue;
+            march_strip<true>(b.envs[e], it, (uni(b.parity[e]) + k) & 1, uni(b.time[e]) + k,
+                              b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
         }
         grid_barrier(bar, (unsigned)(k + 1) * gridDim.x, flag);
     }
@@ -571,47 +745,27 @@ __global__ void __launch_bounds__(kThreads) k_brun_xcd(BatchArgs b, const int* x
                                                        int K, double* history, int nenv_total, int nslots,
                                                        unsigned* bars, int* flag, unsigned zero)
 {
-    static_assert(sizeof(BatchEnv) % 4 == 0 && sizeof(BatchEnv) / 4 < kThreads, "BatchEnv copy");
-    __shared__ BatchEnv hdr;
-    __shared__ int hst[3];  // parity, time, active
     const int g = (int)(blockIdx.x & 7), lb = (int)(blockIdx.x >> 3), nb = (int)(gridDim.x >> 3);
-    // this workgroup's contiguous share of its group's tiles (consecutive
-    // tiles mostly share an environment, whose header then stays in LDS)
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    // this workgroup's contiguous share of its group's step items, dealt to
+    // its wavefronts in order
     const int ib = plan.goff[g], ie = plan.goff[g + 1];
     const int per = (ie - ib + nb - 1) / nb;
     const int i0 = ib + lb * per, i1 = min(i0 + per, ie);
     const int ge0 = plan.geoff[g], ne = plan.geoff[g + 1] - ge0;
     const int gt = lb * kThreads + (int)threadIdx.x, nt = nb * kThreads;
     unsigned* bar = bars + g * kXcdBarStride;
-    int cur = -1;  // environment whose header is in LDS
     for (int k = 0; k < K; ++k) {
         if (k > 0 && history)
             for (int o = gt; o < ne * kBatchMaxSolids * 6; o += nt)
                 sum_output(b, b.partial + (size_t)((k - 1) & 1) * nslots * kPartialWidth, k - 1, history, nenv_total,
                            genv[ge0 + o / (kBatchMaxSolids * 6)], o % (kBatchMaxSolids * 6));
-        for (int i = i0; i < i1; ++i) {
+        for (int i = i0 + wave; i < i1; i += kWaves) {
             const BatchItem it = b.step_items[xitems[i]];
-            const int e = it.env;
-            __syncthreads();  // the previous tile's LDS reads are done
-            if (e != cur) {
-                if (threadIdx.x < sizeof(BatchEnv) / 4) {
-                    reinterpret_cast<uint32_t*>(&hdr)[threadIdx.x] =
-                        as_global(reinterpret_cast<const uint32_t*>(b.envs + e))[threadIdx.x];
-                } else if (threadIdx.x >= kThreads - 64) {
-                    const int p = as_global(b.parity)[e], tm = as_global(b.time)[e];
-                    const int a = b.active ? (int)as_global(b.active)[e] : 1;
-                    if (threadIdx.x == kThreads - 1) {
-                        hst[0] = p;
-                        hst[1] = tm;
-                        hst[2] = a;
-                    }
-                }
-                cur = e;
-                __syncthreads();
-            }
-            if (!hst[2]) continue;
-            bstep_tile<true>(hdr, it, (hst[0] + k) & 1, hst[1] + k,
-                             b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
+            const int e = uni(it.env);
+            if (!env_active(b, e)) continue;
+            march_strip<true>(b.envs[e], it, (uni(b.parity[e]) + k) & 1, uni(b.time[e]) + k,
+                              b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
         }
         xcd_barrier(bar, (unsigned)(k + 1) * nb, flag, zero);
     }
@@ -653,7 +807,8 @@ __global__ void __launch_bounds__(kThreads) k_xcc_ids(int* out)
 struct hl_batch {
     int n = 0;
     std::vector<hl::BatchEnv> envs;  // host mirror of the device table
-    std::vector<hl::BatchItem> step_items, vox_items;
+    std::vector<hl::BatchItem> step_items, vox_items;  // step items: every plain strip, then the masked ones
+    int nplain = 0;
     int nslots = 0;
     unsigned* d_bar = nullptr;  // grid barrier counter of the persistent engine
     int* d_flag = nullptr;      // set when a barrier wait gave up
@@ -675,6 +830,10 @@ struct hl_batch {
     int xcd_grid = 0;            // grid the check ran for (0: none yet)
     bool xcd_ok = false;         // workgroup w ran on XCD w % 8 for that grid
     hipStream_t cap_stream = nullptr;
+    // the body strips' side of a step (voxelizer, masked march), forked from
+    // and joined back to the caller's stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -707,13 +866,63 @@ template <typename T> hipError_t upload(T** dptr, size_t* cap, const T* data, si
     return hipMemcpy(*dptr, data, count * sizeof(T), hipMemcpyHostToDevice);
 }
 
-// Step tiles cover every environment; tiles that meet the force region
-// (interior cells within one cell of the mask box, where every cut link has
-// its fluid end) get a partial slot. Voxelizer items tile the mask box.
+// Whether a strip reads mask values, as the device decides for the masked
+// march: its source rows and columns reach the body box.
+bool strip_reaches_box(const hl::EnvArgs& a, const hl::BatchItem& it)
+{
+    const int ya = std::max(it.y0, 1), yb = std::min(it.y0 + it.h, a.ny - 1);
+    return a.mask != nullptr && it.x0 - 2 <= a.box[1] && it.x0 + it.w + 1 >= a.box[0] - 1 && ya - 2 <= a.box[3] &&
+           yb + 1 >= a.box[2] - 1;
+}
+
+// Step items (strips) cover every environment: columns in strips of
+// near-equal widths (at most kSW, so at least 31 columns, or the whole width
+// of at least 3), rows in strips of at most kSH rows, and of at most kSHB rows
+// where a column strip crosses the body box (the wall rule and the forces of
+// those rows are then shared by more wavefronts); the face rows' strips hold
+// at least 2 rows, so every face cell's projection lies in its strip. Each
+// environment lists its plain strips, then the strips that reach the box
+// (the masked march), which alone can meet the force region (interior cells
+// within one cell of the box, where every cut link has its fluid end) and
+// then get a partial slot. Voxelizer items tile the mask box.
 void rebuild_items(hl_batch* b)
 {
     b->step_items.clear();
     b->vox_items.clear();
+    std::vector<std::vector<hl::BatchItem>> masked((size_t)b->n);
+    for (int e = 0; e < b->n; ++e) {
+        hl::BatchEnv& be = b->envs[e];
+        const hl::EnvArgs& a = be.base;
+        const bool body = a.mask && a.box[0] < a.box[1] && a.box[2] < a.box[3];
+        be.step_begin = (int)b->step_items.size();
+        const int nsx = (a.nx + hl::kSW - 1) / hl::kSW;
+        for (int i = 0; i < nsx; ++i) {
+            const int x = (int)((int64_t)a.nx * i / nsx), xw = (int)((int64_t)a.nx * (i + 1) / nsx) - x;
+            // row segments: the rows within 3 of the box apart when the strip
+            // column crosses it (cuts kept within [2, ny - 2])
+            const bool xb = body && x - 2 <= a.box[1] && x + xw + 1 >= a.box[0] - 1;
+            std::vector<int> cuts = {0, a.ny};
+            const int c0 = a.box[2] - 3, c1 = a.box[3] + 3;
+            if (xb && c0 >= 2 && c0 <= a.ny - 2) cuts.push_back(c0);
+            if (xb && c1 >= 2 && c1 <= a.ny - 2) cuts.push_back(c1);
+            std::sort(cuts.begin(), cuts.end());
+            cuts.erase(std::unique(cuts.begin(), cuts.end()), cuts.end());
+            for (size_t k = 0; k + 1 < cuts.size(); ++k) {
+                const int s0 = cuts[k], s1 = cuts[k + 1];
+                const bool inner = xb && s0 < c1 && s1 > c0;
+                const int hmax = inner ? hl::kSHB : hl::kSH, n = (s1 - s0 + hmax - 1) / hmax;
+                for (int j = 0; j < n; ++j) {
+                    const int y = s0 + (int)((int64_t)(s1 - s0) * j / n);
+                    const int yh = s0 + (int)((int64_t)(s1 - s0) * (j + 1) / n) - y;
+                    const hl::BatchItem it{e, x, y, xw, yh, -1};
+                    if (strip_reaches_box(a, it)) masked[(size_t)e].push_back(it);
+                    else b->step_items.push_back(it);
+                }
+            }
+        }
+        be.step_count = (int)b->step_items.size() - be.step_begin;
+    }
+    b->nplain = (int)b->step_items.size();
     int slot = 0;
     for (int e = 0; e < b->n; ++e) {
         hl::BatchEnv& be = b->envs[e];
@@ -722,14 +931,13 @@ void rebuild_items(hl_batch* b)
         const int fx0 = std::max(a.box[0] - 1, 1), fx1 = std::min(a.box[1] + 1, a.nx - 1);
         const int fy0 = std::max(a.box[2] - 1, 1), fy1 = std::min(a.box[3] + 1, a.ny - 1);
         const bool forces = body && a.nsolid > 0 && fx0 < fx1 && fy0 < fy1;
-        be.step_begin = (int)b->step_items.size();
+        be.mstep_begin = (int)b->step_items.size();
         be.slot_begin = slot;
-        for (int y = 0; y < a.ny; y += hl::kTY)
-            for (int x = 0; x < a.nx; x += hl::kTX) {
-                const bool meets = forces && x < fx1 && x + hl::kTX > fx0 && y < fy1 && y + hl::kTY > fy0;
-                b->step_items.push_back({e, x, y, hl::kTX, hl::kTY, meets ? slot++ : -1});
-            }
-        be.step_count = (int)b->step_items.size() - be.step_begin;
+        for (hl::BatchItem it : masked[(size_t)e]) {
+            if (forces && it.x0 < fx1 && it.x0 + it.w > fx0 && it.y0 < fy1 && it.y0 + it.h > fy0) it.slot = slot++;
+            b->step_items.push_back(it);
+        }
+        be.mstep_count = (int)b->step_items.size() - be.mstep_begin;
         be.slot_count = slot - be.slot_begin;
         be.vox_begin = (int)b->vox_items.size();
         if (body && be.caps)
@@ -774,7 +982,10 @@ int32_t build_plan(hl_batch* b)
     std::vector<int> order((size_t)b->n);
     for (int e = 0; e < b->n; ++e) order[(size_t)e] = e;
     std::stable_sort(order.begin(), order.end(),
-                     [&](int p, int q) { return b->envs[(size_t)p].step_count > b->envs[(size_t)q].step_count; });
+                     [&](int p, int q) {
+                         const hl::BatchEnv &u = b->envs[(size_t)p], &v = b->envs[(size_t)q];
+                         return u.step_count + u.mstep_count > v.step_count + v.mstep_count;
+                     });
     std::vector<std::vector<int>> groups(8);
     int64_t load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int e : order) {
@@ -782,7 +993,7 @@ int32_t build_plan(hl_batch* b)
         for (int h = 1; h < 8; ++h)
             if (load[h] < load[g]) g = h;
         groups[(size_t)g].push_back(e);
-        load[g] += b->envs[(size_t)e].step_count;
+        load[g] += b->envs[(size_t)e].step_count + b->envs[(size_t)e].mstep_count;
     }
     std::vector<int> xitems, genv;
     for (int g = 0; g < 8; ++g) {
@@ -793,6 +1004,7 @@ int32_t build_plan(hl_batch* b)
             genv.push_back(e);
             const hl::BatchEnv& be = b->envs[(size_t)e];
             for (int t = 0; t < be.step_count; ++t) xitems.push_back(be.step_begin + t);
+            for (int t = 0; t < be.mstep_count; ++t) xitems.push_back(be.mstep_begin + t);
         }
     }
     b->plan.goff[8] = (int)xitems.size();
@@ -824,15 +1036,37 @@ int32_t enqueue(hl_batch* b, const uint8_t* active, int32_t nsteps, double* hist
     const int nvox = (int)b->vox_items.size(), nstep = (int)b->step_items.size();
     for (int k = 0; k < nsteps; ++k) {
         if (!per_env) {
-            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0);
-            hipLaunchKernelGGL(hl::k_bstep, dim3(nstep), dim3(hl::kThreads), 0, s, a, 0);
+            // the body strips (voxelizer, then the masked march) on the side
+            // stream beside the plain strips on s: they touch different cells,
+            // and the plain march reads no mask
+            const int nm = nstep - b->nplain;
+            const bool fork = nvox + nm > 0;
+            hipError_t fe = hipSuccess;
+            if (fork && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
+                         (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
+                return hip_report(fe, "batch stream fork");
+            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, b->side, a, 0);
+            if (nm > 0)
+                hipLaunchKernelGGL(hl::k_bstep<true>, dim3((nm + hl::kStepWaves - 1) / hl::kStepWaves),
+                                   dim3(64 * hl::kStepWaves), 0, b->side, a, b->nplain, nm);
+            if (b->nplain > 0)
+                hipLaunchKernelGGL(hl::k_bstep<false>, dim3((b->nplain + hl::kStepWaves - 1) / hl::kStepWaves),
+                                   dim3(64 * hl::kStepWaves), 0, s, a, 0, b->nplain);
+            if (fork && ((fe = hipEventRecord(b->ev_join, b->side)) != hipSuccess ||
+                         (fe = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess))
+                return hip_report(fe, "batch stream join");
             hipLaunchKernelGGL(hl::k_badvance, dim3(b->n), dim3(64), 0, s, a, 0, history, b->n, k);
         } else {
             // the same kernels, one environment per launch
             for (int e = 0; e < b->n; ++e) {
                 const hl::BatchEnv& be = b->envs[e];
                 if (be.vox_count) hipLaunchKernelGGL(hl::k_bvox, dim3(be.vox_count), dim3(256), 0, s, a, be.vox_begin);
-                hipLaunchKernelGGL(hl::k_bstep, dim3(be.step_count), dim3(hl::kThreads), 0, s, a, be.step_begin);
+                if (be.mstep_count)
+                    hipLaunchKernelGGL(hl::k_bstep<true>, dim3((be.mstep_count + hl::kStepWaves - 1) / hl::kStepWaves),
+                                       dim3(64 * hl::kStepWaves), 0, s, a, be.mstep_begin, be.mstep_count);
+                if (be.step_count)
+                    hipLaunchKernelGGL(hl::k_bstep<false>, dim3((be.step_count + hl::kStepWaves - 1) / hl::kStepWaves),
+                                       dim3(64 * hl::kStepWaves), 0, s, a, be.step_begin, be.step_count);
                 hipLaunchKernelGGL(hl::k_badvance, dim3(1), dim3(64), 0, s, a, e, history, b->n, k);
             }
         }
@@ -868,6 +1102,12 @@ int32_t hl_batch_create(const hl_env* envs, int32_t n, hl_batch** out)
                             ": batches hold D2Q9 FP32 lattices with no body-force field and at most 16 solids";
             return report(HL_E_UNSUPPORTED, m.c_str());
         }
+        // the step's buffer offsets are 32-bit with the out-of-range marker at 2^31
+        const int64_t st = x.stride > 0 ? x.stride : (int64_t)x.nx * x.ny;
+        if (st * 6 * 4 >= (int64_t)1 << 31) {
+            std::string m = "batch environment " + std::to_string(e) + ": fields of more than 2^31 bytes";
+            return report(HL_E_UNSUPPORTED, m.c_str());
+        }
     }
     hl_batch* b = new (std::nothrow) hl_batch();
     if (!b) return report(HL_E_NOMEM, "out of host memory");
@@ -882,6 +1122,12 @@ int32_t hl_batch_create(const hl_env* envs, int32_t n, hl_batch** out)
     rebuild_items(b);
     std::vector<int32_t> zeros(n, 0);
     hipError_t he;
+    if ((he = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
+        (he = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (he = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming)) != hipSuccess) {
+        hl_batch_destroy(b);
+        return hip_report(he, "batch stream");
+    }
     int32_t rc = sync_device(b);
     if (!rc && ((he = upload(&b->d_parity, &b->cap_parity, zeros.data(), zeros.size())) != hipSuccess ||
                 (he = upload(&b->d_time, &b->cap_time, zeros.data(), zeros.size())) != hipSuccess))
@@ -899,6 +1145,9 @@ void hl_batch_destroy(hl_batch* b)
     if (!b) return;
     drop_graph(b);
     if (b->cap_stream) hipStreamDestroy(b->cap_stream);
+    if (b->side) hipStreamDestroy(b->side);
+    if (b->ev_fork) hipEventDestroy(b->ev_fork);
+    if (b->ev_join) hipEventDestroy(b->ev_join);
     hipFree(b->d_envs);
     hipFree(b->d_step);
     hipFree(b->d_vox);
@@ -1040,17 +1289,17 @@ int32_t hl_batch_run(hl_batch* b, const uint8_t* active, int32_t nsteps, double*
     occ = std::min(occ, 2);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const hl::BatchArgs a = batch_args(b, active);
-    auto launch = [&](int i0, int ni, int e0, int ne) -> int32_t {
-        if (ni <= 0) return HL_OK;
-        // every workgroup resident (a cooperative launch); no more than the tiles
-        const int grid = std::max(1, std::min(occ * cus, ni));
+    auto launch = [&](int i0, int ni, int j0, int nj, int e0, int ne) -> int32_t {
+        if (ni + nj <= 0) return HL_OK;
+        // every workgroup resident (a cooperative launch); no more wavefronts than items
+        const int grid = std::max(1, std::min(occ * cus, (ni + nj + hl::kWaves - 1) / hl::kWaves));
         hipLaunchKernelGGL(hl::k_brun_reset, dim3(1), dim3(1), 0, s, b->d_bar, b->d_flag);
-        int i0v = i0, niv = ni, K = nsteps, ntot = b->n, e0v = e0, nev = ne, nsl = b->nslots;
+        int i0v = i0, niv = ni, j0v = j0, njv = nj, K = nsteps, ntot = b->n, e0v = e0, nev = ne, nsl = b->nslots;
         double* hist = force_history;
         unsigned* bar = b->d_bar;
         int* flag = b->d_flag;
         hl::BatchArgs av = a;
-        void* args[] = {&av, &i0v, &niv, &K, &hist, &ntot, &e0v, &nev, &nsl, &bar, &flag};
+        void* args[] = {&av, &i0v, &niv, &j0v, &njv, &K, &hist, &ntot, &e0v, &nev, &nsl, &bar, &flag};
         hipError_t le = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(hl::k_brun), dim3(grid),
                                                    dim3(hl::kThreads), args, 0, s);
         if (le != hipSuccess) return hip_report(le, "persistent batch launch");
@@ -1061,7 +1310,7 @@ int32_t hl_batch_run(hl_batch* b, const uint8_t* active, int32_t nsteps, double*
         // of workgroups to XCDs is the round-robin it assumes
         const int ni = (int)b->step_items.size();
         if (ni <= 0) return HL_OK;
-        const int g8 = std::max(8, std::min(occ * cus, ni) / 8 * 8);
+        const int g8 = std::max(8, std::min(occ * cus, (ni + hl::kWaves - 1) / hl::kWaves) / 8 * 8);
         if (g8 <= occ * cus) {
             if (b->plan_stale && (rc = build_plan(b)) != HL_OK) return rc;
             if (!b->d_bars && (e = hipMalloc(&b->d_bars, 8 * hl::kXcdBarStride * sizeof(unsigned))) != hipSuccess)
@@ -1103,10 +1352,11 @@ int32_t hl_batch_run(hl_batch* b, const uint8_t* active, int32_t nsteps, double*
                 return HL_OK;
             }
         }
-        return launch(0, ni, 0, b->n);
+        return launch(0, ni, 0, 0, 0, b->n);
     }
     for (int env = 0; env < b->n; ++env) {
-        rc = launch(b->envs[env].step_begin, b->envs[env].step_count, env, 1);
+        const hl::BatchEnv& be = b->envs[env];
+        rc = launch(be.step_begin, be.step_count, be.mstep_begin, be.mstep_count, env, 1);
         if (rc) return rc;
     }
     return HL_OK;

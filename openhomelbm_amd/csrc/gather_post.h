@@ -371,6 +371,35 @@ __device__ __forceinline__ void link_force(const EnvArgs& a, const T* mc, int x,
     }
 }
 
+// Cut link i of D2Q9 interior cell (x, y) with moments mc on solid sid: the
+// wall-rule population (d2q9_link_pop's cut branch, clamped) and, FORCE, its
+// momentum exchange Fx, Fy, Tz (link_force), the two sharing one wall
+// velocity. The expressions are theirs, so are the results.
+template <bool FORCE>
+__device__ __forceinline__ float d2q9_cut_link(const EnvArgs& a, const float* mc, int x, int y, int i, int sid,
+                                               const float* poses, float F[3])
+{
+    float up[2], r[2];
+    wall_velocity_2d<float>(poses, a.nsolid, sid, float(x) - 0.5f * float(D2CX[i]), float(y) - 0.5f * float(D2CY[i]),
+                            up, r);
+    const float Sxx_s = up[0] * up[0] + mc[3] - mc[1] * mc[1];
+    const float Syy_s = up[1] * up[1] + mc[4] - mc[2] * mc[2];
+    const float Sxy_s = up[0] * up[1] + mc[5] - mc[1] * mc[2];
+    float pop = d2q9_recon<float>(mc[0], up[0], up[1], Sxx_s, Syy_s, Sxy_s, i);
+    if (pop < 0.0f) pop = 0.0f;
+    if (FORCE) {
+        const int inv = D2INV[i];
+        const float cur_f = d2q9_recon<float>(mc[0], mc[1], mc[2], mc[3], mc[4], mc[5], inv);
+        const float Fx = cur_f * (float(D2CX[inv]) - up[0]) - pop * (float(D2CX[i]) - up[0]);
+        const float Fy = cur_f * (float(D2CY[inv]) - up[1]) - pop * (float(D2CY[i]) - up[1]);
+        const float grid_length = 1.0f / float(a.nx);
+        F[0] = Fx;
+        F[1] = Fy;
+        F[2] = (r[0] * Fy - r[1] * Fx) * grid_length;
+    }
+    return pop;
+}
+
 // Cell (x, y) of a D2Q9 gather step (x, y inside the lattice): a face cell
 // takes the update of its interior projection, then the face chain.
 template <typename T, int DT>
