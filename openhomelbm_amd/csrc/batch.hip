@@ -218,7 +218,7 @@ __device__ __forceinline__ bool strip_masked(const BatchEnv& be, const BatchItem
 // load lands in the buffer it is consumed from). VOX: the source rows' mask
 // values come from the capsule table at t (the strip's own cells written to
 // the mask) instead of the mask.
-template <bool VOX, bool MASKED>
+template <bool VOX, bool MASKED, int NW>
 __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, int par, int t, double* part)
 {
     const int lane = (int)(threadIdx.x & 63);
@@ -243,13 +243,13 @@ __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, i
     // MASKED: this wavefront's copies of the pose table at t and (VOX) the
     // capsule table at t in LDS, read by the wall rule, the forces and the
     // voxelizer (one round of loads instead of one round trip per link)
-    __shared__ float spose[kWaves][kBatchMaxSolids * 8];
-    __shared__ float scaps[kWaves][kRunCaps * 8];
+    __shared__ float spose[NW][kBatchMaxSolids * 8];
+    __shared__ float scaps[NW][kRunCaps * 8];
     // MASKED: a row's cut links listed across the wavefront (lane | link << 6
     // | (solid + 1) << 10), the cells' moments and the links' populations
-    __shared__ uint32_t clist[kWaves][8 * 64];
-    __shared__ float cmom[kWaves][6][64];
-    __shared__ float cpop[kWaves][9][64];
+    __shared__ uint32_t clist[NW][8 * 64];
+    __shared__ float cmom[NW][6][64];
+    __shared__ float cpop[NW][9][64];
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const float* poses = gposes;
     const float* caps = nullptr;
@@ -348,6 +348,41 @@ __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, i
     float Lf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, Rf[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
     int Ly = 0, Ry = 0, nl = 0, nr = 0;
 
+    // near the body: each lane's FP64 force sums in up to kSlots solid slots
+    // (solids in order of first appearance); a slot's sums go through a
+    // wavefront butterfly into acc when the slots run out and at the end
+    constexpr int kSlots = 4;
+    double fa0[kSlots], fa1[kSlots], fa2[kSlots];
+    uint32_t ssl = 0xffffffffu;  // the slots' solids, one byte each (0xff: free)
+    int nsl = 0;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) fa0[j] = fa1[j] = fa2[j] = 0.0;
+    auto flush = [&]() __attribute__((always_inline)) {
+#pragma unroll
+        for (int j = 0; j < kSlots; ++j)
+            if (j < nsl) {
+                const int sv = (int)((ssl >> (8 * j)) & 0xffu);
+                const double s0 = bwave_sum(fa0[j]), s1 = bwave_sum(fa1[j]), s2 = bwave_sum(fa2[j]);
+                const int q = lane - 3 * sv;
+                acc = q == 0 ? acc + s0 : q == 1 ? acc + s1 : q == 2 ? acc + s2 : acc;
+                fa0[j] = fa1[j] = fa2[j] = 0.0;
+            }
+        nsl = 0;
+        ssl = 0xffffffffu;
+    };
+    auto slot_of = [&](int sv) __attribute__((always_inline)) -> int {
+        int j = -1;
+#pragma unroll
+        for (int jj = kSlots - 1; jj >= 0; --jj)
+            if ((int)((ssl >> (8 * jj)) & 0xffu) == sv) j = jj;
+        if (j < 0) {
+            if (nsl == kSlots) flush();
+            j = nsl++;
+            ssl = (ssl & ~(0xffu << (8 * j))) | ((uint32_t)sv << (8 * j));
+        }
+        return j;
+    };
+
     auto dest_row = [&](int y, const float* Mn, int Kraw, float* mout) {
         float dn[3], mdn[3], uyn[3];
         send(Mn, dn, mdn, uyn);
@@ -385,8 +420,7 @@ __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, i
 #pragma unroll
                 for (int f = 0; f < 6; ++f) cmom[wave][f][lane] = My[f];
                 // each listed link by one lane: the wall-rule population and,
-                // near the body, the force (per solid: the lanes' links of a
-                // pass by a butterfly, the passes in order)
+                // near the body, the force into its solid's slot
                 for (uint32_t k0 = 0; k0 < K; k0 += 64) {
                     const uint32_t k = k0 + (uint32_t)lane;
                     float F[3] = {0.0f, 0.0f, 0.0f};
@@ -406,18 +440,22 @@ __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, i
                         if (sd >= nsolid) sd = -1;
                     }
                     if (near) {
-                        uint32_t uni = sd >= 0 ? 1u << sd : 0u;
-#pragma unroll
-                        for (int o = 32; o >= 1; o >>= 1) uni |= (uint32_t)__shfl_xor((int)uni, o, 64);
-                        while (uni) {
-                            const int sv = __builtin_ctz(uni);
-                            uni &= uni - 1;
+                        // this pass's solids in the order of their first lanes
+                        // (ballots, no shuffle chain), each lane's link into its
+                        // solid's slot
+                        uint64_t pend = __builtin_amdgcn_ballot_w64(sd >= 0);
+                        while (pend) {
+                            const int sv = __builtin_amdgcn_readlane(sd, (int)__builtin_ctzll(pend));
                             const bool mine = sd == sv;
-                            const double s0 = bwave_sum(mine ? (double)F[0] : 0.0);
-                            const double s1 = bwave_sum(mine ? (double)F[1] : 0.0);
-                            const double s2 = bwave_sum(mine ? (double)F[2] : 0.0);
-                            const int q = lane - 3 * sv;
-                            acc = q == 0 ? acc + s0 : q == 1 ? acc + s1 : q == 2 ? acc + s2 : acc;
+                            pend &= ~__builtin_amdgcn_ballot_w64(mine);
+                            const int j = slot_of(sv);
+#pragma unroll
+                            for (int jj = 0; jj < kSlots; ++jj)
+                                if (jj == j) {
+                                    fa0[jj] += mine ? (double)F[0] : 0.0;
+                                    fa1[jj] += mine ? (double)F[1] : 0.0;
+                                    fa2[jj] += mine ? (double)F[2] : 0.0;
+                                }
                         }
                     }
                 }
@@ -529,14 +567,15 @@ __device__ __forceinline__ void march(const BatchEnv& be, const BatchItem& it, i
         for (int f = 0; f < 6; ++f)
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, mf[f]), rd, (int)v, (int)(f * fbytes), 0);
     }
+    if (MASKED && near) flush();
     if (near && lane < kPartialWidth) as_global(part)[(int64_t)slot * kPartialWidth + lane] = acc;
 }
 
 template <bool VOX>
 __device__ __forceinline__ void march_strip(const BatchEnv& be, const BatchItem& it, int par, int t, double* part)
 {
-    if (strip_masked(be, it)) march<VOX, true>(be, it, par, t, part);
-    else march<VOX, false>(be, it, par, t, part);
+    if (strip_masked(be, it)) march<VOX, true, kWaves>(be, it, par, t, part);
+    else march<VOX, false, kWaves>(be, it, par, t, part);
 }
 
 // The step of the strips [item0, item0 + count), one per wavefront: the
@@ -550,7 +589,7 @@ __global__ void __launch_bounds__(64 * kStepWaves) k_bstep(BatchArgs b, int item
     const BatchItem it = b.step_items[item0 + i];
     const int e = uni(it.env);
     if (!env_active(b, e)) return;
-    march<false, MASKED>(b.envs[e], it, uni(b.parity[e]) & 1, uni(b.time[e]), b.partial);
+    march<false, MASKED, kStepWaves>(b.envs[e], it, uni(b.parity[e]) & 1, uni(b.time[e]), b.partial);
 }
 
 // Per environment (one block): the fixed-order sum of its partial slots into
