@@ -451,11 +451,12 @@ int32_t hl_batch_replay(hl_batch* b, void* stream);
 /* The persistent engine: nsteps steps of every active environment in one
  * cooperative launch (the reference's per-world loop of LBM_Solver.step,
  * lbm_solver.py:97-141, with all worlds in one grid and all steps in one
- * launch). The workgroups deal out every environment's step tiles, each
- * tile voxelizing the capsule table at its step's time as it stages its
- * cells (environments with at most 32 capsules); a grid barrier separates
- * the steps. Arguments, results and the force history as hl_batch_step, bit
- * for bit (the same tiles, cell arithmetic and summation order). per_env != 0:
+ * launch). The wavefronts deal out every environment's step strips, the
+ * strips at a body voxelizing the capsule table at their step's time as they
+ * load their rows (environments with at most 32 capsules); a barrier
+ * separates the steps. Arguments, results and the force history as
+ * hl_batch_step, bit for bit (the same strips, cell arithmetic and summation
+ * order). per_env != 0:
  * one launch (of nsteps steps) per environment. The barrier waits are
  * bounded; hl_batch_run_status reports (after a stream synchronize) whether
  * the last launch gave one up. Not capturable in a graph. */
@@ -463,7 +464,7 @@ int32_t hl_batch_run(hl_batch* b, const uint8_t* active, int32_t nsteps, double*
                      void* stream);
 int32_t hl_batch_run_status(hl_batch* b, int32_t* timed_out);
 
-/* Environments, step tiles and step tiles that reduce forces (any may be NULL). */
+/* Environments, step items (strips) and step items that reduce forces (any may be NULL). */
 int32_t hl_batch_size(const hl_batch* b, int32_t* nenv, int32_t* step_tiles, int32_t* force_tiles);
 
 #ifdef __cplusplus

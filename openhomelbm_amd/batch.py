@@ -4,7 +4,7 @@ together (hl_batch_* in include/homelbm.h).
 The reference batches worlds of one size along a leading nworld launch
 dimension (LBM_Solver, envs/lbm/lbm_solver.py:41-141). Here every
 environment is an independent Lattice with its own extents, and the HIP
-kernels walk (environment, tile) work-lists, so one launch per kernel kind
+kernels walk (environment, strip) work-lists, so one launch per kernel kind
 advances all of them. Prescribed kinematics come as per-step capsule and pose
 tables; the mask is re-voxelized on the device at the start of each step.
 """

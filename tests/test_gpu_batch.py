@@ -1,6 +1,6 @@
 """GPU parity of the batched D2Q9 engine (hl_batch_*).
 
-Environments of uneven sizes (none a multiple of the 64x4 step tile) carry
+Environments of uneven sizes (none a multiple of the 62-column step strip) carry
 8-capsule eels whose masks are re-voxelized on the device every step from
 per-step capsule and pose tables. Checks:
 

@@ -1,6 +1,7 @@
 """GPU parity of the persistent batched engine (hl_batch_run: every
-environment advanced K steps in one cooperative launch, the workgroups dealing
-out the step tiles, voxelization fused into staging, a grid barrier per step).
+environment advanced K steps in one cooperative launch, the wavefronts dealing
+out the step strips, voxelization fused into the body strips' row loads, a
+barrier per step).
 
 Checks, on the eel batch of test_gpu_batch (uneven sizes, moving 8-capsule
 eels re-voxelized every step):
