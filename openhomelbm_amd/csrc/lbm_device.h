@@ -82,20 +82,17 @@ template <> struct Storage<HL_F16> {
 };
 // bfloat16 as raw 16-bit words. key: the logical element index
 // f * ncell + x + nx (y + ny z) (rounding_key), the same in every kernel,
-// layout and tiling. The hash is a 32-bit integer mix (xor-shift
-// and odd multipliers) of key ^ value bits; its low 16 bits are the random
-// fraction added before truncation.
+// layout and tiling. The hash is a 32-bit integer mix of key ^ value bits
+// (an xor-shift, then an odd multiplier); the product's high 16 bits are the
+// random fraction added before truncation.
 __device__ __forceinline__ uint16_t f32_to_bf16_sr(float v, uint32_t key)
 {
     uint32_t u = __float_as_uint(v);
     if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)0x7fc0;  // NaN
     uint32_t h = key ^ u;
     h ^= h >> 16;
-    h *= 0x7feb352du;
-    h ^= h >> 15;
-    h *= 0x846ca68bu;
-    h ^= h >> 16;
-    u += h & 0xffffu;  // carries into the kept bits with probability fraction / 2^16
+    h *= 0x7feb352du;  // one odd multiplier; the product's high half is the well-mixed part
+    u += h >> 16;      // carries into the kept bits with probability fraction / 2^16
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
