@@ -8,18 +8,17 @@
 // graph and replayed.
 //
 // Per step, for all active environments:
+//   k_bvox      the body masks rebuilt in the body boxes from the capsule
+//               tables at each environment's time t
 //   k_bstep     the D2Q9 update in strips of up to 62 columns, one wavefront
 //               each marching up its rows (march): every source row's
 //               populations rebuilt once and passed between lanes, the
 //               reference's operation order, the mask wall rule. The strips
-//               that reach a body box run the masked march (k_bstep<true>):
-//               their mask values voxelized from the capsule table at the
-//               environment's time t (their own cells written to the mask),
-//               and those near the body reduce the per-solid momentum-exchange
-//               force of the state they read into an FP64 partial; the others
-//               the plain march (k_bstep<false>)
-//   k_bvox      only for an environment with more capsules than the masked
-//               march's LDS table: its mask rebuilt in the body box first
+//               that reach a body box run the masked march (k_bstep<true>,
+//               on the batch's side stream), and those near the body reduce
+//               the per-solid momentum-exchange force of the state they read
+//               into an FP64 partial; the others the plain march
+//               (k_bstep<false>), which reads no mask
 //   k_badvance  per environment: fixed-order sum of its strips' partials into
 //               the force history, then flip the parity and advance time.
 // The same kernels can be launched one environment at a time (the
@@ -1075,16 +1074,16 @@ int32_t enqueue(hl_batch* b, const uint8_t* active, int32_t nsteps, double* hist
     const int nvox = (int)b->vox_items.size(), nstep = (int)b->step_items.size();
     for (int k = 0; k < nsteps; ++k) {
         if (!per_env) {
-            // the body strips (voxelizer, then the masked march) on the side
+            // the voxelizer, then the body strips (masked march) on the side
             // stream beside the plain strips on s: they touch different cells,
-            // and the plain march reads no mask
+            // and the plain march reads no mask (both start together)
             const int nm = nstep - b->nplain;
             const bool fork = nvox + nm > 0;
             hipError_t fe = hipSuccess;
+            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0);
             if (fork && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
                          (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
                 return hip_report(fe, "batch stream fork");
-            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, b->side, a, 0);
             if (nm > 0)
                 hipLaunchKernelGGL(hl::k_bstep<true>, dim3((nm + hl::kStepWaves - 1) / hl::kStepWaves),
                                    dim3(64 * hl::kStepWaves), 0, b->side, a, b->nplain, nm);
