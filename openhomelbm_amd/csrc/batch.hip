@@ -19,8 +19,13 @@
 //               the per-solid momentum-exchange force of the state they read
 //               into an FP64 partial; the others the plain march
 //               (k_bstep<false>), which reads no mask
-//   k_badvance  per environment: fixed-order sum of its strips' partials into
-//               the force history, then flip the parity and advance time.
+//   k_badvance  (one environment per launch) per environment: fixed-order
+//               sum of its strips' partials into the force history, then
+//               flip the parity and advance time. The batched schedule runs
+//               step k of a call k steps past each environment's parity and
+//               time instead, sums step k's rows in step k + 1's plain-strip
+//               launch and closes the call with k_bclose (the last step's
+//               sums, parity and time advanced once).
 // The same kernels can be launched one environment at a time (the
 // per-environment baseline); every cell's arithmetic and every sum's order
 // are the same, so the two schedules give bitwise equal results.
@@ -141,13 +146,13 @@ __device__ __forceinline__ uint8_t capsule_voxel(P caps, int ncap, int x, int y)
 }
 
 // Capsule voxelizer over one item.
-__global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0)
+__global__ void __launch_bounds__(256) k_bvox(BatchArgs b, int item0, int k0)
 {
     const BatchItem it = b.vox_items[item0 + blockIdx.x];
     if (!env_active(b, it.env)) return;
     const BatchEnv& be = b.envs[it.env];
     if (!be.caps || be.T <= 0) return;
-    const float* caps = be.caps + (int64_t)(b.time[it.env] % be.T) * be.ncap * 8;
+    const float* caps = be.caps + (int64_t)((b.time[it.env] + k0) % be.T) * be.ncap * 8;
     const int k = threadIdx.x;
     if (k >= it.w * it.h) return;
     const int x = it.x0 + k % it.w, y = it.y0 + k / it.w;
@@ -580,15 +585,31 @@ __device__ __forceinline__ void march_strip(const BatchEnv& be, const BatchItem&
 // The step of the strips [item0, item0 + count), one per wavefront: the
 // strips that reach a body box (MASKED) and the others are listed apart, so
 // the plain strips' kernel keeps the registers of the plain march only.
+__device__ __forceinline__ void sum_output(const BatchArgs& b, const double* part, int k, double* history,
+                                           int nenv_total, int e, int r);
+
+// Step k of a call (parity and time k steps past the environment's), its
+// force rows into bank k & 1 of nslots rows. The blocks past the strips'
+// (sums > 0) sum step k - 1's rows into the history, one output per thread.
 template <bool MASKED>
-__global__ void __launch_bounds__(64 * kStepWaves) k_bstep(BatchArgs b, int item0, int count)
+__global__ void __launch_bounds__(64 * kStepWaves) k_bstep(BatchArgs b, int item0, int count, int k, int nslots,
+                                                            double* history, int nenv_total)
 {
+    const int nb = (count + kStepWaves - 1) / kStepWaves;
+    if ((int)blockIdx.x >= nb) {
+        const int o = ((int)blockIdx.x - nb) * 64 * kStepWaves + (int)threadIdx.x;
+        if (o < nenv_total * kBatchMaxSolids * 6)
+            sum_output(b, b.partial + (size_t)((k - 1) & 1) * nslots * kPartialWidth, k - 1, history, nenv_total,
+                       o / (kBatchMaxSolids * 6), o % (kBatchMaxSolids * 6));
+        return;
+    }
     const int i = (int)blockIdx.x * kStepWaves + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     if (i >= count) return;
     const BatchItem it = b.step_items[item0 + i];
     const int e = uni(it.env);
     if (!env_active(b, e)) return;
-    march<false, MASKED, kStepWaves>(b.envs[e], it, uni(b.parity[e]) & 1, uni(b.time[e]), b.partial);
+    march<false, MASKED, kStepWaves>(b.envs[e], it, (uni(b.parity[e]) + k) & 1, uni(b.time[e]) + k,
+                                     b.partial + (size_t)(k & 1) * nslots * kPartialWidth);
 }
 
 // Per environment (one block): the fixed-order sum of its partial slots into
@@ -627,6 +648,22 @@ __global__ void __launch_bounds__(64) k_badvance(BatchArgs b, int env0, double* 
     if (q == 0) {
         b.parity[e] ^= 1;
         b.time[e] += 1;
+    }
+}
+
+// The end of a call of K steps (environments [0, nenv_total), one thread per
+// output): step K - 1's force rows summed into the history as k_badvance
+// sums them, then parity and time advanced by K.
+__global__ void __launch_bounds__(64 * kStepWaves) k_bclose(BatchArgs b, int K, int nslots, double* history,
+                                                            int nenv_total)
+{
+    const int o = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (history && o < nenv_total * kBatchMaxSolids * 6)
+        sum_output(b, b.partial + (size_t)((K - 1) & 1) * nslots * kPartialWidth, K - 1, history, nenv_total,
+                   o / (kBatchMaxSolids * 6), o % (kBatchMaxSolids * 6));
+    if (o < nenv_total && env_active(b, o)) {
+        b.parity[o] = (b.parity[o] + K) & 1;
+        b.time[o] += K;
     }
 }
 
@@ -1072,45 +1109,55 @@ int32_t enqueue(hl_batch* b, const uint8_t* active, int32_t nsteps, double* hist
 {
     const hl::BatchArgs a = batch_args(b, active);
     const int nvox = (int)b->vox_items.size(), nstep = (int)b->step_items.size();
-    for (int k = 0; k < nsteps; ++k) {
-        if (!per_env) {
+    const int W = hl::kStepWaves, T = 64 * hl::kStepWaves;
+    const int nout = b->n * hl::kBatchMaxSolids * 6, sum_blocks = (nout + T - 1) / T;
+    if (!per_env) {
+        // Step k of the call runs k steps past each environment's parity and
+        // time (advanced once, by the closing kernel); its force rows go to
+        // bank k & 1 and are summed by step k + 1's plain-strip launch (the
+        // last step's by the closing kernel), so a step is three launches.
+        for (int k = 0; k < nsteps; ++k) {
             // the voxelizer, then the body strips (masked march) on the side
             // stream beside the plain strips on s: they touch different cells,
             // and the plain march reads no mask (both start together)
             const int nm = nstep - b->nplain;
-            const bool fork = nvox + nm > 0;
             hipError_t fe = hipSuccess;
-            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0);
-            if (fork && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
-                         (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
+            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0, k);
+            if (nm > 0 && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
+                           (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
                 return hip_report(fe, "batch stream fork");
             if (nm > 0)
-                hipLaunchKernelGGL(hl::k_bstep<true>, dim3((nm + hl::kStepWaves - 1) / hl::kStepWaves),
-                                   dim3(64 * hl::kStepWaves), 0, b->side, a, b->nplain, nm);
-            if (b->nplain > 0)
-                hipLaunchKernelGGL(hl::k_bstep<false>, dim3((b->nplain + hl::kStepWaves - 1) / hl::kStepWaves),
-                                   dim3(64 * hl::kStepWaves), 0, s, a, 0, b->nplain);
-            if (fork && ((fe = hipEventRecord(b->ev_join, b->side)) != hipSuccess ||
-                         (fe = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess))
+                hipLaunchKernelGGL(hl::k_bstep<true>, dim3((nm + W - 1) / W), dim3(T), 0, b->side, a, b->nplain, nm, k,
+                                   b->nslots, history, b->n);
+            const int extra = (k > 0 && history) ? sum_blocks : 0;
+            if (b->nplain + extra > 0)
+                hipLaunchKernelGGL(hl::k_bstep<false>, dim3((b->nplain + W - 1) / W + extra), dim3(T), 0, s, a, 0,
+                                   b->nplain, k, b->nslots, history, b->n);
+            if (nm > 0 && ((fe = hipEventRecord(b->ev_join, b->side)) != hipSuccess ||
+                           (fe = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess))
                 return hip_report(fe, "batch stream join");
-            hipLaunchKernelGGL(hl::k_badvance, dim3(b->n), dim3(64), 0, s, a, 0, history, b->n, k);
-        } else {
-            // the same kernels, one environment per launch
+        }
+        if (nsteps > 0)
+            hipLaunchKernelGGL(hl::k_bclose, dim3(std::max((std::max(nout, b->n) + T - 1) / T, 1)), dim3(T), 0, s, a,
+                               (int)nsteps, b->nslots, history, b->n);
+    } else {
+        // the same kernels, one environment per launch, each step closed by
+        // k_badvance (parity and time advanced per step)
+        for (int k = 0; k < nsteps; ++k)
             for (int e = 0; e < b->n; ++e) {
                 const hl::BatchEnv& be = b->envs[e];
-                if (be.vox_count) hipLaunchKernelGGL(hl::k_bvox, dim3(be.vox_count), dim3(256), 0, s, a, be.vox_begin);
+                if (be.vox_count) hipLaunchKernelGGL(hl::k_bvox, dim3(be.vox_count), dim3(256), 0, s, a, be.vox_begin, 0);
                 if (be.mstep_count)
-                    hipLaunchKernelGGL(hl::k_bstep<true>, dim3((be.mstep_count + hl::kStepWaves - 1) / hl::kStepWaves),
-                                       dim3(64 * hl::kStepWaves), 0, s, a, be.mstep_begin, be.mstep_count);
+                    hipLaunchKernelGGL(hl::k_bstep<true>, dim3((be.mstep_count + W - 1) / W), dim3(T), 0, s, a,
+                                       be.mstep_begin, be.mstep_count, 0, b->nslots, history, b->n);
                 if (be.step_count)
-                    hipLaunchKernelGGL(hl::k_bstep<false>, dim3((be.step_count + hl::kStepWaves - 1) / hl::kStepWaves),
-                                       dim3(64 * hl::kStepWaves), 0, s, a, be.step_begin, be.step_count);
+                    hipLaunchKernelGGL(hl::k_bstep<false>, dim3((be.step_count + W - 1) / W), dim3(T), 0, s, a,
+                                       be.step_begin, be.step_count, 0, b->nslots, history, b->n);
                 hipLaunchKernelGGL(hl::k_badvance, dim3(1), dim3(64), 0, s, a, e, history, b->n, k);
             }
-        }
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return hip_report(e, "batch kernel launch");
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_report(e, "batch kernel launch");
     return HL_OK;
 }
 
