@@ -908,7 +908,7 @@ struct hl_batch {
     // the body strips' side of a step (voxelizer, masked march), forked from
     // and joined back to the caller's stream
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_plain = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };
@@ -1116,25 +1116,35 @@ int32_t enqueue(hl_batch* b, const uint8_t* active, int32_t nsteps, double* hist
         // time (advanced once, by the closing kernel); its force rows go to
         // bank k & 1 and are summed by step k + 1's plain-strip launch (the
         // last step's by the closing kernel), so a step is three launches.
+        // The side stream carries the voxelizer and the body strips (masked
+        // march), s the plain strips: they touch different cells and the
+        // plain march reads no mask. Step k + 1's voxelizer follows step k's
+        // body strips (the last mask readers) while the plain strips still
+        // run; each kind of strip of step k + 1 waits for the other kind of
+        // step k (the pulls cross strip borders).
+        const int nm = nstep - b->nplain;
+        hipError_t fe = hipSuccess;
+        const bool side = nm > 0 || nvox > 0;
+        if (side && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
+                     (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
+            return hip_report(fe, "batch stream fork");
         for (int k = 0; k < nsteps; ++k) {
-            // the voxelizer, then the body strips (masked march) on the side
-            // stream beside the plain strips on s: they touch different cells,
-            // and the plain march reads no mask (both start together)
-            const int nm = nstep - b->nplain;
-            hipError_t fe = hipSuccess;
-            if (nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, s, a, 0, k);
-            if (nm > 0 && ((fe = hipEventRecord(b->ev_fork, s)) != hipSuccess ||
-                           (fe = hipStreamWaitEvent(b->side, b->ev_fork, 0)) != hipSuccess))
-                return hip_report(fe, "batch stream fork");
+            if (k == 0 && nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, b->side, a, 0, 0);
             if (nm > 0)
                 hipLaunchKernelGGL(hl::k_bstep<true>, dim3((nm + W - 1) / W), dim3(T), 0, b->side, a, b->nplain, nm, k,
                                    b->nslots, history, b->n);
+            if (side && (fe = hipEventRecord(b->ev_join, b->side)) != hipSuccess)  // step k's body strips
+                return hip_report(fe, "batch stream event");
+            if (k + 1 < nsteps && nvox) hipLaunchKernelGGL(hl::k_bvox, dim3(nvox), dim3(256), 0, b->side, a, 0, k + 1);
             const int extra = (k > 0 && history) ? sum_blocks : 0;
             if (b->nplain + extra > 0)
                 hipLaunchKernelGGL(hl::k_bstep<false>, dim3((b->nplain + W - 1) / W + extra), dim3(T), 0, s, a, 0,
                                    b->nplain, k, b->nslots, history, b->n);
-            if (nm > 0 && ((fe = hipEventRecord(b->ev_join, b->side)) != hipSuccess ||
-                           (fe = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess))
+            if (side && k + 1 < nsteps &&
+                ((fe = hipEventRecord(b->ev_plain, s)) != hipSuccess ||
+                 (fe = hipStreamWaitEvent(b->side, b->ev_plain, 0)) != hipSuccess))
+                return hip_report(fe, "batch stream event");
+            if (side && (fe = hipStreamWaitEvent(s, b->ev_join, 0)) != hipSuccess)
                 return hip_report(fe, "batch stream join");
         }
         if (nsteps > 0)
@@ -1209,7 +1219,8 @@ int32_t hl_batch_create(const hl_env* envs, int32_t n, hl_batch** out)
     hipError_t he;
     if ((he = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking)) != hipSuccess ||
         (he = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-        (he = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming)) != hipSuccess) {
+        (he = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming)) != hipSuccess ||
+        (he = hipEventCreateWithFlags(&b->ev_plain, hipEventDisableTiming)) != hipSuccess) {
         hl_batch_destroy(b);
         return hip_report(he, "batch stream");
     }
@@ -1233,6 +1244,7 @@ void hl_batch_destroy(hl_batch* b)
     if (b->side) hipStreamDestroy(b->side);
     if (b->ev_fork) hipEventDestroy(b->ev_fork);
     if (b->ev_join) hipEventDestroy(b->ev_join);
+    if (b->ev_plain) hipEventDestroy(b->ev_plain);
     hipFree(b->d_envs);
     hipFree(b->d_step);
     hipFree(b->d_vox);
