@@ -430,6 +430,14 @@ void hl_batch_destroy(hl_batch* b);
 int32_t hl_batch_set_bodies(hl_batch* b, int32_t env, const uint8_t* mask, const int32_t box[6], const float* caps,
                             int32_t ncap, const float* poses, int32_t nposes);
 
+/* Strip heights of the batched step: rows (default 16) away from the body
+ * boxes, body_rows (default 4) where a column strip crosses one; both in
+ * [3, 64]. No counterpart in the reference (its worlds share one grid): a
+ * tuning knob for uneven batches. States do not depend on it; the force
+ * history's per-strip partial sums are grouped by it, so it agrees to
+ * summation-order rounding. Invalidates a captured graph. */
+int32_t hl_batch_set_strips(hl_batch* b, int32_t rows, int32_t body_rows);
+
 /* Host copies of every environment's buffer parity (0/1) and time index. */
 int32_t hl_batch_set_state(hl_batch* b, const int32_t* parity, const int32_t* time);
 int32_t hl_batch_get_state(hl_batch* b, int32_t* parity, int32_t* time);

@@ -18,7 +18,7 @@ HL_MAX_SOLIDS = 64
 #: every entry point declared in include/homelbm.h
 EXPORTS = ("hl_version", "hl_env_bytes", "hl_last_error", "hl_step", "hl_forces", "hl_init_uniform",
            "hl_voxelize", "hl_voxelize_polygons", "hl_mass", "hl_set_tiling", "hl_get_tiling",
-           "hl_batch_create", "hl_batch_destroy", "hl_batch_set_bodies", "hl_batch_set_state",
+           "hl_batch_create", "hl_batch_destroy", "hl_batch_set_bodies", "hl_batch_set_state", "hl_batch_set_strips",
            "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
            "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
            "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
@@ -121,6 +121,7 @@ def load():
     L.hl_batch_destroy.restype = None
     L.hl_batch_set_bodies.argtypes = [P, I32, P, PI32, P, I32, P, I32]
     L.hl_batch_set_state.argtypes = [P, PI32, PI32]
+    L.hl_batch_set_strips.argtypes = [P, I32, I32]
     L.hl_batch_get_state.argtypes = [P, PI32, PI32]
     L.hl_batch_step.argtypes = [P, P, I32, P, I32, P]
     L.hl_batch_capture.argtypes = [P, P, I32, P]
@@ -148,7 +149,7 @@ def load():
     L.hl_chain_step.argtypes = [P, I32, P, P, P, P, I32, P, P, P, P, P]
     for name in ("hl_step", "hl_forces", "hl_init_uniform", "hl_voxelize", "hl_voxelize_polygons", "hl_mass",
                  "hl_set_tiling", "hl_get_tiling", "hl_batch_create", "hl_batch_set_bodies", "hl_batch_set_state",
-                 "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
+                 "hl_batch_set_strips", "hl_batch_get_state", "hl_batch_step", "hl_batch_capture", "hl_batch_replay", "hl_batch_size",
                  "hl_render_plane", "hl_render_projection", "hl_couple_poses", "hl_couple_forces",
                  "hl_free_bodies_step", "hl_body_capsules", "hl_voxelize_mesh", "hl_step_polygons",
                  "hl_step_polygons_worlds", "hl_kernel_timer", "hl_kernel_timer_read", "hl_copy_state",

@@ -1,4 +1,5 @@
-"""Autotuner for the tiled D3Q27 kernel's launch shape.
+"""Autotuners: the tiled D3Q27 kernel's launch shape, and the batched
+D2Q9 step's strip heights (tune_batch).
 
 The tiled kernel's tile (TX x TY columns staged in LDS) and z-chunk (ZC
 planes marched per workgroup) trade halo re-reads, LDS footprint (hence
@@ -142,6 +143,49 @@ def tune(lat, steps: int = 20, warmup: int = 3, shapes: Optional[Iterable[Tuple[
     if save:
         save_table({size_class(lat.lattice, lat.nx, lat.ny, lat.nz, lat.layout): entry}, path)
     return entry
+
+
+#: strip heights (rows, body_rows) the batch tuner tries
+BATCH_STRIPS = ((16, 4), (12, 4), (24, 4), (16, 3), (16, 6), (8, 4), (32, 4))
+
+
+def tune_batch(batch, steps: int = 20, reps: int = 3, shapes: Optional[Iterable[Tuple[int, int]]] = None) -> dict:
+    """Time the batched step's strip heights (Batch.set_strips) on this batch
+    by replays of a captured graph of `steps` steps, keep the fastest and
+    return {"strips": best, "ms_per_step": {shape: ms}}. The batch's states,
+    parity and time are restored afterwards; states do not depend on the
+    heights (hl_batch_set_strips), so tuning changes speed only."""
+    import torch
+
+    lats = batch.lattices
+    saved = [(L.fields(0).clone(), L.fields(1).clone()) for L in lats]
+    par, tim = batch.pull_state()
+    timings = {}
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for shape in (shapes or BATCH_STRIPS):
+        batch.set_strips(*shape)
+        batch.capture(steps)
+        batch.replay()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            e0.record()
+            batch.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            best = min(best, e0.elapsed_time(e1) / steps)
+        timings[tuple(shape)] = round(best, 5)
+    win = min(timings, key=timings.get)
+    batch.set_strips(*win)
+    for L, (a, b) in zip(lats, saved):
+        L.fields(0).copy_(a)
+        L.fields(1).copy_(b)
+    for L, p in zip(lats, par):
+        L.cur = p
+    batch.time = list(tim)
+    batch._push_state()
+    torch.cuda.synchronize()
+    return {"strips": win, "ms_per_step": {f"{r}x{h}": v for (r, h), v in timings.items()}}
 
 
 def main(argv=None) -> None:

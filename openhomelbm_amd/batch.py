@@ -73,6 +73,14 @@ class Batch:
         self.time[i] = int(t)
         self._push_state()
 
+    def set_strips(self, rows: int = 16, body_rows: int = 4) -> None:
+        """Strip heights of the batched step (hl_batch_set_strips): rows away
+        from the body boxes, body_rows where a column strip crosses one, both
+        in [3, 64]. States do not depend on them; the force history agrees to
+        summation-order rounding."""
+        N.check(self.lib.hl_batch_set_strips(self._h, int(rows), int(body_rows)), "hl_batch_set_strips")
+        self.strips = (int(rows), int(body_rows))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self.lib.hl_batch_destroy(self._h)

@@ -56,7 +56,7 @@ constexpr int kBatchMaxSolids = 16;
 // step item: a strip of up to kSW columns and kSH rows, one wavefront; lane l
 // holds column x0 - 1 + l, so lanes 1..w are the strip's cells and lanes 0
 // and w + 1 its halo columns
-constexpr int kSW = 62, kSH = 16, kSHB = 4;
+constexpr int kSW = 62, kSH = 16, kSHB = 4;  // default strip heights: kSH rows, kSHB at a body
 constexpr int kStepWaves = 4;                  // k_bstep: one step item per wavefront
 constexpr int kThreads = 512, kWaves = kThreads / 64;  // persistent engine workgroup
 constexpr int kVoxW = 16;         // voxelizer item: 16 x 16 cells
@@ -885,6 +885,7 @@ struct hl_batch {
     std::vector<hl::BatchItem> step_items, vox_items;  // step items: every plain strip, then the masked ones
     int nplain = 0;
     int nslots = 0;
+    int rows = hl::kSH, body_rows = hl::kSHB;  // strip heights (hl_batch_set_strips)
     unsigned* d_bar = nullptr;  // grid barrier counter of the persistent engine
     int* d_flag = nullptr;      // set when a barrier wait gave up
     hl::BatchEnv* d_envs = nullptr;
@@ -952,8 +953,8 @@ bool strip_reaches_box(const hl::EnvArgs& a, const hl::BatchItem& it)
 
 // Step items (strips) cover every environment: columns in strips of
 // near-equal widths (at most kSW, so at least 31 columns, or the whole width
-// of at least 3), rows in strips of at most kSH rows, and of at most kSHB rows
-// where a column strip crosses the body box (the wall rule and the forces of
+// of at least 3), rows in strips of at most `rows` rows, and of at most
+// `body_rows` where a column strip crosses the body box (the wall rule and the forces of
 // those rows are then shared by more wavefronts); the face rows' strips hold
 // at least 2 rows, so every face cell's projection lies in its strip. Each
 // environment lists its plain strips, then the strips that reach the box
@@ -985,7 +986,7 @@ void rebuild_items(hl_batch* b)
             for (size_t k = 0; k + 1 < cuts.size(); ++k) {
                 const int s0 = cuts[k], s1 = cuts[k + 1];
                 const bool inner = xb && s0 < c1 && s1 > c0;
-                const int hmax = inner ? hl::kSHB : hl::kSH, n = (s1 - s0 + hmax - 1) / hmax;
+                const int hmax = inner ? b->body_rows : b->rows, n = (s1 - s0 + hmax - 1) / hmax;
                 for (int j = 0; j < n; ++j) {
                     const int y = s0 + (int)((int64_t)(s1 - s0) * j / n);
                     const int yh = s0 + (int)((int64_t)(s1 - s0) * (j + 1) / n) - y;
@@ -1282,6 +1283,20 @@ int32_t hl_batch_set_bodies(hl_batch* b, int32_t env, const uint8_t* mask, const
     be.ncap = ncap;
     be.poses = poses;
     be.T = nposes;
+    drop_graph(b);
+    rebuild_items(b);
+    return sync_device(b);
+}
+
+int32_t hl_batch_set_strips(hl_batch* b, int32_t rows, int32_t body_rows)
+{
+    if (!b) return report(HL_E_INVALID, "batch is NULL");
+    // at least 3 rows, so every face-row strip keeps its projection row; at
+    // most 64, the lanes that collect a strip's face-column cells
+    if (rows < 3 || rows > 64 || body_rows < 3 || body_rows > 64)
+        return report(HL_E_INVALID, "strip heights must be in [3, 64]");
+    b->rows = rows;
+    b->body_rows = body_rows;
     drop_graph(b);
     rebuild_items(b);
     return sync_device(b);

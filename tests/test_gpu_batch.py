@@ -197,3 +197,32 @@ def test_batched_drag_within_one_percent_of_fp64():
     print(r)
     assert r["mean_drag_rel_err_max"] < 0.01
     assert r["batch_mean_drag_rel_err"] < 0.01
+
+
+def test_strip_heights_change_speed_only():
+    """The same batch stepped with other strip heights (hl_batch_set_strips):
+    states bitwise equal, force histories to summation-order rounding; the
+    batch tuner leaves the state where it found it."""
+    from openhomelbm_amd import autotune
+    nsteps = 5
+    _, lats_a, ba, _ = _setup()
+    _, lats_b, bb, _ = _setup()
+    bb.set_strips(8, 3)
+    ha, hb = ba.history_buffer(nsteps), bb.history_buffer(nsteps)
+    ba.step(nsteps, force_history=ha)
+    bb.step(nsteps, force_history=hb)
+    torch.cuda.synchronize()
+    for La, Lb in zip(lats_a, lats_b):
+        assert torch.equal(La.fields(), Lb.fields())
+    scale = ha.abs().max().item()
+    assert scale > 0
+    torch.testing.assert_close(hb, ha, rtol=1e-10, atol=1e-12 * scale)
+    before = [L.fields().clone() for L in lats_b]
+    state = bb.pull_state()
+    r = autotune.tune_batch(bb, steps=3, reps=1, shapes=((16, 4), (8, 3)))
+    assert set(r["ms_per_step"]) == {"16x4", "8x3"} and tuple(r["strips"]) in ((16, 4), (8, 3))
+    assert bb.pull_state() == state
+    for L, f in zip(lats_b, before):
+        assert torch.equal(L.fields(), f)
+    with pytest.raises(Exception):
+        bb.set_strips(2, 4)
