@@ -62,6 +62,25 @@ def quat_to_matrix(q) -> np.ndarray:
                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
 
 
+
+def first_step_last_pose(now_row: np.ndarray, linear_v, angle_v) -> np.ndarray:
+    """Previous-step pose row (p, R) of a body whose transforms are not
+    initialized yet, chosen so that the wall velocity the kernels take from
+    the pose pair (the body point's position now minus the same body-frame
+    point's position a step earlier) is the reference's fallback
+    u_p = linear_v + angle_v x (h - p) (lbm_func_3d.py:459-474):
+    p_last = p - v and R_last = R - [w]x R, so R q + p - (R_last q + p_last)
+    = w x (R q) + v with R q = h - p. With zero velocities the body is at rest
+    (R_last = R, p_last = p)."""
+    row = np.array(now_row, np.float64, copy=True)
+    v = np.asarray(linear_v, np.float64).reshape(3)
+    w = np.asarray(angle_v, np.float64).reshape(3)
+    R = row[3:12].reshape(3, 3)
+    W = np.array([[0.0, -w[2], w[1]], [w[2], 0.0, -w[0]], [-w[1], w[0], 0.0]])
+    row[0:3] = row[0:3] - v
+    row[3:12] = (R - W @ R).reshape(9)
+    return row
+
 class HomeFlow3D:
     """D3Q27 state of one world (envs/lbm3d/lbm_core_3d.py:45-227)."""
 
@@ -103,6 +122,10 @@ class HomeFlow3D:
             "MoI": HostArray(np.zeros(n)),
             "mesh_scale_sizes": HostArray(np.ones((n, 3))),
             "mesh_transforms_initialized": HostArray(np.zeros(n, np.int32), dtype=np.int32),
+            # lattice units per step: the wall velocity of a body whose
+            # transforms are not initialized yet (lbm_core_3d.py:105-106, 202-203)
+            "linear_v": HostArray(np.zeros((n, 3))),
+            "angle_v": HostArray(np.zeros((n, 3))),
         }
         self._shapes: Dict[int, Tuple[np.ndarray, np.ndarray, float]] = {}  # body-frame capsules (lattice units)
         self._meshes: Dict[int, Tuple[np.ndarray, np.ndarray]] = {}  # body-frame meshes (lattice units)
@@ -269,9 +292,10 @@ class HomeFlow3D:
         now = self._pose_table()
         init = self._solid["mesh_transforms_initialized"].numpy()
         last = now.copy() if self._last is None else self._last.copy()
+        lv, av = self._solid["linear_v"].numpy(), self._solid["angle_v"].numpy()
         for k in range(self.n_objects):
             if not init[k]:
-                last[k] = now[k]
+                last[k] = first_step_last_pose(now[k], lv[k], av[k])
         init[: self.n_objects] = 1
         self._solid["mesh_transforms_initialized"]._a[...] = init
         table = np.concatenate([now, last], axis=1)[: self.n_objects].astype(np.float32)

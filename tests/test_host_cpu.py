@@ -79,3 +79,30 @@ def test_host_array_behaves_like_warp_array():
     a.zero_()
     assert not a.numpy().any()
     assert np.asarray(a).shape == (3, 2)
+
+
+def test_first_step_wall_velocity_is_the_reference_fallback():
+    """A 3D body whose transforms are not initialized: the pose pair built by
+    first_step_last_pose, read the way the kernels read it (wall_velocity_3d:
+    q = R^T (h - p), then R q + p minus R_last q + p_last), gives the
+    reference's u_p = linear_v + angle_v x (h - p) (lbm_func_3d.py:464-474)
+    for random poses, velocities and wall points; zero velocities give a body
+    at rest."""
+    import numpy as np
+    from openhomelbm_amd.lbm3d import first_step_last_pose, quat_to_matrix
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        q = rng.standard_normal(4)
+        q /= np.linalg.norm(q)
+        R = quat_to_matrix(q)
+        p = rng.uniform(10, 50, 3)
+        row = np.concatenate([p, R.reshape(9)])
+        v, w = rng.uniform(-0.05, 0.05, 3), rng.uniform(-0.01, 0.01, 3)
+        last = first_step_last_pose(row, v, w)
+        h = p + rng.uniform(-8, 8, 3)
+        M, ML = row[3:12].reshape(3, 3), last[3:12].reshape(3, 3)
+        qb = M.T @ (h - p)
+        up = (M @ qb + p) - (ML @ qb + last[0:3])
+        np.testing.assert_allclose(up, v + np.cross(w, h - p), rtol=0, atol=1e-12)
+        rest = first_step_last_pose(row, np.zeros(3), np.zeros(3))
+        assert np.array_equal(rest, row)
